@@ -1,0 +1,113 @@
+/*
+ * sfeng.h -- C-ABI of the MI355X (gfx950) ERI and J/K (Fock-build) engine.
+ *
+ * Conventions
+ *   - Every entry point returns an int status: 0 = OK, 1 = bad argument,
+ *     2 = HIP runtime error, 3 = out of memory, 4 = wrong call order,
+ *     5 = unsupported size / angular momentum.  sf_last_error() gives the
+ *     calling thread's last message.  No exception or abort crosses the ABI.
+ *   - Buffers are caller-owned.  "device" pointers are HIP device memory
+ *     (e.g. torch.cuda tensors' data_ptr()); "host" pointers are host memory.
+ *   - `stream` is a hipStream_t (NULL = the default stream).
+ *   - Functions are numbered (ij|kl) in chemists' notation, in the AO order
+ *     of the reference (src/hf/molecule.py:138-165): atom by atom, shell by
+ *     shell, Cartesian components s; x,y,z; xx,xy,xz,yy,yz,zz.
+ *   - Each Cartesian component is normalised on its own
+ *     (molecule.py:210-226); see sf_basis_create for how that is passed.
+ */
+#ifndef SFENG_H
+#define SFENG_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct sf_basis sf_basis;
+
+/* Last error message of the calling thread (empty string if none). */
+const char *sf_last_error(void);
+/* Library version, major*10000 + minor*100 + patch. */
+int sf_version(void);
+
+/* ---- Boys function and Rys quadrature (device buffers) ----------------
+ * Replace mmd.boys (src/hf/core/core.f90:23-28, GSL 1F1) and rys.root5
+ * (core.f90:232-812) in batched form.
+ * sf_boys: F[i*(mmax+1)+m] = F_m(T[i]), 0 <= mmax <= 24.
+ * sf_rys_roots: nodes as t^2 (the convention of root5's output,
+ * core.f90:810-811), ascending, and weights; 1 <= nroots <= 5. */
+int sf_boys(long long n, int mmax, const double *T, double *F, void *stream);
+int sf_rys_roots(long long n, int nroots, const double *X, double *u, double *w,
+                 void *stream);
+
+/* ---- Contracted Cartesian-component quartets (host buffers) ------------
+ * sf_eri_rys: one quartet with the argument meaning of rys.eri_rys
+ * (core.f90:967-1002; called at src/hf/hf.py:26-30 and :100-104):
+ * for function k (0..3), npgf[k] primitives stored back to back in coefs,
+ * expns and norms (norms = the reference's normalize_factors), shells[3k..]
+ * = (lx,ly,lz), centers[3k..] in bohr.  Total angular momentum <= 9.
+ * sf_eri_components: nq quartets over a table of nf such functions
+ * (primitives of function f at [poff[f], poff[f]+npgf[f])); the batched
+ * form of the Pool.map over _build_eri (hf.py:18-47). */
+int sf_eri_rys(const int npgf[4], const double *coefs, const double *expns,
+               const double *norms, const int shells[12], const double centers[12],
+               double *out);
+int sf_eri_components(int nf, const int *npgf, const int *poff, const double *coefs,
+                      const double *expns, const double *norms, const int *shell,
+                      const double *center, long long nq, const int *quartets,
+                      double *out);
+
+/* ---- Shell basis handle -------------------------------------------------
+ * nshell contracted shells (L <= 2) in AO order.  Shell s: angular momentum
+ * L[s], nprim[s] primitives (stored back to back in expns/coefs), centre
+ * center[3s..3s+2] (bohr), atom shell_atom[s].  coefs carry the contraction
+ * coefficient times sqrt(2^(2L+1.5) a^(L+1.5) / pi^1.5), i.e. the
+ * reference's normalize_factors*coefs of the x^L component times
+ * sqrt((2L-1)!!); the library applies 1/sqrt((2lx-1)!!(2ly-1)!!(2lz-1)!!)
+ * per component itself. */
+int sf_basis_create(int nshell, const int *L, const int *nprim, const double *center,
+                    const double *expns, const double *coefs, int natom,
+                    const int *shell_atom, sf_basis **out);
+int sf_basis_destroy(sf_basis *b);
+int sf_basis_nbf(const sf_basis *b, int *nbf);
+
+/* Schwarz factors Q_ab = sqrt(max |(ij|ij)|), i in a, j in b, for every
+ * shell pair a >= b, at Q[a*(a+1)/2 + b] (host buffer).  No counterpart in
+ * the reference (which screens nothing). */
+int sf_schwarz(sf_basis *b, double *Q);
+
+/* Build the screened shell-quartet lists and the digestion plan: a quartet
+ * survives when Q_ab * Q_cd >= tau (computed as that product in FP64);
+ * in mixed precision, surviving quartets with Q_ab * Q_cd < tau_mixed run
+ * in FP32.  tau = 0 keeps every quartet (the unscreened set of
+ * build_eri_parallel, hf.py:37-43).  Replaces hf.py:33-61's enumeration. */
+int sf_screen(sf_basis *b, double tau, double tau_mixed);
+/* counts[3*c + {0,1,2}] = surviving / FP64 / FP32 shell quartets of
+ * quartet class c (0..20); *funcq = surviving function-level quartets. */
+int sf_quartet_counts(const sf_basis *b, long long *counts, long long *funcq,
+                      long long *primq);
+/* Sorted significant pairs of pair class pc (0..5): shells and Q. */
+int sf_pair_list(const sf_basis *b, int pc, int *n, int *sa, int *sb, double *Q);
+/* Per-bra ket counts of quartet class c (all, FP64 part). */
+int sf_quartet_list(const sf_basis *b, int c, int *nbra, int *n_all, int *n_fp64);
+
+/* Dense N^4 ERI tensor (host), drop-in for build_eri_parallel
+ * (hf.py:33-61) after sf_screen(b, 0, 0). */
+int sf_eri_dense(sf_basis *b, double *eri);
+
+/* Direct-SCF J and K (device buffers, N x N row-major) for a symmetric
+ * density P: J_pq = sum_rs (pq|rs) P_rs, K_pq = sum_rs (pr|qs) P_rs,
+ * drop-in for the two einsums of hf.py:300 (G = J - K/2).
+ * precision 0 = all FP64, 1 = mixed (FP32 below tau_mixed, FP64
+ * accumulation).  Deterministic: no atomics. */
+int sf_jk(sf_basis *b, const double *P, double *J, double *K, int precision, void *stream);
+/* sf_jk with HIP events between launch groups on `stream` (for measurement;
+ * no counterpart in the reference, whose only timer is utils.timeit,
+ * src/hf/utils.py:76-83).  ms[44] receives milliseconds: ms[2c] / ms[2c+1]
+ * = FP64 / FP32 ERI+digestion kernel of quartet class c, ms[42] = density
+ * scaling, ms[43] = deterministic plan reduction and finalisation. */
+int sf_jk_profile(sf_basis *b, const double *P, double *J, double *K, int precision,
+                  void *stream, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

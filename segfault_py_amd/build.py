@@ -1,0 +1,82 @@
+"""Build libsfeng.so (the HIP library) in-tree for gfx950 with hipcc.
+
+    python -m segfault_py_amd.build        # or: from segfault_py_amd.build import build
+
+Each .hip translation unit is compiled in parallel to an object under
+segfault_py_amd/_build/, then linked into segfault_py_amd/libsfeng.so.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+OUT = os.path.join(HERE, "libsfeng.so")
+# (source, extra defines, object name): k_eri.hip is compiled once per
+# quartet class so the 21 classes build in parallel.
+UNITS = ([("k_eri.hip", [], "k_eri_dispatch.o")]
+         + [("k_eri.hip", [f"-DSF_CLASS={c}"], f"k_eri_c{c:02d}.o") for c in range(21)]
+         + [("k_util.hip", [], "k_util.o"), ("sf_api.hip", [], "sf_api.o")])
+ARCH = os.environ.get("SF_OFFLOAD_ARCH", "gfx950")
+
+HIPCC_FLAGS = [
+    "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
+    # packed FP32 instructions are disabled on purpose: keep them out of all code
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-command-line-argument",
+]
+
+
+def hipcc() -> str:
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if cand and os.path.exists(cand):
+            return cand
+    raise RuntimeError("hipcc not found")
+
+
+def build(verbose: bool = False, extra_flags: list[str] | None = None) -> str:
+    bdir = os.path.join(HERE, "_build")
+    os.makedirs(bdir, exist_ok=True)
+    cc = hipcc()
+    flags = HIPCC_FLAGS + list(extra_flags or [])
+    objs = []
+    cmds = []
+    for src, defs, oname in UNITS:
+        obj = os.path.join(bdir, oname)
+        objs.append(obj)
+        cmds.append((oname, [cc, *flags, *defs, "-c", os.path.join(CSRC, src), "-o", obj]))
+    jobs = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", os.cpu_count() or 4))))
+    failed = []
+    running = []
+    pending = list(cmds)
+    # longest units first: the d-shell classes dominate the build time
+    pending.sort(key=lambda c: -int(c[0][7:9]) if c[0].startswith("k_eri_c") else 0)
+    while pending or running:
+        while pending and len(running) < jobs:
+            name, cmd = pending.pop(0)
+            if verbose:
+                print(" ".join(cmd), flush=True)
+            running.append((name, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))
+        name, p = running.pop(0)
+        out, _ = p.communicate()
+        if p.returncode != 0:
+            failed.append((name, out.decode(errors="replace")))
+        elif verbose and out:
+            print(out.decode(errors="replace"))
+    if failed:
+        msg = "\n".join(f"--- {s} ---\n{o}" for s, o in failed)
+        raise RuntimeError(f"hipcc failed:\n{msg}")
+    tmp = OUT + ".tmp"
+    cmd = [cc, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, OUT)
+    return OUT
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

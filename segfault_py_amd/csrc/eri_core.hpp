@@ -1,0 +1,249 @@
+// eri_core.hpp -- contracted shell-quartet ERIs by Rys quadrature, and the
+// J/K digestion of one shell quartet.  Host+device code (see sf_math.hpp).
+//
+// Reference path restated (per Cartesian component there, per shell here):
+//   rys.eri_rys     src/hf/core/core.f90:967-1002  (contraction)
+//   rys.eri_3d_rys  core.f90:917-964              (rho, P, Q, X, 2 sqrt(rho/pi))
+//   rys.recur       core.f90:815-874              (2-D recurrence)
+//   rys.shift       core.f90:877-899              (horizontal transfer)
+//   hf.py:300       G = J - K/2 from the N^4 tensor (digestion)
+// Differences by design: the number of Rys nodes is floor(L/2)+1 (exact for
+// the polynomial degree; the reference always uses 5), all Cartesian
+// components of a shell quartet share one set of nodes and 2-D integrals,
+// and the whole prefactor pi^3/(p q)^1.5 e^{...} 2 sqrt(rho/pi) is carried on
+// the x direction.
+#pragma once
+#include "sf_math.hpp"
+
+namespace sf {
+
+// Cartesian component i of angular momentum L in the reference order
+// (molecule.py:138-165): lx descending, then ly descending.
+constexpr int cart_n(int L) { return (L + 1) * (L + 2) / 2; }
+constexpr int cart_l(int L, int i, int axis)
+{
+    int c = 0;
+    for (int lx = L; lx >= 0; --lx)
+        for (int ly = L - lx; ly >= 0; --ly) {
+            if (c == i) return axis == 0 ? lx : axis == 1 ? ly : L - lx - ly;
+            ++c;
+        }
+    return 0;
+}
+constexpr double binom_c(int n, int k)
+{
+    double r = 1.0;
+    for (int i = 1; i <= k; ++i) r = r * (n - k + i) / i;
+    return r;
+}
+
+// One primitive pair of a shell pair (a, b): p = a_i + b_j, P = (a_i A + b_j B)/p,
+// K = c_i c_j exp(-a_i b_j/p |A-B|^2) with the radial normalisation in c.
+struct PrimPair {
+    double p, Px, Py, Pz, K, pad0, pad1, pad2;
+};
+
+template <int LA, int LB, int LC, int LD>
+struct QuartetShape {
+    static constexpr int L = LA + LB + LC + LD;
+    static constexpr int NR = L / 2 + 1;
+    static constexpr int NA = cart_n(LA), NB = cart_n(LB), NC = cart_n(LC), ND = cart_n(LD);
+    static constexpr int NINT = NA * NB * NC * ND;
+    static constexpr int NAB = LA + LB + 1, NCD = LC + LD + 1;
+    // digestion tiles: AB, CD, AC, AD, BC, BD
+    static constexpr int T_AB = 0;
+    static constexpr int T_CD = T_AB + NA * NB;
+    static constexpr int T_AC = T_CD + NC * ND;
+    static constexpr int T_AD = T_AC + NA * NC;
+    static constexpr int T_BC = T_AD + NA * ND;
+    static constexpr int T_BD = T_BC + NB * NC;
+    static constexpr int NTILE = T_BD + NB * ND;
+};
+
+// Rys nodes for N, with the closed-form F0/F1 rule when N == 1.
+template <int N>
+SF_HD SF_INLINE void rys_nodes(double X, double* u, double* w)
+{
+    if constexpr (N == 1) rys_roots1_closed(X, u, w);
+    else rys_roots<N>(X, u, w);
+}
+
+// Accumulate (ab|cd) for all Cartesian components into acc[NINT] (not
+// zeroed here).  Bra primitive pairs bra[0..nbra), ket pairs ket[0..nket).
+// R is the type of the recurrences and of acc; X, nodes and weights are
+// always evaluated in double.
+template <int LA, int LB, int LC, int LD, typename R>
+SF_HD SF_INLINE void eri_shell_quartet(const PrimPair* __restrict__ bra, int nbra,
+                                       const PrimPair* __restrict__ ket, int nket,
+                                       const double* A, const double* B, const double* C,
+                                       const double* D, R* __restrict__ acc)
+{
+    using S = QuartetShape<LA, LB, LC, LD>;
+    constexpr int NR = S::NR, NAB = S::NAB, NCD = S::NCD;
+    R ABp[3][LB + 1], CDp[3][LD + 1];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const R ab = (R)(A[k] - B[k]), cd = (R)(C[k] - D[k]);
+        ABp[k][0] = (R)1;
+        CDp[k][0] = (R)1;
+#pragma unroll
+        for (int e = 1; e <= LB; ++e) ABp[k][e] = ABp[k][e - 1] * ab;
+#pragma unroll
+        for (int e = 1; e <= LD; ++e) CDp[k][e] = CDp[k][e - 1] * cd;
+    }
+    for (int ib = 0; ib < nbra; ++ib) {
+        const PrimPair pb = bra[ib];
+        const double p = pb.p;
+        const double PA[3] = {pb.Px - A[0], pb.Py - A[1], pb.Pz - A[2]};
+        for (int ik = 0; ik < nket; ++ik) {
+            const PrimPair pk = ket[ik];
+            const double q = pk.p;
+            const double pq = p + q;
+            const double ipq = 1.0 / pq;
+            const double rho = p * q * ipq;
+            const double PQ[3] = {pb.Px - pk.Px, pb.Py - pk.Py, pb.Pz - pk.Pz};
+            const double X = rho * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]);
+            // 2 pi^{5/2} / (p q sqrt(p+q)) K_ab K_cd
+            const double pref = 34.986836655249725 * pb.K * pk.K / (p * q * sqrt(pq));
+            double u[NR], w[NR];
+            rys_nodes<NR>(X, u, w);
+            const double QC[3] = {pk.Px - C[0], pk.Py - C[1], pk.Pz - C[2]};
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                const double t2 = u[r];
+                // 2-D integrals G[n][m], n <= LA+LB, m <= LC+LD (rys.recur)
+                R G[3][NAB][NCD];
+                const R b00 = (R)(0.5 * ipq * t2);
+                const R b10 = (R)(0.5 / p * (1.0 - q * ipq * t2));
+                const R b01 = (R)(0.5 / q * (1.0 - p * ipq * t2));
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    const R c00 = (R)(PA[k] - q * ipq * PQ[k] * t2);
+                    const R c00p = (R)(QC[k] + p * ipq * PQ[k] * t2);
+                    G[k][0][0] = (k == 0) ? (R)(pref * w[r]) : (R)1;
+                    if (NAB > 1) G[k][1][0] = c00 * G[k][0][0];
+#pragma unroll
+                    for (int n = 1; n < NAB - 1; ++n)
+                        G[k][n + 1][0] = (R)n * b10 * G[k][n - 1][0] + c00 * G[k][n][0];
+                    if (NCD > 1) {
+#pragma unroll
+                        for (int n = 0; n < NAB; ++n) {
+                            G[k][n][1] = c00p * G[k][n][0];
+                            if (n > 0) G[k][n][1] += (R)n * b00 * G[k][n - 1][0];
+                        }
+#pragma unroll
+                        for (int m = 1; m < NCD - 1; ++m)
+#pragma unroll
+                            for (int n = 0; n < NAB; ++n) {
+                                R g = (R)m * b01 * G[k][n][m - 1] + c00p * G[k][n][m];
+                                if (n > 0) g += (R)n * b00 * G[k][n - 1][m];
+                                G[k][n][m + 1] = g;
+                            }
+                    }
+                }
+                // horizontal transfer (rys.shift): ket then bra
+                R H[3][NAB][LC + 1][LD + 1];
+#pragma unroll
+                for (int k = 0; k < 3; ++k)
+#pragma unroll
+                    for (int n = 0; n < NAB; ++n)
+#pragma unroll
+                        for (int ic = 0; ic <= LC; ++ic)
+#pragma unroll
+                            for (int id = 0; id <= LD; ++id) {
+                                R s = (R)0;
+#pragma unroll
+                                for (int m = 0; m <= id; ++m)
+                                    s += (R)binom_c(id, m) * CDp[k][id - m] * G[k][n][ic + m];
+                                H[k][n][ic][id] = s;
+                            }
+                R I[3][LA + 1][LB + 1][LC + 1][LD + 1];
+#pragma unroll
+                for (int k = 0; k < 3; ++k)
+#pragma unroll
+                    for (int ia = 0; ia <= LA; ++ia)
+#pragma unroll
+                        for (int ib2 = 0; ib2 <= LB; ++ib2)
+#pragma unroll
+                            for (int ic = 0; ic <= LC; ++ic)
+#pragma unroll
+                                for (int id = 0; id <= LD; ++id) {
+                                    R s = (R)0;
+#pragma unroll
+                                    for (int n = 0; n <= ib2; ++n)
+                                        s += (R)binom_c(ib2, n) * ABp[k][ib2 - n] * H[k][ia + n][ic][id];
+                                    I[k][ia][ib2][ic][id] = s;
+                                }
+#pragma unroll
+                for (int fa = 0; fa < S::NA; ++fa)
+#pragma unroll
+                    for (int fb = 0; fb < S::NB; ++fb)
+#pragma unroll
+                        for (int fc = 0; fc < S::NC; ++fc)
+#pragma unroll
+                            for (int fd = 0; fd < S::ND; ++fd) {
+                                const int idx = ((fa * S::NB + fb) * S::NC + fc) * S::ND + fd;
+                                acc[idx] += I[0][cart_l(LA, fa, 0)][cart_l(LB, fb, 0)][cart_l(LC, fc, 0)][cart_l(LD, fd, 0)] *
+                                            I[1][cart_l(LA, fa, 1)][cart_l(LB, fb, 1)][cart_l(LC, fc, 1)][cart_l(LD, fd, 1)] *
+                                            I[2][cart_l(LA, fa, 2)][cart_l(LB, fb, 2)][cart_l(LC, fc, 2)][cart_l(LD, fd, 2)];
+                            }
+            }
+        }
+    }
+}
+
+// J/K digestion of one shell quartet.  v[NINT] are the integrals in the
+// shell-normalised basis times the degeneracy factor; P is the density in
+// the same basis (row-major, ld = nbf).  Writes tiles[NTILE]:
+//   X_ab += 2 sum_cd v P_cd, X_cd += 2 sum_ab v P_ab,
+//   Y_ac += sum v P_bd, Y_ad += sum v P_bc, Y_bc += sum v P_ad, Y_bd += sum v P_ac,
+// so that over all unique quartets J = X + X^T and K = Y + Y^T reproduce
+// einsum('pqrs,rs') and einsum('prqs,rs') of hf.py:300.
+template <int LA, int LB, int LC, int LD, typename VT>
+SF_HD SF_INLINE void digest_quartet(const VT* __restrict__ v, const double* __restrict__ P, int nbf,
+                                    int fa0, int fb0, int fc0, int fd0, double* __restrict__ t)
+{
+    using S = QuartetShape<LA, LB, LC, LD>;
+    double Pab[S::NA][S::NB], Pcd[S::NC][S::ND], Pac[S::NA][S::NC], Pad[S::NA][S::ND],
+        Pbc[S::NB][S::NC], Pbd[S::NB][S::ND];
+#pragma unroll
+    for (int a = 0; a < S::NA; ++a) {
+#pragma unroll
+        for (int b = 0; b < S::NB; ++b) Pab[a][b] = P[(long)(fa0 + a) * nbf + fb0 + b];
+#pragma unroll
+        for (int c = 0; c < S::NC; ++c) Pac[a][c] = P[(long)(fa0 + a) * nbf + fc0 + c];
+#pragma unroll
+        for (int d = 0; d < S::ND; ++d) Pad[a][d] = P[(long)(fa0 + a) * nbf + fd0 + d];
+    }
+#pragma unroll
+    for (int b = 0; b < S::NB; ++b) {
+#pragma unroll
+        for (int c = 0; c < S::NC; ++c) Pbc[b][c] = P[(long)(fb0 + b) * nbf + fc0 + c];
+#pragma unroll
+        for (int d = 0; d < S::ND; ++d) Pbd[b][d] = P[(long)(fb0 + b) * nbf + fd0 + d];
+    }
+#pragma unroll
+    for (int c = 0; c < S::NC; ++c)
+#pragma unroll
+        for (int d = 0; d < S::ND; ++d) Pcd[c][d] = P[(long)(fc0 + c) * nbf + fd0 + d];
+#pragma unroll
+    for (int i = 0; i < S::NTILE; ++i) t[i] = 0.0;
+#pragma unroll
+    for (int a = 0; a < S::NA; ++a)
+#pragma unroll
+        for (int b = 0; b < S::NB; ++b)
+#pragma unroll
+            for (int c = 0; c < S::NC; ++c)
+#pragma unroll
+                for (int d = 0; d < S::ND; ++d) {
+                    const double x = (double)v[((a * S::NB + b) * S::NC + c) * S::ND + d];
+                    t[S::T_AB + a * S::NB + b] += 2.0 * x * Pcd[c][d];
+                    t[S::T_CD + c * S::ND + d] += 2.0 * x * Pab[a][b];
+                    t[S::T_AC + a * S::NC + c] += x * Pbd[b][d];
+                    t[S::T_AD + a * S::ND + d] += x * Pbc[b][c];
+                    t[S::T_BC + b * S::NC + c] += x * Pad[a][d];
+                    t[S::T_BD + b * S::ND + d] += x * Pac[a][c];
+                }
+}
+
+}  // namespace sf

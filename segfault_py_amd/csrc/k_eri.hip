@@ -1,0 +1,342 @@
+// k_eri.hip -- per-angular-momentum-class ERI kernels for gfx950.
+//
+// One thread computes one contracted shell quartet (ab|cd) of its class
+// (all primitive quartets, all Cartesian components, nodes floor(L/2)+1)
+// and either
+//   * digests it into six J/K partial tiles (kModeJK), written as SoA planes
+//     out[tile_element * qstride + q] so a wavefront's stores are contiguous,
+//   * or stores the raw component integrals (kModeRaw, dense export).
+// The flat quartet index of a segment is mapped back to (bra, ket) by a
+// binary search of the per-bra prefix sums; consecutive threads share the
+// bra pair, so its primitive data is read once per wavefront from cache.
+//
+// Replaces the per-quartet Python fan-out + f2py call of hf.py:18-61 and
+// the einsum digestion of hf.py:300; the integral math follows
+// rys.eri_rys (src/hf/core/core.f90:917-1002) -- see eri_core.hpp.
+#include "sf_internal.hpp"
+
+// Built once per quartet class with -DSF_CLASS=<id> (that class's kernels
+// only, so the classes compile in parallel) and once without SF_CLASS (the
+// dispatch tables).
+#define SF_FOR_EACH_CLASS(X)                                                                     \
+    X(0, 0, 0, 0, 0) X(1, 1, 0, 0, 0) X(2, 1, 0, 1, 0) X(3, 1, 1, 0, 0) X(4, 1, 1, 1, 0)          \
+    X(5, 1, 1, 1, 1) X(6, 2, 0, 0, 0) X(7, 2, 0, 1, 0) X(8, 2, 0, 1, 1) X(9, 2, 0, 2, 0)          \
+    X(10, 2, 1, 0, 0) X(11, 2, 1, 1, 0) X(12, 2, 1, 1, 1) X(13, 2, 1, 2, 0) X(14, 2, 1, 2, 1)    \
+    X(15, 2, 2, 0, 0) X(16, 2, 2, 1, 0) X(17, 2, 2, 1, 1) X(18, 2, 2, 2, 0) X(19, 2, 2, 2, 1)    \
+    X(20, 2, 2, 2, 2)
+
+namespace sf {
+
+using SchwarzLaunch = int (*)(const int*, const int*, const int*, const int*, int, const PrimPair*,
+                              const double*, double*, hipStream_t);
+#define X(id, a, b, c, d)                                                                       \
+    int launch_class_##id(int fp32, int mode, const ClassParams& cp, hipStream_t s);           \
+    int launch_schwarz_##id(const int*, const int*, const int*, const int*, int,               \
+                            const PrimPair*, const double*, double*, hipStream_t);
+SF_FOR_EACH_CLASS(X)
+#undef X
+
+#ifdef SF_CLASS
+
+__device__ __forceinline__ int find_bra(const long long* __restrict__ base, int n, long long q)
+{
+    int lo = 0, hi = n;  // base[lo] <= q < base[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (base[mid] <= q) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+template <int LA, int LB, int LC, int LD, typename R, int MODE>
+__global__ __launch_bounds__(256) void k_eri_class(const ClassParams cp)
+{
+    using S = QuartetShape<LA, LB, LC, LD>;
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= cp.nq) return;
+    const int i = find_bra(cp.seg_base, cp.bra.n, q);
+    const int j = cp.seg_k0[i] + (int)(q - cp.seg_base[i]);
+    const long long qa = cp.qall_base[i] + j;
+    const int a = cp.bra.sa[i], b = cp.bra.sb[i];
+    const int c = cp.ket.sa[j], d = cp.ket.sb[j];
+    const double A[3] = {cp.shell_xyz[3 * a], cp.shell_xyz[3 * a + 1], cp.shell_xyz[3 * a + 2]};
+    const double B[3] = {cp.shell_xyz[3 * b], cp.shell_xyz[3 * b + 1], cp.shell_xyz[3 * b + 2]};
+    const double C[3] = {cp.shell_xyz[3 * c], cp.shell_xyz[3 * c + 1], cp.shell_xyz[3 * c + 2]};
+    const double D[3] = {cp.shell_xyz[3 * d], cp.shell_xyz[3 * d + 1], cp.shell_xyz[3 * d + 2]};
+    R acc[S::NINT];
+#pragma unroll
+    for (int k = 0; k < S::NINT; ++k) acc[k] = (R)0;
+    eri_shell_quartet<LA, LB, LC, LD, R>(cp.prim + cp.bra.poff[i], cp.bra.np[i],
+                                         cp.prim + cp.ket.poff[j], cp.ket.np[j], A, B, C, D, acc);
+    if constexpr (MODE == kModeRaw) {
+#pragma unroll
+        for (int k = 0; k < S::NINT; ++k) cp.out[k * cp.qstride + qa] = (double)acc[k];
+    } else {
+        const double fac = ((a == b) ? 0.5 : 1.0) * ((c == d) ? 0.5 : 1.0) *
+                           ((cp.same && i == j) ? 0.5 : 1.0);
+        double v[S::NINT];
+#pragma unroll
+        for (int k = 0; k < S::NINT; ++k) v[k] = fac * (double)acc[k];
+        double t[S::NTILE];
+        digest_quartet<LA, LB, LC, LD, double>(v, cp.P, cp.nbf, cp.shell_foff[a], cp.shell_foff[b],
+                                               cp.shell_foff[c], cp.shell_foff[d], t);
+#pragma unroll
+        for (int k = 0; k < S::NTILE; ++k) cp.out[k * cp.qstride + qa] = t[k];
+    }
+}
+
+// Schwarz factor Q_ab = sqrt(max_{i in a, j in b} |(ij|ij)|) for one pair class.
+template <int LA, int LB>
+__global__ __launch_bounds__(256) void k_schwarz(const int* __restrict__ sa, const int* __restrict__ sb,
+                                                 const int* __restrict__ poff, const int* __restrict__ np,
+                                                 int n, const PrimPair* __restrict__ prim,
+                                                 const double* __restrict__ xyz, double* __restrict__ Q)
+{
+    using S = QuartetShape<LA, LB, LA, LB>;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int a = sa[i], b = sb[i];
+    const double A[3] = {xyz[3 * a], xyz[3 * a + 1], xyz[3 * a + 2]};
+    const double B[3] = {xyz[3 * b], xyz[3 * b + 1], xyz[3 * b + 2]};
+    double acc[S::NINT];
+#pragma unroll
+    for (int k = 0; k < S::NINT; ++k) acc[k] = 0.0;
+    eri_shell_quartet<LA, LB, LA, LB, double>(prim + poff[i], np[i], prim + poff[i], np[i], A, B, A,
+                                              B, acc);
+    double m = 0.0;
+#pragma unroll
+    for (int fa = 0; fa < S::NA; ++fa)
+#pragma unroll
+        for (int fb = 0; fb < S::NB; ++fb)
+            m = fmax(m, fabs(acc[((fa * S::NB + fb) * S::NA + fa) * S::NB + fb]));
+    Q[i] = sqrt(m);
+}
+
+
+template <int LA, int LB, int LC, int LD>
+static int launch_class(int fp32, int mode, const ClassParams& cp, hipStream_t s)
+{
+    if (cp.nq <= 0) return kOk;
+    const int bs = 256;
+    const long long nb = (cp.nq + bs - 1) / bs;
+    if (nb > 0x7fffffffLL) {
+        set_error(kEUnsupported, "quartet segment too large (%lld)", cp.nq);
+        return kEUnsupported;
+    }
+    dim3 grid((unsigned)nb), block(bs);
+    if (mode == kModeRaw)
+        hipLaunchKernelGGL((k_eri_class<LA, LB, LC, LD, double, kModeRaw>), grid, block, 0, s, cp);
+    else if (fp32)
+        hipLaunchKernelGGL((k_eri_class<LA, LB, LC, LD, float, kModeJK>), grid, block, 0, s, cp);
+    else
+        hipLaunchKernelGGL((k_eri_class<LA, LB, LC, LD, double, kModeJK>), grid, block, 0, s, cp);
+    SF_HIP(hipGetLastError());
+    return kOk;
+}
+
+template <int LA, int LB, int LC, int LD>
+static int launch_schwarz_t(const int* sa, const int* sb, const int* poff, const int* np, int n,
+                            const PrimPair* prim, const double* xyz, double* Q, hipStream_t s)
+{
+    if constexpr (LA == LC && LB == LD) {
+        if (n <= 0) return kOk;
+        hipLaunchKernelGGL((k_schwarz<LA, LB>), dim3((n + 127) / 128), dim3(128), 0, s, sa, sb, poff,
+                           np, n, prim, xyz, Q);
+        SF_HIP(hipGetLastError());
+        return kOk;
+    } else {
+        set_error(kEUnsupported, "no Schwarz kernel in an off-diagonal class");
+        return kEUnsupported;
+    }
+}
+
+#define SF_INST(id, a, b, c, d)                                                                 \
+    int launch_class_##id(int fp32, int mode, const ClassParams& cp, hipStream_t s)            \
+    {                                                                                          \
+        return launch_class<a, b, c, d>(fp32, mode, cp, s);                                    \
+    }                                                                                          \
+    int launch_schwarz_##id(const int* sa, const int* sb, const int* poff, const int* np, int n, \
+                            const PrimPair* prim, const double* xyz, double* Q, hipStream_t s)  \
+    {                                                                                          \
+        return launch_schwarz_t<a, b, c, d>(sa, sb, poff, np, n, prim, xyz, Q, s);             \
+    }
+#define SF_PICK(id, a, b, c, d) SF_PICK_##id(id, a, b, c, d)
+#if SF_CLASS == 0
+#define SF_PICK_0(id, a, b, c, d) SF_INST(id, a, b, c, d)
+#else
+#define SF_PICK_0(id, a, b, c, d)
+#endif
+#if SF_CLASS == 1
+#define SF_PICK_1(id, a, b, c, d) SF_INST(id, a, b, c, d)
+#else
+#define SF_PICK_1(id, a, b, c, d)
+#endif
+#if SF_CLASS == 2
+#define SF_PICK_2(id, a, b, c, d) SF_INST(id, a, b, c, d)
+#else
+#define SF_PICK_2(id, a, b, c, d)
+#endif
+#if SF_CLASS == 3
+#define SF_PICK_3(id, a, b, c, d) SF_INST(id, a, b, c, d)
+#else
+#define SF_PICK_3(id, a, b, c, d)
+#endif
+#if SF_CLASS == 4
+#define SF_PICK_4(id, a, b, c, d) SF_INST(id, a, b, c, d)
+#else
+#define SF_PICK_4(id, a, b, c, d)
+#endif
+#if SF_CLASS == 5
+#define SF_PICK_5(id, a, b, c, d) SF_INST(id, a, b, c, d)
+#else
+#define SF_PICK_5(id, a, b, c, d)
+#endif
+#if SF_CLASS == 6
+#define SF_PICK_6(id, a, b, c, d) SF_INST(id, a, b, c, d)
+#else
+#define SF_PICK_6(id, a, b, c, d)
+#endif
+#if SF_CLASS == 7
+#define SF_PICK_7(id, a, b, c, d) SF_INST(id, a, b, c, d)
+#else
+#define SF_PICK_7(id, a, b, c, d)
+#endif
+#if SF_CLASS == 8
+#define SF_PICK_8(id, a, b, c, d) SF_INST(id, a, b, c, d)
+#else
+#define SF_PICK_8(id, a, b, c, d)
+#endif
+#if SF_CLASS == 9
+#define SF_PICK_9(id, a, b, c, d) SF_INST(id, a, b, c, d)
+#else
+#define SF_PICK_9(id, a, b, c, d)
+#endif
+#if SF_CLASS == 10
+#define SF_PICK_10(id, a, b, c, d) SF_INST(id, a, b, c, d)
+#else
+#define SF_PICK_10(id, a, b, c, d)
+#endif
+#if SF_CLASS == 11
+#define SF_PICK_11(id, a, b, c, d) SF_INST(id, a, b, c, d)
+#else
+#define SF_PICK_11(id, a, b, c, d)
+#endif
+#if SF_CLASS == 12
+#define SF_PICK_12(id, a, b, c, d) SF_INST(id, a, b, c, d)
+#else
+#define SF_PICK_12(id, a, b, c, d)
+#endif
+#if SF_CLASS == 13
+#define SF_PICK_13(id, a, b, c, d) SF_INST(id, a, b, c, d)
+#else
+#define SF_PICK_13(id, a, b, c, d)
+#endif
+#if SF_CLASS == 14
+#define SF_PICK_14(id, a, b, c, d) SF_INST(id, a, b, c, d)
+#else
+#define SF_PICK_14(id, a, b, c, d)
+#endif
+#if SF_CLASS == 15
+#define SF_PICK_15(id, a, b, c, d) SF_INST(id, a, b, c, d)
+#else
+#define SF_PICK_15(id, a, b, c, d)
+#endif
+#if SF_CLASS == 16
+#define SF_PICK_16(id, a, b, c, d) SF_INST(id, a, b, c, d)
+#else
+#define SF_PICK_16(id, a, b, c, d)
+#endif
+#if SF_CLASS == 17
+#define SF_PICK_17(id, a, b, c, d) SF_INST(id, a, b, c, d)
+#else
+#define SF_PICK_17(id, a, b, c, d)
+#endif
+#if SF_CLASS == 18
+#define SF_PICK_18(id, a, b, c, d) SF_INST(id, a, b, c, d)
+#else
+#define SF_PICK_18(id, a, b, c, d)
+#endif
+#if SF_CLASS == 19
+#define SF_PICK_19(id, a, b, c, d) SF_INST(id, a, b, c, d)
+#else
+#define SF_PICK_19(id, a, b, c, d)
+#endif
+#if SF_CLASS == 20
+#define SF_PICK_20(id, a, b, c, d) SF_INST(id, a, b, c, d)
+#else
+#define SF_PICK_20(id, a, b, c, d)
+#endif
+
+SF_FOR_EACH_CLASS(SF_PICK)
+
+#else  // dispatch tables
+
+static_assert(quartet_class(pair_class(2, 1), pair_class(1, 1)) == 12, "class numbering");
+
+int launch_eri_class(int qcls, int fp32, int mode, const ClassParams& cp, hipStream_t s)
+{
+    switch (qcls) {
+#define X(id, a, b, c, d) \
+    case id: return launch_class_##id(fp32, mode, cp, s);
+        SF_FOR_EACH_CLASS(X)
+#undef X
+    }
+    set_error(kEUnsupported, "quartet class %d", qcls);
+    return kEUnsupported;
+}
+
+int class_ntile(int qcls)
+{
+    switch (qcls) {
+#define X(id, a, b, c, d) \
+    case id: return QuartetShape<a, b, c, d>::NTILE;
+        SF_FOR_EACH_CLASS(X)
+#undef X
+    }
+    return 0;
+}
+
+int class_nint(int qcls)
+{
+    switch (qcls) {
+#define X(id, a, b, c, d) \
+    case id: return QuartetShape<a, b, c, d>::NINT;
+        SF_FOR_EACH_CLASS(X)
+#undef X
+    }
+    return 0;
+}
+
+void class_tile_offsets(int qcls, int off[6])
+{
+    switch (qcls) {
+#define X(id, a, b, c, d)                                        \
+    case id: {                                                   \
+        using S = QuartetShape<a, b, c, d>;                      \
+        off[0] = S::T_AB; off[1] = S::T_CD; off[2] = S::T_AC;    \
+        off[3] = S::T_AD; off[4] = S::T_BC; off[5] = S::T_BD;    \
+        return;                                                  \
+    }
+        SF_FOR_EACH_CLASS(X)
+#undef X
+    }
+}
+
+int launch_schwarz(int pc, const int* sa, const int* sb, const int* poff, const int* np, int n,
+                   const PrimPair* prim, const double* shell_xyz, double* Q, hipStream_t s)
+{
+    // the diagonal quartet class (pc|pc) holds the pair class's Schwarz kernel
+    static const SchwarzLaunch diag[kNumPairClass] = {launch_schwarz_0, launch_schwarz_2,
+                                                      launch_schwarz_5, launch_schwarz_9,
+                                                      launch_schwarz_14, launch_schwarz_20};
+    if (pc < 0 || pc >= kNumPairClass) {
+        set_error(kEUnsupported, "pair class %d", pc);
+        return kEUnsupported;
+    }
+    return diag[pc](sa, sb, poff, np, n, prim, shell_xyz, Q, s);
+}
+
+#endif  // SF_CLASS
+
+}  // namespace sf

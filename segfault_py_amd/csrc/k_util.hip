@@ -1,0 +1,423 @@
+// k_util.hip -- batched Boys / Rys kernels, the Schwarz quartet-list builder,
+// the digestion plan and its deterministic (atomics-free) reduction, and the
+// generic per-component quartet kernel behind the rys.eri_rys drop-in.
+#include "sf_internal.hpp"
+
+namespace sf {
+
+// --------------------------------------------------------------- Boys -----
+__global__ __launch_bounds__(256) void k_boys(long long n, int mmax, const double* __restrict__ T,
+                                              double* __restrict__ F)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double t = T[i];
+    if (!(t >= 0.0) || isinf(t)) {  // negative, NaN or infinite argument
+        for (int m = 0; m <= mmax; ++m) F[i * (mmax + 1) + m] = (t == INFINITY) ? 0.0 : NAN;
+        return;
+    }
+    double f[SF_BOYS_MTAB];
+    boys_grid(mmax, t, f);
+    for (int m = 0; m <= mmax; ++m) F[i * (mmax + 1) + m] = f[m];
+}
+
+int launch_boys(long long n, int mmax, const double* T, double* F, hipStream_t s)
+{
+    if (mmax < 0 || mmax > SF_BOYS_MTAB - 10) {
+        set_error(kEInval, "mmax %d outside [0, %d]", mmax, SF_BOYS_MTAB - 10);
+        return kEInval;
+    }
+    if (n <= 0) return kOk;
+    hipLaunchKernelGGL(k_boys, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, mmax, T, F);
+    SF_HIP(hipGetLastError());
+    return kOk;
+}
+
+// ---------------------------------------------------------------- Rys -----
+template <int N>
+__global__ __launch_bounds__(256) void k_rys(long long n, const double* __restrict__ X,
+                                             double* __restrict__ U, double* __restrict__ W)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double u[N], w[N];
+    const double x = X[i];
+    if (!(x >= 0.0)) {  // negative or NaN argument: no table lookup
+#pragma unroll
+        for (int r = 0; r < N; ++r) U[i * N + r] = W[i * N + r] = NAN;
+        return;
+    }
+    rys_roots<N>(x, u, w);
+#pragma unroll
+    for (int r = 0; r < N; ++r) {
+        U[i * N + r] = u[r];
+        W[i * N + r] = w[r];
+    }
+}
+
+int launch_rys(long long n, int nroots, const double* X, double* u, double* w, hipStream_t s)
+{
+    if (n <= 0) return kOk;
+    dim3 g((unsigned)((n + 255) / 256)), b(256);
+    switch (nroots) {
+    case 1: hipLaunchKernelGGL(k_rys<1>, g, b, 0, s, n, X, u, w); break;
+    case 2: hipLaunchKernelGGL(k_rys<2>, g, b, 0, s, n, X, u, w); break;
+    case 3: hipLaunchKernelGGL(k_rys<3>, g, b, 0, s, n, X, u, w); break;
+    case 4: hipLaunchKernelGGL(k_rys<4>, g, b, 0, s, n, X, u, w); break;
+    case 5: hipLaunchKernelGGL(k_rys<5>, g, b, 0, s, n, X, u, w); break;
+    default: set_error(kEInval, "nroots %d outside [1, 5]", nroots); return kEInval;
+    }
+    SF_HIP(hipGetLastError());
+    return kOk;
+}
+
+// ------------------------------------------------------ quartet list ------
+// For bra i (Q sorted descending in both lists): the surviving kets are the
+// prefix j < cut with Qbra[i] * Qket[j] >= tau (the product is monotone in
+// Qket), limited to j <= i when bra and ket are the same pair class.  The
+// FP64 part is the prefix that also meets tau_mixed.
+__device__ __forceinline__ int count_prefix(double qi, const double* __restrict__ Qk, int n, double tau)
+{
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (qi * Qk[mid] >= tau) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ void k_count_kets(const double* __restrict__ Qb, int nb, const double* __restrict__ Qk,
+                             int nk, int same, double tau, double tau_mixed, int* __restrict__ n_all,
+                             int* __restrict__ n_fp64)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nb) return;
+    const double qi = Qb[i];
+    int n = count_prefix(qi, Qk, nk, tau);
+    if (same && n > i + 1) n = i + 1;
+    int n64 = n;
+    if (tau_mixed > tau) {
+        n64 = count_prefix(qi, Qk, nk, tau_mixed);
+        if (n64 > n) n64 = n;
+    }
+    n_all[i] = n;
+    n_fp64[i] = n64;
+}
+
+int launch_count_kets(const double* Qbra, int nbra, const double* Qket, int nket, int same,
+                      double tau, double tau_mixed, int* n_all, int* n_fp64, hipStream_t s)
+{
+    if (nbra <= 0) return kOk;
+    hipLaunchKernelGGL(k_count_kets, dim3((nbra + 255) / 256), dim3(256), 0, s, Qbra, nbra, Qket,
+                       nket, same, tau, tau_mixed, n_all, n_fp64);
+    SF_HIP(hipGetLastError());
+    return kOk;
+}
+
+// ------------------------------------------------------ digestion plan ----
+__device__ __forceinline__ int find_base(const long long* __restrict__ base, int n, long long q)
+{
+    int lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (base[mid] <= q) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(256) void k_plan_entries(int qcls, const PairListDev bra,
+                                                      const PairListDev ket,
+                                                      const long long* __restrict__ qall_base,
+                                                      long long nq, int ns,
+                                                      unsigned int* __restrict__ keys,
+                                                      unsigned long long* __restrict__ vals)
+{
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nq) return;
+    const int i = find_base(qall_base, bra.n, q);
+    const int j = (int)(q - qall_base[i]);
+    const unsigned a = bra.sa[i], b = bra.sb[i], c = ket.sa[j], d = ket.sb[j];
+    const unsigned n2 = (unsigned)ns * (unsigned)ns;
+    const unsigned k[6] = {a * ns + b, c * ns + d, n2 + a * ns + c, n2 + a * ns + d,
+                           n2 + b * ns + c, n2 + b * ns + d};
+#pragma unroll
+    for (int t = 0; t < 6; ++t) {
+        keys[6 * q + t] = k[t];
+        vals[6 * q + t] = ((unsigned long long)qcls << 58) | ((unsigned long long)t << 55) |
+                          (unsigned long long)q;
+    }
+}
+
+int launch_plan_entries(int qcls, const PairListDev& bra, const PairListDev& ket,
+                        const long long* qall_base, long long nq, int nshell, unsigned int* keys,
+                        unsigned long long* vals, hipStream_t s)
+{
+    if (nq <= 0) return kOk;
+    hipLaunchKernelGGL(k_plan_entries, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, s, qcls,
+                       bra, ket, qall_base, nq, nshell, keys, vals);
+    SF_HIP(hipGetLastError());
+    return kOk;
+}
+
+__global__ __launch_bounds__(256) void k_segments(const unsigned int* __restrict__ keys, long long n,
+                                                  long long* __restrict__ seg_begin,
+                                                  long long* __restrict__ seg_end)
+{
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const unsigned k = keys[p];
+    if (p == 0 || keys[p - 1] != k) seg_begin[k] = p;
+    if (p == n - 1 || keys[p + 1] != k) seg_end[k] = p + 1;
+}
+
+int launch_segments(const unsigned int* keys, long long n, long long* seg_begin, long long* seg_end,
+                    hipStream_t s)
+{
+    if (n <= 0) return kOk;
+    hipLaunchKernelGGL(k_segments, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, keys, n,
+                       seg_begin, seg_end);
+    SF_HIP(hipGetLastError());
+    return kOk;
+}
+
+// One 256-thread block per destination shell pair: fixed strided assignment
+// of its plan entries to threads, a fixed xor-butterfly per wavefront and a
+// fixed-order sum of the four wavefronts -- bitwise reproducible, no atomics.
+template <int N1, int N2>
+__global__ __launch_bounds__(256) void k_reduce(const int* __restrict__ dest_keys,
+                                                const long long* __restrict__ seg_begin,
+                                                const long long* __restrict__ seg_end,
+                                                const unsigned long long* __restrict__ vals,
+                                                const ClassTab* __restrict__ tab,
+                                                const int* __restrict__ foff, int ns, int nbf,
+                                                double* __restrict__ X, double* __restrict__ Y)
+{
+    constexpr int NE = N1 * N2;
+    const int key = dest_keys[blockIdx.x];
+    const int n2 = ns * ns;
+    const int m = key / n2;
+    const int rem = key - m * n2;
+    const int s1 = rem / ns, s2 = rem - (rem / ns) * ns;
+    double acc[NE];
+#pragma unroll
+    for (int k = 0; k < NE; ++k) acc[k] = 0.0;
+    const long long e1 = seg_end[key];
+    for (long long e = seg_begin[key] + threadIdx.x; e < e1; e += 256) {
+        const unsigned long long v = vals[e];
+        const int cls = (int)(v >> 58);
+        const int t = (int)((v >> 55) & 7);
+        const long long q = (long long)(v & ((1ULL << 55) - 1));
+        const long long qs = tab[cls].qstride;
+        const double* src = tab[cls].out + (long long)tab[cls].off[t] * qs + q;
+#pragma unroll
+        for (int k = 0; k < NE; ++k) acc[k] += src[k * qs];
+    }
+#pragma unroll
+    for (int k = 0; k < NE; ++k)
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) acc[k] += __shfl_xor(acc[k], o, 64);
+    __shared__ double red[4][NE];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < NE; ++k) red[wave][k] = acc[k];
+    }
+    __syncthreads();
+    if (threadIdx.x < NE) {
+        const int k = threadIdx.x;
+        const double sum = ((red[0][k] + red[1][k]) + red[2][k]) + red[3][k];
+        double* dst = (m == 0) ? X : Y;
+        dst[(long long)(foff[s1] + k / N2) * nbf + foff[s2] + k % N2] = sum;
+    }
+}
+
+int launch_reduce(int n1, int n2, const int* dest_keys, int ndest, const long long* seg_begin,
+                  const long long* seg_end, const unsigned long long* vals, const ClassTab* tab,
+                  const int* shell_foff, int nshell, int nbf, double* X, double* Y, hipStream_t s)
+{
+    if (ndest <= 0) return kOk;
+    dim3 g(ndest), b(256);
+#define R_(a, c)                                                                             \
+    if (n1 == a && n2 == c) {                                                                \
+        hipLaunchKernelGGL((k_reduce<a, c>), g, b, 0, s, dest_keys, seg_begin, seg_end, vals,  \
+                           tab, shell_foff, nshell, nbf, X, Y);                              \
+        SF_HIP(hipGetLastError());                                                           \
+        return kOk;                                                                          \
+    }
+    R_(1, 1) R_(1, 3) R_(1, 6) R_(3, 1) R_(3, 3) R_(3, 6) R_(6, 1) R_(6, 3) R_(6, 6)
+#undef R_
+    set_error(kEUnsupported, "tile %dx%d", n1, n2);
+    return kEUnsupported;
+}
+
+// --------------------------------------------------- small dense kernels --
+__global__ void k_scale_density(const double* __restrict__ P, const double* __restrict__ f, int n,
+                                double* __restrict__ Ps)
+{
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)n * n) return;
+    const int i = (int)(idx / n), j = (int)(idx % n);
+    Ps[idx] = f[i] * f[j] * P[idx];
+}
+
+int launch_scale_density(const double* P, const double* fnorm, int nbf, double* Ps, hipStream_t s)
+{
+    const long long n = (long long)nbf * nbf;
+    hipLaunchKernelGGL(k_scale_density, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, P, fnorm,
+                       nbf, Ps);
+    SF_HIP(hipGetLastError());
+    return kOk;
+}
+
+__global__ void k_finalize(const double* __restrict__ X, const double* __restrict__ Y,
+                           const double* __restrict__ f, int n, double* __restrict__ J,
+                           double* __restrict__ K)
+{
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)n * n) return;
+    const int i = (int)(idx / n), j = (int)(idx % n);
+    const long long tr = (long long)j * n + i;
+    const double s = f[i] * f[j];
+    J[idx] = s * (X[idx] + X[tr]);
+    K[idx] = s * (Y[idx] + Y[tr]);
+}
+
+int launch_finalize(const double* X, const double* Y, const double* fnorm, int nbf, double* J,
+                    double* K, hipStream_t s)
+{
+    const long long n = (long long)nbf * nbf;
+    hipLaunchKernelGGL(k_finalize, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, X, Y, fnorm,
+                       nbf, J, K);
+    SF_HIP(hipGetLastError());
+    return kOk;
+}
+
+__global__ void k_fill(double* __restrict__ p, long long n, double v)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+int launch_fill(double* p, long long n, double v, hipStream_t s)
+{
+    if (n <= 0) return kOk;
+    hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
+    SF_HIP(hipGetLastError());
+    return kOk;
+}
+
+// -------------------------------------- component quartets (rys.eri_rys) --
+// One thread per contracted Cartesian-component quartet, with the argument
+// meaning of rys.eri_rys (src/hf/core/core.f90:967-1002): per-primitive
+// normalisation factors and contraction coefficients, per-function
+// (lx, ly, lz).  Total angular momentum <= 9 (5 nodes), like root5.
+template <int N>
+__device__ double component_prim(const int* l, const double* xyz, const double* ex)
+{
+    const double a = ex[0], b = ex[1], c = ex[2], d = ex[3];
+    const double p = a + b, q = c + d, pq = p + q, ipq = 1.0 / pq, rho = p * q * ipq;
+    double P[3], Q[3], PQ2 = 0.0, Kab = 0.0, Kcd = 0.0;
+    for (int k = 0; k < 3; ++k) {
+        P[k] = (a * xyz[k] + b * xyz[3 + k]) / p;
+        Q[k] = (c * xyz[6 + k] + d * xyz[9 + k]) / q;
+        PQ2 += (P[k] - Q[k]) * (P[k] - Q[k]);
+        Kab += (xyz[k] - xyz[3 + k]) * (xyz[k] - xyz[3 + k]);
+        Kcd += (xyz[6 + k] - xyz[9 + k]) * (xyz[6 + k] - xyz[9 + k]);
+    }
+    const double pref = 34.986836655249725 * exp(-a * b / p * Kab - c * d / q * Kcd) / (p * q * sqrt(pq));
+    double u[N], w[N];
+    rys_roots<N>(rho * PQ2, u, w);
+    double res = 0.0;
+    for (int r = 0; r < N; ++r) {
+        const double t2 = u[r];
+        const double b00 = 0.5 * ipq * t2, b10 = 0.5 / p * (1.0 - q * ipq * t2),
+                     b01 = 0.5 / q * (1.0 - p * ipq * t2);
+        double prod = pref * w[r];
+        for (int k = 0; k < 3; ++k) {
+            const int li = l[k], lj = l[3 + k], lk = l[6 + k], ll = l[9 + k];
+            const int nn = li + lj, mm = lk + ll;
+            double G[10][10];
+            const double c00 = (P[k] - xyz[k]) - q * ipq * (P[k] - Q[k]) * t2;
+            const double c00p = (Q[k] - xyz[6 + k]) + p * ipq * (P[k] - Q[k]) * t2;
+            G[0][0] = 1.0;
+            if (nn > 0) G[1][0] = c00;
+            for (int n = 1; n < nn; ++n) G[n + 1][0] = n * b10 * G[n - 1][0] + c00 * G[n][0];
+            for (int m = 0; m < mm; ++m)
+                for (int n = 0; n <= nn; ++n) {
+                    double g = c00p * G[n][m];
+                    if (m > 0) g += m * b01 * G[n][m - 1];
+                    if (n > 0) g += n * b00 * G[n - 1][m];
+                    G[n][m + 1] = g;
+                }
+            const double xij = xyz[k] - xyz[3 + k], xkl = xyz[6 + k] - xyz[9 + k];
+            double s = 0.0;
+            for (int m = 0; m <= ll; ++m) {
+                double t = 0.0;
+                for (int n = 0; n <= lj; ++n)
+                    t += binom_c(lj, n) * pow(xij, (double)(lj - n)) * G[n + li][m + lk];
+                s += binom_c(ll, m) * pow(xkl, (double)(ll - m)) * t;
+            }
+            prod *= s;
+        }
+        res += prod;
+    }
+    return res;
+}
+
+__global__ __launch_bounds__(128) void k_eri_components(
+    const int* __restrict__ npgf, const int* __restrict__ poff, const double* __restrict__ coefs,
+    const double* __restrict__ expns, const double* __restrict__ norms,
+    const int* __restrict__ shell, const double* __restrict__ center, long long nq,
+    const int* __restrict__ quartets, double* __restrict__ out)
+{
+    const long long qi = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (qi >= nq) return;
+    int f[4], l[12], L = 0;
+    double xyz[12];
+    for (int k = 0; k < 4; ++k) {
+        f[k] = quartets[4 * qi + k];
+        for (int e = 0; e < 3; ++e) {
+            l[3 * k + e] = shell[3 * f[k] + e];
+            xyz[3 * k + e] = center[3 * f[k] + e];
+            L += l[3 * k + e];
+        }
+    }
+    const int nr = L / 2 + 1;
+    double res = 0.0;
+    for (int i1 = 0; i1 < npgf[f[0]]; ++i1)
+        for (int i2 = 0; i2 < npgf[f[1]]; ++i2)
+            for (int i3 = 0; i3 < npgf[f[2]]; ++i3)
+                for (int i4 = 0; i4 < npgf[f[3]]; ++i4) {
+                    const int p1 = poff[f[0]] + i1, p2 = poff[f[1]] + i2, p3 = poff[f[2]] + i3,
+                              p4 = poff[f[3]] + i4;
+                    const double ex[4] = {expns[p1], expns[p2], expns[p3], expns[p4]};
+                    double v;
+                    switch (nr) {
+                    case 1: v = component_prim<1>(l, xyz, ex); break;
+                    case 2: v = component_prim<2>(l, xyz, ex); break;
+                    case 3: v = component_prim<3>(l, xyz, ex); break;
+                    case 4: v = component_prim<4>(l, xyz, ex); break;
+                    default: v = component_prim<5>(l, xyz, ex); break;
+                    }
+                    res += norms[p1] * norms[p2] * norms[p3] * norms[p4] * coefs[p1] * coefs[p2] *
+                           coefs[p3] * coefs[p4] * v;
+                }
+    out[qi] = res;
+}
+
+int launch_eri_components(int nf, const int* npgf, const int* poff, const double* coefs,
+                          const double* expns, const double* norms, const int* shell,
+                          const double* center, long long nq, const int* quartets, double* out,
+                          hipStream_t s)
+{
+    (void)nf;
+    if (nq <= 0) return kOk;
+    hipLaunchKernelGGL(k_eri_components, dim3((unsigned)((nq + 127) / 128)), dim3(128), 0, s, npgf,
+                       poff, coefs, expns, norms, shell, center, nq, quartets, out);
+    SF_HIP(hipGetLastError());
+    return kOk;
+}
+
+}  // namespace sf

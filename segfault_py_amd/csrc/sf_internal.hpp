@@ -1,0 +1,109 @@
+// sf_internal.hpp -- launch parameter blocks and launcher declarations shared
+// by the kernel translation units and the C-ABI layer.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "eri_core.hpp"
+
+namespace sf {
+
+// Status codes returned by every C-ABI entry point.
+enum Status : int {
+    kOk = 0,
+    kEInval = 1,      // bad argument
+    kEHip = 2,        // HIP runtime error
+    kENoMem = 3,      // allocation failed
+    kEState = 4,      // call made in the wrong order (e.g. no screening yet)
+    kEUnsupported = 5 // angular momentum or size beyond what is built
+};
+
+void set_error(int code, const char* fmt, ...);
+
+#define SF_HIP(x)                                                                        \
+    do {                                                                                 \
+        hipError_t e_ = (x);                                                             \
+        if (e_ != hipSuccess) {                                                          \
+            ::sf::set_error(::sf::kEHip, "%s:%d %s: %s", __FILE__, __LINE__, #x,         \
+                            hipGetErrorString(e_));                                      \
+            return ::sf::kEHip;                                                          \
+        }                                                                                \
+    } while (0)
+
+constexpr int kMaxL = 2;           // s, p, d shells
+constexpr int kNumPairClass = 6;   // (00) (10) (11) (20) (21) (22)
+constexpr int kNumQuartetClass = 21;
+
+constexpr int pair_class(int la, int lb) { return la * (la + 1) / 2 + lb; }  // la >= lb
+constexpr int pair_la(int pc) { return pc == 0 ? 0 : pc < 3 ? 1 : 2; }
+constexpr int pair_lb(int pc) { return pc - pair_la(pc) * (pair_la(pc) + 1) / 2; }
+constexpr int quartet_class(int pcb, int pck) { return pcb * (pcb + 1) / 2 + pck; }  // pcb >= pck
+
+// One sorted pair class on the device.
+struct PairListDev {
+    const int* sa;    // first shell (L(sa) >= L(sb))
+    const int* sb;
+    const int* poff;  // first primitive pair
+    const int* np;    // number of primitive pairs
+    int n;
+};
+
+// One launch of a quartet-class kernel over a precision segment.
+struct ClassParams {
+    PairListDev bra, ket;
+    const PrimPair* prim;
+    const double* shell_xyz;   // 3 per shell
+    const int* shell_foff;     // first function of each shell
+    const long long* seg_base; // nbra+1: flat segment index of each bra's first quartet
+    const int* seg_k0;         // nbra: first ket position of the segment for each bra
+    const long long* qall_base;// nbra+1: class-wide quartet index of ket 0 of each bra
+    long long nq;              // quartets in this segment
+    long long qstride;         // quartets in the class (plane stride of out)
+    int same;                  // bra and ket pair classes coincide
+    int nbf;
+    const double* P;           // density in the shell-normalised basis
+    double* out;               // SoA planes: out[k * qstride + q]
+};
+
+enum EriMode : int { kModeJK = 0, kModeRaw = 1 };
+
+// Digestion tile plane offsets of one quartet class.
+struct ClassTab {
+    const double* out;
+    long long qstride;
+    int off[6];
+    int pad;
+};
+
+// Launchers (k_eri.hip).
+int launch_eri_class(int qcls, int fp32, int mode, const ClassParams& cp, hipStream_t s);
+int launch_schwarz(int pc, const int* sa, const int* sb, const int* poff, const int* np, int n,
+                   const PrimPair* prim, const double* shell_xyz, double* Q, hipStream_t s);
+int class_ntile(int qcls);
+int class_nint(int qcls);
+void class_tile_offsets(int qcls, int off[6]);
+
+// Launchers (k_util.hip).
+int launch_boys(long long n, int mmax, const double* T, double* F, hipStream_t s);
+int launch_rys(long long n, int nroots, const double* X, double* u, double* w, hipStream_t s);
+int launch_count_kets(const double* Qbra, int nbra, const double* Qket, int nket, int same,
+                      double tau, double tau_mixed, int* n_all, int* n_fp64, hipStream_t s);
+int launch_plan_entries(int qcls, const PairListDev& bra, const PairListDev& ket,
+                        const long long* qall_base, long long nq, int nshell,
+                        unsigned int* keys, unsigned long long* vals, hipStream_t s);
+int launch_segments(const unsigned int* keys, long long n, long long* seg_begin,
+                    long long* seg_end, hipStream_t s);
+int launch_reduce(int n1, int n2, const int* dest_keys, int ndest, const long long* seg_begin,
+                  const long long* seg_end, const unsigned long long* vals, const ClassTab* tab,
+                  const int* shell_foff, int nshell, int nbf, double* X, double* Y, hipStream_t s);
+int launch_scale_density(const double* P, const double* fnorm, int nbf, double* Ps, hipStream_t s);
+int launch_finalize(const double* X, const double* Y, const double* fnorm, int nbf, double* J,
+                    double* K, hipStream_t s);
+int launch_fill(double* p, long long n, double v, hipStream_t s);
+int launch_eri_components(int nf, const int* npgf, const int* poff, const double* coefs,
+                          const double* expns, const double* norms, const int* shell,
+                          const double* center, long long nq, const int* quartets, double* out,
+                          hipStream_t s);
+
+}  // namespace sf

@@ -1,0 +1,153 @@
+"""CPU model of the GPU J/K pipeline, for tests.
+
+Drives the kernel math headers compiled for the host (tests/harness) through
+the same steps as libsfeng's sf_screen / sf_jk: canonical shell pairs,
+Schwarz factors, Q-sorted significant pairs, per-bra ket prefixes, digestion
+tiles, and J = D (X + X^T) D, K = D (Y + Y^T) D.  It checks the algorithm
+(degeneracy factors, tile bookkeeping, list construction) against the
+oracle on the CPU, and gives the CPU list builder that the GPU lists must
+match bit for bit.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from conftest import build_harness
+
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+PAIR_CLASSES = [(0, 0), (1, 0), (1, 1), (2, 0), (2, 1), (2, 2)]
+_H = None
+
+
+def harness():
+    global _H
+    if _H is None:
+        H = ctypes.CDLL(build_harness())
+        H.h_eri.argtypes = [ctypes.c_int] * 4 + [_f64p, ctypes.c_int, _f64p, ctypes.c_int, _f64p,
+                                                 _f64p, _f64p, _f64p, _f64p, ctypes.c_int]
+        H.h_digest.argtypes = [ctypes.c_int] * 4 + [_f64p, _f64p, ctypes.c_int, _i32p, _f64p]
+        H.h_rys.argtypes = [ctypes.c_int, ctypes.c_double, _f64p, _f64p]
+        H.h_boys.argtypes = [ctypes.c_int, ctypes.c_double, _f64p]
+        H.h_boys01.argtypes = [ctypes.c_double, _f64p]
+        _H = H
+    return _H
+
+
+def ncart(L):
+    return (L + 1) * (L + 2) // 2
+
+
+def prim_pairs(sa, sb):
+    A, B = sa.center, sb.center
+    AB2 = float(((A - B) ** 2).sum())
+    out = []
+    for a, ca in zip(sa.expns, sa.coefs):
+        for b, cb in zip(sb.expns, sb.coefs):
+            p = a + b
+            P = (a * A + b * B) / p
+            out.append([p, *P, ca * cb * np.exp(-a * b / p * AB2), 0.0, 0.0, 0.0])
+    return np.ascontiguousarray(np.array(out).reshape(-1))
+
+
+def shell_quartet(shells, a, b, c, d, fp32=False):
+    S = [shells[a], shells[b], shells[c], shells[d]]
+    out = np.zeros(int(np.prod([s.nfunc for s in S])))
+    rc = harness().h_eri(S[0].L, S[1].L, S[2].L, S[3].L, prim_pairs(S[0], S[1]),
+                         S[0].nprim * S[1].nprim, prim_pairs(S[2], S[3]), S[2].nprim * S[3].nprim,
+                         S[0].center, S[1].center, S[2].center, S[3].center, out, int(fp32))
+    assert rc == 0
+    return out
+
+
+def canonical_pairs(shells):
+    """All pairs a >= b, oriented so L(sa) >= L(sb); grouped by pair class."""
+    groups = {pc: [] for pc in range(6)}
+    for a in range(len(shells)):
+        for b in range(a + 1):
+            sa, sb = (a, b) if shells[a].L >= shells[b].L else (b, a)
+            la, lb = shells[sa].L, shells[sb].L
+            groups[la * (la + 1) // 2 + lb].append((sa, sb))
+    return groups
+
+
+def schwarz(shells, groups):
+    Q = {}
+    for pc, pairs in groups.items():
+        for sa, sb in pairs:
+            v = shell_quartet(shells, sa, sb, sa, sb)
+            na, nb = shells[sa].nfunc, shells[sb].nfunc
+            v = v.reshape(na, nb, na, nb)
+            Q[(sa, sb)] = float(np.sqrt(max(abs(v[i, j, i, j]) for i in range(na) for j in range(nb))))
+    return Q
+
+
+def sorted_significant(groups, Q, tau):
+    qmax = max(Q.values())
+    out = {}
+    for pc, pairs in groups.items():
+        keep = [p for p in pairs if Q[p] * qmax >= tau]
+        keep = sorted(keep, key=lambda p: -Q[p])  # stable: ties keep input order
+        out[pc] = keep
+    return out
+
+
+def ket_counts(Qb, Qk, same, tau, tau_mixed):
+    """The CPU quartet-list builder: per bra, surviving ket prefix length and
+    its FP64 part (the GPU builder must agree exactly)."""
+    Qb = np.asarray(Qb, dtype=np.float64)
+    Qk = np.asarray(Qk, dtype=np.float64)
+    n_all = np.zeros(len(Qb), dtype=np.int32)
+    n64 = np.zeros(len(Qb), dtype=np.int32)
+    for i, qi in enumerate(Qb):
+        prod = qi * Qk
+        n = int(np.sum(prod >= tau))
+        # prefix property: products are non-increasing along the sorted kets
+        assert np.all(prod[:n] >= tau)
+        if same:
+            n = min(n, i + 1)
+        m = n
+        if tau_mixed > tau:
+            m = min(int(np.sum(prod >= tau_mixed)), n)
+        n_all[i], n64[i] = n, m
+    return n_all, n64
+
+
+def jk_model(mol, P, tau=1e-12, tau_mixed=0.0, fp32_below=False):
+    shells = mol.shells()
+    fn = mol.function_norms()
+    groups = canonical_pairs(shells)
+    Q = schwarz(shells, groups)
+    lists = sorted_significant(groups, Q, tau)
+    n = mol.nbf
+    Ps = np.ascontiguousarray(P * fn[:, None] * fn[None, :])
+    X = np.zeros((n, n))
+    Y = np.zeros((n, n))
+    for pcb in range(6):
+        for pck in range(pcb + 1):
+            bra, ket = lists[pcb], lists[pck]
+            if not bra or not ket:
+                continue
+            same = pcb == pck
+            n_all, n64 = ket_counts([Q[p] for p in bra], [Q[p] for p in ket], same, tau, tau_mixed)
+            for i, (a, b) in enumerate(bra):
+                for j in range(n_all[i]):
+                    c, d = ket[j]
+                    v = shell_quartet(shells, a, b, c, d, fp32=fp32_below and j >= n64[i])
+                    fac = (0.5 if a == b else 1.0) * (0.5 if c == d else 1.0) * (0.5 if same and i == j else 1.0)
+                    v = np.ascontiguousarray(v * fac)
+                    Ls = [shells[s].L for s in (a, b, c, d)]
+                    nA, nB, nC, nD = [ncart(L) for L in Ls]
+                    t = np.zeros(nA * nB + nC * nD + nA * nC + nA * nD + nB * nC + nB * nD)
+                    f0 = np.array([shells[s].offset for s in (a, b, c, d)], dtype=np.int32)
+                    assert harness().h_digest(*Ls, v, Ps, n, f0, t) == 0
+                    o = 0
+                    for (s1, s2, n1, n2, M) in ((a, b, nA, nB, X), (c, d, nC, nD, X), (a, c, nA, nC, Y),
+                                                (a, d, nA, nD, Y), (b, c, nB, nC, Y), (b, d, nB, nD, Y)):
+                        r, q0 = shells[s1].offset, shells[s2].offset
+                        M[r:r + n1, q0:q0 + n2] += t[o:o + n1 * n2].reshape(n1, n2)
+                        o += n1 * n2
+    D = fn[:, None] * fn[None, :]
+    return D * (X + X.T), D * (Y + Y.T)

@@ -1,0 +1,77 @@
+// CPU harness: compiles the HIP kernels' host+device math headers with g++
+// so that tests can check them against the oracle without a GPU.
+// Test infrastructure only.
+#include "../../segfault_py_amd/csrc/eri_core.hpp"
+
+using namespace sf;
+
+extern "C" void h_boys(int mmax, double X, double* F) { boys_grid(mmax, X, F); }
+extern "C" void h_boys01(double X, double* F) { boys01(X, F[0], F[1]); }
+
+extern "C" int h_rys(int n, double X, double* u, double* w)
+{
+    switch (n) {
+    case 1: rys_roots<1>(X, u, w); return 0;
+    case 2: rys_roots<2>(X, u, w); return 0;
+    case 3: rys_roots<3>(X, u, w); return 0;
+    case 4: rys_roots<4>(X, u, w); return 0;
+    case 5: rys_roots<5>(X, u, w); return 0;
+    }
+    return 1;
+}
+
+template <int LA, int LB, int LC, int LD>
+static void run(const double* bra, int nb, const double* ket, int nk, const double* A,
+                const double* B, const double* C, const double* D, double* out, int fp32)
+{
+    using S = QuartetShape<LA, LB, LC, LD>;
+    if (fp32) {
+        float acc[S::NINT] = {};
+        eri_shell_quartet<LA, LB, LC, LD, float>((const PrimPair*)bra, nb, (const PrimPair*)ket, nk,
+                                                 A, B, C, D, acc);
+        for (int i = 0; i < S::NINT; ++i) out[i] = acc[i];
+    } else {
+        double acc[S::NINT] = {};
+        eri_shell_quartet<LA, LB, LC, LD, double>((const PrimPair*)bra, nb, (const PrimPair*)ket, nk,
+                                                  A, B, C, D, acc);
+        for (int i = 0; i < S::NINT; ++i) out[i] = acc[i];
+    }
+}
+
+template <int LA, int LB, int LC, int LD>
+static void run_digest(const double* v, const double* P, int nbf, const int* f0, double* t)
+{
+    digest_quartet<LA, LB, LC, LD, double>(v, P, nbf, f0[0], f0[1], f0[2], f0[3], t);
+}
+
+#define SF_CLASSES(X)                                                                        \
+    X(0, 0, 0, 0) X(1, 0, 0, 0) X(1, 0, 1, 0) X(1, 1, 0, 0) X(1, 1, 1, 0) X(1, 1, 1, 1)       \
+    X(2, 0, 0, 0) X(2, 0, 1, 0) X(2, 0, 1, 1) X(2, 0, 2, 0) X(2, 1, 0, 0) X(2, 1, 1, 0)     \
+    X(2, 1, 1, 1) X(2, 1, 2, 0) X(2, 1, 2, 1) X(2, 2, 0, 0) X(2, 2, 1, 0) X(2, 2, 1, 1)     \
+    X(2, 2, 2, 0) X(2, 2, 2, 1) X(2, 2, 2, 2)
+
+extern "C" int h_eri(int la, int lb, int lc, int ld, const double* bra, int nb, const double* ket, int nk,
+          const double* A, const double* B, const double* C, const double* D, double* out, int fp32)
+{
+#define X(a, b, c, d)                                                          \
+    if (la == a && lb == b && lc == c && ld == d) {                            \
+        run<a, b, c, d>(bra, nb, ket, nk, A, B, C, D, out, fp32);              \
+        return 0;                                                              \
+    }
+    SF_CLASSES(X)
+#undef X
+    return 1;
+}
+
+extern "C" int h_digest(int la, int lb, int lc, int ld, const double* v, const double* P, int nbf,
+             const int* f0, double* t)
+{
+#define X(a, b, c, d)                                      \
+    if (la == a && lb == b && lc == c && ld == d) {        \
+        run_digest<a, b, c, d>(v, P, nbf, f0, t);          \
+        return 0;                                          \
+    }
+    SF_CLASSES(X)
+#undef X
+    return 1;
+}

@@ -1,0 +1,136 @@
+"""The HIP kernels' math headers (sf_math.hpp, eri_core.hpp) compiled for
+the CPU and checked against mpmath golden vectors and the oracle, and the
+J/K pipeline modelled on the CPU against the oracle's dense einsum."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import cpu_model as CM
+from conftest import GOLDEN
+
+
+def _golden(name):
+    with open(os.path.join(GOLDEN, name)) as fh:
+        return json.load(fh)
+
+
+def test_rys_tables_match_mpmath():
+    H = CM.harness()
+    worst_u = worst_w = 0.0
+    for pt in _golden("rys_mpmath.json")["points"]:
+        n = pt["n"]
+        u, w = np.zeros(n), np.zeros(n)
+        assert H.h_rys(n, float(pt["X"]), u, w) == 0
+        ur = np.array([float(v) for v in pt["u"]])
+        wr = np.array([float(v) for v in pt["w"]])
+        worst_u = max(worst_u, float(np.max(np.abs(u - ur) / ur)))
+        worst_w = max(worst_w, float(np.max(np.abs(w - wr) / wr)))
+    assert worst_u < 1e-14 and worst_w < 1e-14, (worst_u, worst_w)
+
+
+def test_boys_grid_matches_mpmath():
+    H = CM.harness()
+    worst = 0.0
+    for pt in _golden("boys_mpmath.json")["points"]:
+        F = np.zeros(25)
+        H.h_boys(24, float(pt["T"]), F)
+        ref = np.array([float(v) for v in pt["F"]])
+        worst = max(worst, float(np.max(np.abs(F - ref) / ref)))
+    assert worst < 1e-14, worst  # 24 recursion steps of rounding
+
+
+def test_boys01_closed_form_matches_mpmath():
+    H = CM.harness()
+    worst = 0.0
+    for pt in _golden("boys_mpmath.json")["points"]:
+        F = np.zeros(2)
+        H.h_boys01(float(pt["T"]), F)
+        ref = np.array([float(v) for v in pt["F"][:2]])
+        worst = max(worst, float(np.max(np.abs(F - ref) / ref)))
+    assert worst < 5e-15, worst
+
+
+def _check_quartets(oracle, mol, nsample, seed, fp32=False):
+    import itertools
+    shells = mol.shells()
+    fn = mol.function_norms()
+    rng = np.random.default_rng(seed)
+    worst = 0.0
+    seen = set()
+    # one quartet per angular class present (last shell of each L, so
+    # centres differ where the basis allows), then random quartets
+    last = {}
+    for k, sh in enumerate(shells):
+        last[sh.L] = k
+    picks = []
+    for la in last:
+        for lb in last:
+            for lc in last:
+                for ld in last:
+                    picks.append((last[la], last[lb], last[lc], last[ld]))
+    picks += [tuple(int(x) for x in rng.integers(0, len(shells), 4)) for _ in range(nsample)]
+    for a, b, c, d in picks:
+        if shells[a].L < shells[b].L:
+            a, b = b, a
+        if shells[c].L < shells[d].L:
+            c, d = d, c
+        if (shells[a].L, shells[b].L) < (shells[c].L, shells[d].L):
+            a, b, c, d = c, d, a, b
+        seen.add(tuple(shells[s].L for s in (a, b, c, d)))
+        v = CM.shell_quartet(shells, a, b, c, d, fp32=fp32)
+        idx = np.array([(shells[a].offset + i, shells[b].offset + j, shells[c].offset + k,
+                         shells[d].offset + l) for i, j, k, l in itertools.product(
+                            *[range(shells[s].nfunc) for s in (a, b, c, d)])])
+        ref = oracle.eri_list(mol, idx, 8)
+        got = v * fn[idx[:, 0]] * fn[idx[:, 1]] * fn[idx[:, 2]] * fn[idx[:, 3]]
+        worst = max(worst, float(np.max(np.abs(got - ref))))
+    return worst, seen
+
+
+def test_shell_quartets_match_oracle_sp(oracle, water_sp):
+    worst, _ = _check_quartets(oracle, water_sp, 200, 0)
+    assert worst < 1e-12, worst
+
+
+def test_shell_quartets_match_oracle_spd(oracle, water_ccpvdz):
+    worst, seen = _check_quartets(oracle, water_ccpvdz, 400, 1)
+    assert worst < 1e-12, worst
+    assert (2, 2, 2, 2) in seen and (2, 1, 1, 0) in seen
+
+
+def test_fp32_quartets_error_bound(oracle, water_ccpvdz):
+    worst, _ = _check_quartets(oracle, water_ccpvdz, 150, 2, fp32=True)
+    # FP32 recurrences: relative ~1e-7 of integrals of size <= ~5
+    assert worst < 5e-6, worst
+
+
+def _density(n, seed):
+    rng = np.random.default_rng(seed)
+    C = rng.standard_normal((n, n // 3 + 1))
+    return np.ascontiguousarray(C @ C.T / n)
+
+
+@pytest.mark.parametrize("fixture", ["water_sp", "water_ccpvdz"])
+def test_jk_pipeline_model_matches_einsum(oracle, request, fixture):
+    mol = request.getfixturevalue(fixture)
+    eri = oracle.eri_tensor(mol, 8)
+    P = _density(mol.nbf, 7)
+    Jr, Kr = oracle.jk_dense(eri, P)
+    J, K = CM.jk_model(mol, P, tau=0.0)
+    assert np.max(np.abs(J - Jr)) < 1e-11
+    assert np.max(np.abs(K - Kr)) < 1e-11
+    # screened at 1e-12: deviation bounded by what was dropped
+    J2, K2 = CM.jk_model(mol, P, tau=1e-12)
+    assert np.max(np.abs(J2 - Jr)) < 1e-9 and np.max(np.abs(K2 - Kr)) < 1e-9
+
+
+def test_ket_count_builder_prefix_and_caps():
+    Qb = np.array([3.0, 2.0, 1e-3, 1e-7])
+    Qk = np.array([3.0, 2.0, 1e-3, 1e-7])
+    n_all, n64 = CM.ket_counts(Qb, Qk, True, 1e-9, 1e-5)
+    assert list(n_all) == [1, 2, 3, 2]
+    assert list(n64) == [1, 2, 2, 0]
+    n_all, n64 = CM.ket_counts(Qb, Qk[:2], False, 1e-9, 0.0)
+    assert list(n_all) == [2, 2, 2, 2] and list(n64) == list(n_all)

@@ -1,0 +1,214 @@
+"""Pins of the CPU oracle (oracle/sf_oracle.c) before it is trusted as the
+checker of the GPU path.
+
+Each oracle function is checked against something it does not compute
+itself: mpmath golden vectors, closed forms, invariants, an independent
+second method, and the reference's published SCF energies
+(src/hf/benchmark/molecule_test.json, copied to tests/golden/).
+"""
+import itertools
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+
+
+def _golden(name):
+    with open(os.path.join(GOLDEN, name)) as fh:
+        return json.load(fh)
+
+
+# ---------------------------------------------------------------- Boys ----
+def test_boys_matches_mpmath(oracle):
+    g = _golden("boys_mpmath.json")
+    worst = 0.0
+    for pt in g["points"]:
+        F = oracle.boys(24, float(pt["T"]))
+        ref = np.array([float(v) for v in pt["F"]])
+        worst = max(worst, float(np.max(np.abs(F - ref) / ref)))
+    assert worst < 5e-15, worst
+
+
+def test_boys_closed_forms(oracle):
+    # F_m(0) = 1/(2m+1); F_0(T) = sqrt(pi/T) erf(sqrt T)/2; large-T asymptote
+    F = oracle.boys(12, 0.0)
+    assert np.allclose(F, 1.0 / (2 * np.arange(13) + 1), rtol=0, atol=1e-16)
+    for T in (0.2, 1.7, 12.0, 31.0):
+        assert abs(oracle.boys(0, T)[0] - 0.5 * math.sqrt(math.pi / T) * math.erf(math.sqrt(T))) < 1e-15
+    T = 200.0
+    F = oracle.boys(6, T)
+    for m in range(7):
+        asym = math.prod(range(2 * m - 1, 0, -2)) / 2 ** (m + 1) * math.sqrt(math.pi / T ** (2 * m + 1))
+        assert abs(F[m] - asym) <= 1e-14 * asym
+
+
+def test_boys_recursions(oracle):
+    # downward recursion F_{m-1} = (2T F_m + e^-T)/(2m-1) holds to rounding
+    for T in (0.01, 3.3, 29.0, 45.0):
+        F = oracle.boys(10, T)
+        for m in range(1, 11):
+            assert abs(F[m - 1] - (2 * T * F[m] + math.exp(-T)) / (2 * m - 1)) <= 4e-16 * F[m - 1]
+
+
+# ------------------------------------------------------------------ Rys ----
+def test_rys_matches_mpmath(oracle):
+    g = _golden("rys_mpmath.json")
+    worst_u = worst_w = 0.0
+    for pt in g["points"]:
+        u, w = oracle.rys_roots(pt["n"], float(pt["X"]))
+        ur = np.array([float(v) for v in pt["u"]])
+        wr = np.array([float(v) for v in pt["w"]])
+        worst_u = max(worst_u, float(np.max(np.abs(u - ur) / ur)))
+        worst_w = max(worst_w, float(np.max(np.abs(w - wr) / wr)))
+    assert worst_u < 2e-13 and worst_w < 2e-13, (worst_u, worst_w)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 6])
+def test_rys_moment_identities(oracle, n):
+    for X in (0.0, 1e-9, 0.7, 6.0, 22.0, 58.0, 75.0, 400.0):
+        u, w = oracle.rys_roots(n, X)
+        F = oracle.boys(2 * n - 1, X)
+        assert np.all(np.diff(u) > 0) and np.all(u > 0) and np.all(u < 1) and np.all(w > 0)
+        for k in range(2 * n):
+            assert abs(np.sum(w * u ** k) - F[k]) <= 3e-13 * F[k], (n, X, k)
+
+
+# ---------------------------------------------------------- primitives ----
+def _rand_prim(rng, ls):
+    xyz = rng.uniform(-1.5, 1.5, 12)
+    ex = rng.uniform(0.2, 3.0, 4)
+    return np.array(ls, dtype=np.int32), np.ascontiguousarray(xyz), np.ascontiguousarray(ex)
+
+
+def test_ssss_closed_form(oracle):
+    rng = np.random.default_rng(1)
+    for _ in range(50):
+        l, xyz, ex = _rand_prim(rng, [0] * 12)
+        a, b, c, d = ex
+        A, B, C, D = xyz.reshape(4, 3)
+        p, q = a + b, c + d
+        P, Q = (a * A + b * B) / p, (c * C + d * D) / q
+        rho = p * q / (p + q)
+        T = rho * np.sum((P - Q) ** 2)
+        F0 = 0.5 * math.sqrt(math.pi / T) * math.erf(math.sqrt(T)) if T > 0 else 1.0
+        ref = (2 * math.pi ** 2.5 / (p * q * math.sqrt(p + q)) * math.exp(-a * b / p * np.sum((A - B) ** 2))
+               * math.exp(-c * d / q * np.sum((C - D) ** 2)) * F0)
+        got = oracle.lib().orc_eri_3d_rys(l, xyz, ex)
+        assert abs(got - ref) <= 1e-14 * abs(ref)
+
+
+def test_hermite_e_overlap_quadrature(oracle):
+    # E_0^{ij} sqrt(pi/p) = int (x-A)^i (x-B)^j e^{-a(x-A)^2 - b(x-B)^2} dx (Gauss-Hermite)
+    xs, ws = np.polynomial.hermite.hermgauss(40)
+    rng = np.random.default_rng(2)
+    for _ in range(30):
+        i, j = rng.integers(0, 4, 2)
+        a, b = rng.uniform(0.3, 2.0, 2)
+        A, B = rng.uniform(-1, 1, 2)
+        p = a + b
+        P = (a * A + b * B) / p
+        x = P + xs / math.sqrt(p)
+        f = (x - A) ** i * (x - B) ** j * math.exp(-a * b / p * (A - B) ** 2)
+        ref = np.sum(ws * f) / math.sqrt(p)
+        got = oracle.lib().orc_hermite_e(int(i), int(j), 0, A - B, a, b) * math.sqrt(math.pi / p)
+        assert abs(got - ref) <= 1e-13 * max(1.0, abs(ref))
+
+
+def test_coulomb_r_derivative(oracle):
+    # R_{100} = d/dPCx R_{000}: central difference of (-2p)^0 F_0(p R^2)
+    p, pc = 0.9, np.array([0.3, -0.4, 0.7])
+    h = 1e-5
+    L = oracle.lib()
+
+    def r000(v):
+        return L.orc_coulomb_r(0, 0, 0, 0, p, *v, float(np.linalg.norm(v)))
+
+    for axis, (t, u, v) in enumerate([(1, 0, 0), (0, 1, 0), (0, 0, 1)]):
+        e = np.zeros(3)
+        e[axis] = h
+        fd = (r000(pc + e) - r000(pc - e)) / (2 * h)
+        got = L.orc_coulomb_r(t, u, v, 0, p, *pc, float(np.linalg.norm(pc)))
+        assert abs(got - fd) < 1e-9
+
+
+def test_rys_equals_mmd(oracle, water_ccpvdz):
+    # two independent restated engines (rys.eri_rys vs mmd.eri) agree
+    cg = water_ccpvdz.cgfs
+    rng = np.random.default_rng(3)
+    worst = 0.0
+    for _ in range(60):
+        i, j, k, l = rng.integers(0, len(cg), 4)
+        r = oracle.eri_rys(cg[i], cg[j], cg[k], cg[l])
+        m = oracle.eri_mmd(cg[i], cg[j], cg[k], cg[l])
+        worst = max(worst, abs(r - m))
+    assert worst < 1e-13, worst
+
+
+def test_eri_tensor_symmetry_and_schwarz(oracle, water_sp):
+    eri = oracle.eri_tensor(water_sp, 8)
+    assert np.array_equal(eri, eri.transpose(1, 0, 2, 3))
+    assert np.array_equal(eri, eri.transpose(2, 3, 0, 1))
+    n = eri.shape[0]
+    diag = np.einsum("ijij->ij", eri)
+    Q = np.sqrt(np.abs(diag))
+    assert np.all(np.abs(eri) <= Q[:, :, None, None] * Q[None, None, :, :] * (1 + 1e-12) + 1e-150)  # (kl|kl) underflows first
+    assert n == 16
+
+
+def test_eri_translation_invariance(oracle, water_sp):
+    from segfault_py_amd.molecule import Molecule
+    from segfault_py_amd import workloads as W
+    shifted = Molecule.from_geometry(list("OHH"), W.water_geometry() + np.array([3.1, -2.0, 0.7]),
+                                     W.even_tempered_text(W.BASIS_SP))
+    # Molecule centres coordinates (molecule.py:68-70); un-centre by hand
+    for a in shifted.atoms:
+        for c in a.cgfs:
+            c.center = c.center + np.array([1.3, 0.2, -0.9])
+    cg0, cg1 = water_sp.cgfs, shifted.cgfs
+    rng = np.random.default_rng(4)
+    for _ in range(40):
+        q = rng.integers(0, len(cg0), 4)
+        a = oracle.eri_rys(*[cg0[t] for t in q])
+        b = oracle.eri_rys(*[cg1[t] for t in q])
+        assert abs(a - b) < 1e-13
+
+
+# ----------------------------------------------------------- end to end ----
+REF_MOLECULES = [("water", "others"), ("methane", "alkane"), ("ammonia", "others")]
+
+
+@pytest.mark.parametrize("name,group", REF_MOLECULES)
+def test_scf_energy_matches_reference(oracle, name, group):
+    """The oracle's ERIs + one-electron integrals + DIIS SCF reproduce the
+    reference's published cc-pVDZ energies (molecule_test.json)."""
+    from segfault_py_amd.molecule import Molecule
+    ref = _golden("molecule_test.json")[group][name]
+    m = Molecule(os.path.join(GOLDEN, "molecules", f"{name}.xyz"),
+                 os.path.join(GOLDEN, "cc-pvdz-HCNO.dat"))
+    assert m.nbf == ref["ncgf"]
+    S, T, V = oracle.one_electron(m, 8)
+    assert np.max(np.abs(np.diag(S) - 1.0)) < 1e-14  # per-component normalisation
+    eri = oracle.eri_tensor(m, 8)
+    nocc = int(m.numbers.sum()) // 2
+    E, _, conv, _ = oracle.scf_diis(S, T + V, nocc, lambda P: oracle.jk_dense(eri, P), m.e_nuc)
+    assert conv
+    assert abs(E - ref["etot"]) < 5e-8, (E, ref["etot"])
+
+
+def test_jk_dense_matches_loops(oracle):
+    rng = np.random.default_rng(5)
+    n = 5
+    eri = rng.standard_normal((n, n, n, n))
+    eri = eri + eri.transpose(1, 0, 2, 3)
+    eri = eri + eri.transpose(0, 1, 3, 2)
+    eri = eri + eri.transpose(2, 3, 0, 1)
+    P = rng.standard_normal((n, n))
+    P = P + P.T
+    J, K = oracle.jk_dense(eri, P)
+    for p, q in itertools.product(range(n), repeat=2):
+        assert abs(J[p, q] - sum(eri[p, q, r, s] * P[r, s] for r in range(n) for s in range(n))) < 1e-12
+        assert abs(K[p, q] - sum(eri[p, r, q, s] * P[r, s] for r in range(n) for s in range(n))) < 1e-12
