@@ -144,6 +144,10 @@ def main():
     _, _, ms = basis.jk_profile(P, 0, out=(J, K))
     kern = [(ms[2 * w["class"]], w) for w in work if w["shellq"] > 0]
     dom_ms, dom = max(kern, key=lambda x: x[0])
+    top = sorted(kern, key=lambda x: -x[0])[:6]
+    kernel_ms = {"eri_" + "".join(map(str, w["ls"])): round(t, 4) for t, w in top}
+    kernel_ms["reduce+finalize"] = round(float(ms[43]), 4)
+    kernel_ms["scale_density"] = round(float(ms[42]), 4)
     achieved = dom["flops"] / (dom_ms * 1e-3) / 1e12
     total_flops = sum(w["flops"] for w in work)
     shellq = sum(w["shellq"] for w in work)
@@ -175,6 +179,7 @@ def main():
         "prim_quartets_per_s_fp64": primq / t64,
         "quartets_per_build": {"function": funcq, "shell": shellq, "primitive": primq},
         "build_tflops_fp64": total_flops / t64 / 1e12,
+        "kernel_ms_fp64": kernel_ms,
         "setup_s": setup_s,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
