@@ -13,9 +13,22 @@
 // and the whole prefactor pi^3/(p q)^1.5 e^{...} 2 sqrt(rho/pi) is carried on
 // the x direction.
 #pragma once
+#include <type_traits>
+
 #include "sf_math.hpp"
 
 namespace sf {
+
+// Calls f(std::integral_constant<int, s>) for a runtime s in [I, N): turns a
+// (wave-uniform) runtime index into a compile-time one.
+template <int I, int N, typename F>
+SF_HD SF_INLINE void static_switch(int s, F&& f)
+{
+    if constexpr (I < N) {
+        if (s == I) f(std::integral_constant<int, I>{});
+        else static_switch<I + 1, N>(s, f);
+    }
+}
 
 // Cartesian component i of angular momentum L in the reference order
 // (molecule.py:138-165): lx descending, then ly descending.
@@ -38,9 +51,13 @@ constexpr double binom_c(int n, int k)
 }
 
 // One primitive pair of a shell pair (a, b): p = a_i + b_j, P = (a_i A + b_j B)/p,
-// K = c_i c_j exp(-a_i b_j/p |A-B|^2) with the radial normalisation in c.
+// K = c_i c_j exp(-a_i b_j/p |A-B|^2) with the radial normalisation in c,
+// and the per-pair constants the kernels would otherwise divide for.
 struct PrimPair {
-    double p, Px, Py, Pz, K, pad0, pad1, pad2;
+    double p, Px, Py, Pz, K;
+    double ip;   // 1/p
+    double Kp;   // K/p
+    double eb;   // exponent b_j of the second shell
 };
 
 template <int LA, int LB, int LC, int LD>
@@ -58,6 +75,15 @@ struct QuartetShape {
     static constexpr int T_BC = T_AD + NA * ND;
     static constexpr int T_BD = T_BC + NB * NC;
     static constexpr int NTILE = T_BD + NB * ND;
+    // classes with more than 81 integrals per quartet run one wavefront per
+    // bra component pair (eri_bra_component) and digest in a second kernel
+    static constexpr bool SPLIT = NINT > 81;
+    static constexpr int tile_n1(int T) { return (T == 0 || T == 2 || T == 3) ? NA : (T == 1) ? NC : NB; }
+    static constexpr int tile_n2(int T) { return (T == 0) ? NB : (T == 2 || T == 4) ? NC : ND; }
+    static constexpr int tile_off(int T)
+    {
+        return T == 0 ? T_AB : T == 1 ? T_CD : T == 2 ? T_AC : T == 3 ? T_AD : T == 4 ? T_BC : T_BD;
+    }
 };
 
 // Rys nodes for N, with the closed-form F0/F1 rule when N == 1.
@@ -99,12 +125,13 @@ SF_HD SF_INLINE void eri_shell_quartet(const PrimPair* __restrict__ bra, int nbr
             const PrimPair pk = ket[ik];
             const double q = pk.p;
             const double pq = p + q;
-            const double ipq = 1.0 / pq;
+            const double rs = 1.0 / sqrt(pq);
+            const double ipq = rs * rs;
             const double rho = p * q * ipq;
             const double PQ[3] = {pb.Px - pk.Px, pb.Py - pk.Py, pb.Pz - pk.Pz};
             const double X = rho * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]);
             // 2 pi^{5/2} / (p q sqrt(p+q)) K_ab K_cd
-            const double pref = 34.986836655249725 * pb.K * pk.K / (p * q * sqrt(pq));
+            const double pref = 34.986836655249725 * pb.Kp * pk.Kp * rs;
             double u[NR], w[NR];
             rys_nodes<NR>(X, u, w);
             const double QC[3] = {pk.Px - C[0], pk.Py - C[1], pk.Pz - C[2]};
@@ -114,8 +141,8 @@ SF_HD SF_INLINE void eri_shell_quartet(const PrimPair* __restrict__ bra, int nbr
                 // 2-D integrals G[n][m], n <= LA+LB, m <= LC+LD (rys.recur)
                 R G[3][NAB][NCD];
                 const R b00 = (R)(0.5 * ipq * t2);
-                const R b10 = (R)(0.5 / p * (1.0 - q * ipq * t2));
-                const R b01 = (R)(0.5 / q * (1.0 - p * ipq * t2));
+                const R b10 = (R)(0.5 * pb.ip * (1.0 - q * ipq * t2));
+                const R b01 = (R)(0.5 * pk.ip * (1.0 - p * ipq * t2));
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
                     const R c00 = (R)(PA[k] - q * ipq * PQ[k] * t2);
@@ -190,6 +217,174 @@ SF_HD SF_INLINE void eri_shell_quartet(const PrimPair* __restrict__ bra, int nbr
             }
         }
     }
+}
+
+// One Cartesian direction of (a_k b_k | c d) for fixed bra exponents
+// AK, BK and all ket exponents ic <= LC, id <= LD: 2-D recurrence up to
+// row AK+BK, ket transfer on rows AK..AK+BK, bra transfer to I[ic][id].
+template <int AK, int BK, int LC, int LD, typename R>
+SF_HD SF_INLINE void dir_component(R g00, R c00, R c00p, R b10, R b01, R b00, R ab,
+                                   const R* CDp, R (&I)[LC + 1][LD + 1])
+{
+    constexpr int NN = AK + BK + 1, NM = LC + LD + 1;
+    R G[NN][NM];
+    G[0][0] = g00;
+    if (NN > 1) G[1][0] = c00 * g00;
+#pragma unroll
+    for (int n = 1; n < NN - 1; ++n) G[n + 1][0] = (R)n * b10 * G[n - 1][0] + c00 * G[n][0];
+    if (NM > 1) {
+#pragma unroll
+        for (int n = 0; n < NN; ++n) {
+            G[n][1] = c00p * G[n][0];
+            if (n > 0) G[n][1] += (R)n * b00 * G[n - 1][0];
+        }
+#pragma unroll
+        for (int m = 1; m < NM - 1; ++m)
+#pragma unroll
+            for (int n = 0; n < NN; ++n) {
+                R g = (R)m * b01 * G[n][m - 1] + c00p * G[n][m];
+                if (n > 0) g += (R)n * b00 * G[n - 1][m];
+                G[n][m + 1] = g;
+            }
+    }
+    R H[BK + 1][LC + 1][LD + 1];
+#pragma unroll
+    for (int nn = 0; nn <= BK; ++nn)
+#pragma unroll
+        for (int ic = 0; ic <= LC; ++ic)
+#pragma unroll
+            for (int id = 0; id <= LD; ++id) {
+                R s = (R)0;
+#pragma unroll
+                for (int m = 0; m <= id; ++m)
+                    s += (R)binom_c(id, m) * CDp[id - m] * G[AK + nn][ic + m];
+                H[nn][ic][id] = s;
+            }
+    R abp[BK + 1];
+    abp[0] = (R)1;
+#pragma unroll
+    for (int e = 1; e <= BK; ++e) abp[e] = abp[e - 1] * ab;
+#pragma unroll
+    for (int ic = 0; ic <= LC; ++ic)
+#pragma unroll
+        for (int id = 0; id <= LD; ++id) {
+            R s = (R)0;
+#pragma unroll
+            for (int nn = 0; nn <= BK; ++nn) s += (R)binom_c(BK, nn) * abp[BK - nn] * H[nn][ic][id];
+            I[ic][id] = s;
+        }
+}
+
+// (fa fb | fc fd) for ONE bra component pair (FA of shell A, FB of shell B)
+// and all ket components, accumulated into acc[NC*ND].  Used where the full
+// class does not fit in registers: one wavefront per bra component pair.
+template <int LA, int LB, int LC, int LD, int FA, int FB, typename R>
+SF_HD SF_INLINE void eri_bra_component(const PrimPair* __restrict__ bra, int nbra,
+                                       const PrimPair* __restrict__ ket, int nket,
+                                       const double* A, const double* B, const double* C,
+                                       const double* D, R* __restrict__ acc)
+{
+    using S = QuartetShape<LA, LB, LC, LD>;
+    constexpr int NR = S::NR;
+    constexpr int ax = cart_l(LA, FA, 0), ay = cart_l(LA, FA, 1), az = cart_l(LA, FA, 2);
+    constexpr int bx = cart_l(LB, FB, 0), by = cart_l(LB, FB, 1), bz = cart_l(LB, FB, 2);
+    R CDp[3][LD + 1];
+    R AB[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const R cd = (R)(C[k] - D[k]);
+        AB[k] = (R)(A[k] - B[k]);
+        CDp[k][0] = (R)1;
+#pragma unroll
+        for (int e = 1; e <= LD; ++e) CDp[k][e] = CDp[k][e - 1] * cd;
+    }
+    for (int ib = 0; ib < nbra; ++ib) {
+        const PrimPair pb = bra[ib];
+        const double p = pb.p;
+        const double PA[3] = {pb.Px - A[0], pb.Py - A[1], pb.Pz - A[2]};
+        for (int ik = 0; ik < nket; ++ik) {
+            const PrimPair pk = ket[ik];
+            const double q = pk.p;
+            const double pq = p + q;
+            const double rs = 1.0 / sqrt(pq);
+            const double ipq = rs * rs;
+            const double rho = p * q * ipq;
+            const double PQ[3] = {pb.Px - pk.Px, pb.Py - pk.Py, pb.Pz - pk.Pz};
+            const double X = rho * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]);
+            const double pref = 34.986836655249725 * pb.Kp * pk.Kp * rs;
+            double u[NR], w[NR];
+            rys_nodes<NR>(X, u, w);
+            const double QC[3] = {pk.Px - C[0], pk.Py - C[1], pk.Pz - C[2]};
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                const double t2 = u[r];
+                const R b00 = (R)(0.5 * ipq * t2);
+                const R b10 = (R)(0.5 * pb.ip * (1.0 - q * ipq * t2));
+                const R b01 = (R)(0.5 * pk.ip * (1.0 - p * ipq * t2));
+                R c00[3], c00p[3];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    c00[k] = (R)(PA[k] - q * ipq * PQ[k] * t2);
+                    c00p[k] = (R)(QC[k] + p * ipq * PQ[k] * t2);
+                }
+                R Ix[LC + 1][LD + 1], Iy[LC + 1][LD + 1], Iz[LC + 1][LD + 1];
+                dir_component<ax, bx, LC, LD, R>((R)(pref * w[r]), c00[0], c00p[0], b10, b01, b00,
+                                                 AB[0], CDp[0], Ix);
+                dir_component<ay, by, LC, LD, R>((R)1, c00[1], c00p[1], b10, b01, b00, AB[1],
+                                                 CDp[1], Iy);
+                dir_component<az, bz, LC, LD, R>((R)1, c00[2], c00p[2], b10, b01, b00, AB[2],
+                                                 CDp[2], Iz);
+#pragma unroll
+                for (int fc = 0; fc < S::NC; ++fc)
+#pragma unroll
+                    for (int fd = 0; fd < S::ND; ++fd)
+                        acc[fc * S::ND + fd] += Ix[cart_l(LC, fc, 0)][cart_l(LD, fd, 0)] *
+                                                Iy[cart_l(LC, fc, 1)][cart_l(LD, fd, 1)] *
+                                                Iz[cart_l(LC, fc, 2)][cart_l(LD, fd, 2)];
+            }
+        }
+    }
+}
+
+// One digestion tile of a shell quartet from its raw integrals v (plane
+// stride vs): T = 0 X_ab, 1 X_cd, 2 Y_ac, 3 Y_ad, 4 Y_bc, 5 Y_bd (see
+// digest_quartet).  fac is the degeneracy factor.
+template <int LA, int LB, int LC, int LD, int T>
+SF_HD SF_INLINE void digest_tile(const double* __restrict__ v, long long vs, double fac,
+                                 const double* __restrict__ P, int nbf, int fa0, int fb0, int fc0,
+                                 int fd0, double* __restrict__ t)
+{
+    using S = QuartetShape<LA, LB, LC, LD>;
+    constexpr int N1 = (T == 0 || T == 2 || T == 3) ? S::NA : (T == 1) ? S::NC : S::NB;
+    constexpr int N2 = (T == 0) ? S::NB : (T == 2 || T == 4) ? S::NC : S::ND;
+    // density block contracted against: T0 P_cd, T1 P_ab, T2 P_bd, T3 P_bc, T4 P_ad, T5 P_ac
+    constexpr int M1 = (T == 0) ? S::NC : (T == 1) ? S::NA : (T == 2 || T == 3) ? S::NB : S::NA;
+    constexpr int M2 = (T == 0) ? S::ND : (T == 1) ? S::NB : (T == 2) ? S::ND : (T == 3) ? S::NC : (T == 4) ? S::ND : S::NC;
+    const int r0 = (T == 0) ? fc0 : (T == 1) ? fa0 : (T == 2 || T == 3) ? fb0 : fa0;
+    const int c0 = (T == 0) ? fd0 : (T == 1) ? fb0 : (T == 2) ? fd0 : (T == 3) ? fc0 : (T == 4) ? fd0 : fc0;
+    double Pm[M1][M2];
+#pragma unroll
+    for (int x = 0; x < M1; ++x)
+#pragma unroll
+        for (int y = 0; y < M2; ++y) Pm[x][y] = P[(long)(r0 + x) * nbf + c0 + y];
+#pragma unroll
+    for (int e = 0; e < N1 * N2; ++e) t[e] = 0.0;
+#pragma unroll
+    for (int a = 0; a < S::NA; ++a)
+#pragma unroll
+        for (int b = 0; b < S::NB; ++b)
+#pragma unroll
+            for (int c = 0; c < S::NC; ++c)
+#pragma unroll
+                for (int d = 0; d < S::ND; ++d) {
+                    const double x = fac * v[(long long)(((a * S::NB + b) * S::NC + c) * S::ND + d) * vs];
+                    if constexpr (T == 0) t[a * N2 + b] += 2.0 * x * Pm[c][d];
+                    else if constexpr (T == 1) t[c * N2 + d] += 2.0 * x * Pm[a][b];
+                    else if constexpr (T == 2) t[a * N2 + c] += x * Pm[b][d];
+                    else if constexpr (T == 3) t[a * N2 + d] += x * Pm[b][c];
+                    else if constexpr (T == 4) t[b * N2 + c] += x * Pm[a][d];
+                    else t[b * N2 + d] += x * Pm[a][c];
+                }
 }
 
 // J/K digestion of one shell quartet.  v[NINT] are the integrals in the

@@ -81,8 +81,13 @@ __global__ __launch_bounds__(256) void k_eri_class(const ClassParams cp)
         double t[S::NTILE];
         digest_quartet<LA, LB, LC, LD, double>(v, cp.P, cp.nbf, cp.shell_foff[a], cp.shell_foff[b],
                                                cp.shell_foff[c], cp.shell_foff[d], t);
+        // each tile goes to its destination-sorted slot (contiguous elements)
 #pragma unroll
-        for (int k = 0; k < S::NTILE; ++k) cp.out[k * cp.qstride + qa] = t[k];
+        for (int T = 0; T < 6; ++T) {
+            double* dst = cp.tiles + cp.pos[T * cp.qstride + qa];
+#pragma unroll
+            for (int e = 0; e < S::tile_n1(T) * S::tile_n2(T); ++e) dst[e] = t[S::tile_off(T) + e];
+        }
     }
 }
 
@@ -114,10 +119,107 @@ __global__ __launch_bounds__(256) void k_schwarz(const int* __restrict__ sa, con
 }
 
 
+// Split classes, pass 1: one wavefront per (64 quartets, bra component
+// pair); the component pair index is wave-uniform, so each wavefront runs
+// one compile-time instance of eri_bra_component.  Writes raw integral
+// planes raw[(fa*NB+fb)*NC*ND + fc*ND + fd][q].
+template <int LA, int LB, int LC, int LD, typename R>
+__global__ __launch_bounds__(256) void k_eri_split(const ClassParams cp, double* __restrict__ raw)
+{
+    using S = QuartetShape<LA, LB, LC, LD>;
+    constexpr int NS = S::NA * S::NB, NK = S::NC * S::ND;
+    const long long gw = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int s = __builtin_amdgcn_readfirstlane((int)(gw % NS));
+    const long long q = (gw / NS) * 64 + (threadIdx.x & 63);
+    if (q >= cp.nq) return;
+    const int i = find_bra(cp.seg_base, cp.bra.n, q);
+    const int j = cp.seg_k0[i] + (int)(q - cp.seg_base[i]);
+    const long long qa = cp.qall_base[i] + j;
+    const int a = cp.bra.sa[i], b = cp.bra.sb[i];
+    const int c = cp.ket.sa[j], d = cp.ket.sb[j];
+    const double A[3] = {cp.shell_xyz[3 * a], cp.shell_xyz[3 * a + 1], cp.shell_xyz[3 * a + 2]};
+    const double B[3] = {cp.shell_xyz[3 * b], cp.shell_xyz[3 * b + 1], cp.shell_xyz[3 * b + 2]};
+    const double C[3] = {cp.shell_xyz[3 * c], cp.shell_xyz[3 * c + 1], cp.shell_xyz[3 * c + 2]};
+    const double D[3] = {cp.shell_xyz[3 * d], cp.shell_xyz[3 * d + 1], cp.shell_xyz[3 * d + 2]};
+    R acc[NK];
+#pragma unroll
+    for (int k = 0; k < NK; ++k) acc[k] = (R)0;
+    static_switch<0, NS>(s, [&](auto sc) {
+        constexpr int FA = decltype(sc)::value / S::NB, FB = decltype(sc)::value % S::NB;
+        eri_bra_component<LA, LB, LC, LD, FA, FB, R>(cp.prim + cp.bra.poff[i], cp.bra.np[i],
+                                                     cp.prim + cp.ket.poff[j], cp.ket.np[j], A, B,
+                                                     C, D, acc);
+    });
+#pragma unroll
+    for (int k = 0; k < NK; ++k) raw[(long long)(s * NK + k) * cp.qstride + qa] = (double)acc[k];
+}
+
+// Split classes, pass 2: one wavefront per (64 quartets, digestion tile).
+template <int LA, int LB, int LC, int LD>
+__global__ __launch_bounds__(256) void k_digest_split(const ClassParams cp,
+                                                      const double* __restrict__ raw)
+{
+    using S = QuartetShape<LA, LB, LC, LD>;
+    const long long gw = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int t = __builtin_amdgcn_readfirstlane((int)(gw % 6));
+    const long long q = (gw / 6) * 64 + (threadIdx.x & 63);
+    if (q >= cp.nq) return;
+    const int i = find_bra(cp.seg_base, cp.bra.n, q);
+    const int j = cp.seg_k0[i] + (int)(q - cp.seg_base[i]);
+    const long long qa = cp.qall_base[i] + j;
+    const int a = cp.bra.sa[i], b = cp.bra.sb[i];
+    const int c = cp.ket.sa[j], d = cp.ket.sb[j];
+    const double fac = ((a == b) ? 0.5 : 1.0) * ((c == d) ? 0.5 : 1.0) *
+                       ((cp.same && i == j) ? 0.5 : 1.0);
+    static_switch<0, 6>(t, [&](auto tc) {
+        constexpr int T = decltype(tc)::value;
+        constexpr int NE = S::tile_n1(T) * S::tile_n2(T);
+        double tile[NE];
+        digest_tile<LA, LB, LC, LD, T>(raw + qa, cp.qstride, fac, cp.P, cp.nbf, cp.shell_foff[a],
+                                       cp.shell_foff[b], cp.shell_foff[c], cp.shell_foff[d], tile);
+        double* dst = cp.tiles + cp.pos[T * cp.qstride + qa];
+#pragma unroll
+        for (int e = 0; e < NE; ++e) dst[e] = tile[e];
+    });
+}
+
+template <int LA, int LB, int LC, int LD>
+static int launch_split(int fp32, int mode, const ClassParams& cp, hipStream_t s)
+{
+    using S = QuartetShape<LA, LB, LC, LD>;
+    const long long waves = (cp.nq + 63) / 64;
+    const long long nb1 = (waves * S::NA * S::NB + 3) / 4, nb2 = (waves * 6 + 3) / 4;
+    if (nb1 > 0x7fffffffLL) {
+        set_error(kEUnsupported, "quartet segment too large (%lld)", cp.nq);
+        return kEUnsupported;
+    }
+    double* raw = (mode == kModeRaw) ? cp.out : cp.raw;
+    if (!raw) {
+        set_error(kEState, "split class without a raw buffer");
+        return kEState;
+    }
+    if (fp32 && mode != kModeRaw)
+        hipLaunchKernelGGL((k_eri_split<LA, LB, LC, LD, float>), dim3((unsigned)nb1), dim3(256), 0, s,
+                           cp, raw);
+    else
+        hipLaunchKernelGGL((k_eri_split<LA, LB, LC, LD, double>), dim3((unsigned)nb1), dim3(256), 0,
+                           s, cp, raw);
+    SF_HIP(hipGetLastError());
+    if (mode == kModeJK) {
+        hipLaunchKernelGGL((k_digest_split<LA, LB, LC, LD>), dim3((unsigned)nb2), dim3(256), 0, s, cp,
+                           (const double*)raw);
+        SF_HIP(hipGetLastError());
+    }
+    return kOk;
+}
+
 template <int LA, int LB, int LC, int LD>
 static int launch_class(int fp32, int mode, const ClassParams& cp, hipStream_t s)
 {
     if (cp.nq <= 0) return kOk;
+    if constexpr (QuartetShape<LA, LB, LC, LD>::SPLIT) {
+        return launch_split<LA, LB, LC, LD>(fp32, mode, cp, s);
+    } else {
     const int bs = 256;
     const long long nb = (cp.nq + bs - 1) / bs;
     if (nb > 0x7fffffffLL) {
@@ -133,6 +235,7 @@ static int launch_class(int fp32, int mode, const ClassParams& cp, hipStream_t s
         hipLaunchKernelGGL((k_eri_class<LA, LB, LC, LD, double, kModeJK>), grid, block, 0, s, cp);
     SF_HIP(hipGetLastError());
     return kOk;
+    }
 }
 
 template <int LA, int LB, int LC, int LD>
@@ -291,6 +394,17 @@ int class_ntile(int qcls)
     switch (qcls) {
 #define X(id, a, b, c, d) \
     case id: return QuartetShape<a, b, c, d>::NTILE;
+        SF_FOR_EACH_CLASS(X)
+#undef X
+    }
+    return 0;
+}
+
+int class_split(int qcls)
+{
+    switch (qcls) {
+#define X(id, a, b, c, d) \
+    case id: return QuartetShape<a, b, c, d>::SPLIT ? 1 : 0;
         SF_FOR_EACH_CLASS(X)
 #undef X
     }

@@ -182,69 +182,96 @@ int launch_segments(const unsigned int* keys, long long n, long long* seg_begin,
     return kOk;
 }
 
-// One 256-thread block per destination shell pair: fixed strided assignment
-// of its plan entries to threads, a fixed xor-butterfly per wavefront and a
-// fixed-order sum of the four wavefronts -- bitwise reproducible, no atomics.
+// For sorted plan position p (destination key k, entry code v = class | tile
+// | quartet): slot of that tile in the destination-sorted tile buffer,
+// key_base[k] + (p - seg_begin[k]) * NE(k), stored where the ERI kernel of
+// that class will look it up.
+__global__ __launch_bounds__(256) void k_plan_positions(const unsigned int* __restrict__ keys,
+                                                        const unsigned long long* __restrict__ vals,
+                                                        long long n,
+                                                        const long long* __restrict__ seg_begin,
+                                                        const unsigned long long* __restrict__ key_base,
+                                                        const int* __restrict__ key_ne,
+                                                        const ClassTab* __restrict__ tab)
+{
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const unsigned k = keys[p];
+    const unsigned long long v = vals[p];
+    const int cls = (int)(v >> 58);
+    const int t = (int)((v >> 55) & 7);
+    const long long q = (long long)(v & ((1ULL << 55) - 1));
+    tab[cls].pos[t * tab[cls].qstride + q] =
+        (unsigned)(key_base[k] + (unsigned long long)(p - seg_begin[k]) * key_ne[k]);
+}
+
+int launch_plan_positions(const unsigned int* keys, const unsigned long long* vals, long long n,
+                          const long long* seg_begin, const unsigned long long* key_base,
+                          const int* key_ne, const ClassTab* tab, hipStream_t s)
+{
+    if (n <= 0) return kOk;
+    hipLaunchKernelGGL(k_plan_positions, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, keys,
+                       vals, n, seg_begin, key_base, key_ne, tab);
+    SF_HIP(hipGetLastError());
+    return kOk;
+}
+
+// One wavefront per destination shell pair (four per 256-thread block).  The
+// tiles of a destination are contiguous (NE doubles each), so lanes read
+// them coalesced; a fixed strided assignment of tiles to lanes and a fixed
+// xor butterfly make the sum bitwise reproducible without atomics.
 template <int N1, int N2>
-__global__ __launch_bounds__(256) void k_reduce(const int* __restrict__ dest_keys,
+__global__ __launch_bounds__(256) void k_reduce(const int* __restrict__ dest_keys, int ndest,
                                                 const long long* __restrict__ seg_begin,
                                                 const long long* __restrict__ seg_end,
-                                                const unsigned long long* __restrict__ vals,
-                                                const ClassTab* __restrict__ tab,
+                                                const unsigned long long* __restrict__ key_base,
+                                                const double* __restrict__ tiles,
                                                 const int* __restrict__ foff, int ns, int nbf,
                                                 double* __restrict__ X, double* __restrict__ Y)
 {
     constexpr int NE = N1 * N2;
-    const int key = dest_keys[blockIdx.x];
+    const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (w >= ndest) return;
+    const int lane = threadIdx.x & 63;
+    const int key = dest_keys[w];
     const int n2 = ns * ns;
     const int m = key / n2;
     const int rem = key - m * n2;
     const int s1 = rem / ns, s2 = rem - (rem / ns) * ns;
+    const long long cnt = seg_end[key] - seg_begin[key];
+    const double* src = tiles + key_base[key];
     double acc[NE];
 #pragma unroll
     for (int k = 0; k < NE; ++k) acc[k] = 0.0;
-    const long long e1 = seg_end[key];
-    for (long long e = seg_begin[key] + threadIdx.x; e < e1; e += 256) {
-        const unsigned long long v = vals[e];
-        const int cls = (int)(v >> 58);
-        const int t = (int)((v >> 55) & 7);
-        const long long q = (long long)(v & ((1ULL << 55) - 1));
-        const long long qs = tab[cls].qstride;
-        const double* src = tab[cls].out + (long long)tab[cls].off[t] * qs + q;
+    for (long long e = lane; e < cnt; e += 64) {
+        const double* t = src + e * NE;
 #pragma unroll
-        for (int k = 0; k < NE; ++k) acc[k] += src[k * qs];
+        for (int k = 0; k < NE; ++k) acc[k] += t[k];
     }
 #pragma unroll
     for (int k = 0; k < NE; ++k)
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) acc[k] += __shfl_xor(acc[k], o, 64);
-    __shared__ double red[4][NE];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     if (lane == 0) {
-#pragma unroll
-        for (int k = 0; k < NE; ++k) red[wave][k] = acc[k];
-    }
-    __syncthreads();
-    if (threadIdx.x < NE) {
-        const int k = threadIdx.x;
-        const double sum = ((red[0][k] + red[1][k]) + red[2][k]) + red[3][k];
         double* dst = (m == 0) ? X : Y;
-        dst[(long long)(foff[s1] + k / N2) * nbf + foff[s2] + k % N2] = sum;
+#pragma unroll
+        for (int k = 0; k < NE; ++k)
+            dst[(long long)(foff[s1] + k / N2) * nbf + foff[s2] + k % N2] = acc[k];
     }
 }
 
 int launch_reduce(int n1, int n2, const int* dest_keys, int ndest, const long long* seg_begin,
-                  const long long* seg_end, const unsigned long long* vals, const ClassTab* tab,
+                  const long long* seg_end, const unsigned long long* key_base, const double* tiles,
                   const int* shell_foff, int nshell, int nbf, double* X, double* Y, hipStream_t s)
 {
     if (ndest <= 0) return kOk;
-    dim3 g(ndest), b(256);
-#define R_(a, c)                                                                             \
-    if (n1 == a && n2 == c) {                                                                \
-        hipLaunchKernelGGL((k_reduce<a, c>), g, b, 0, s, dest_keys, seg_begin, seg_end, vals,  \
-                           tab, shell_foff, nshell, nbf, X, Y);                              \
-        SF_HIP(hipGetLastError());                                                           \
-        return kOk;                                                                          \
+    dim3 g((ndest + 3) / 4), b(256);
+#define R_(a, c)                                                                              \
+    if (n1 == a && n2 == c) {                                                                 \
+        hipLaunchKernelGGL((k_reduce<a, c>), g, b, 0, s, dest_keys, ndest, seg_begin, seg_end,  \
+                           key_base, tiles, shell_foff, nshell, nbf, X, Y);                   \
+        SF_HIP(hipGetLastError());                                                            \
+        return kOk;                                                                           \
     }
     R_(1, 1) R_(1, 3) R_(1, 6) R_(3, 1) R_(3, 3) R_(3, 6) R_(6, 1) R_(6, 3) R_(6, 6)
 #undef R_

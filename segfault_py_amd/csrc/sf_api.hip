@@ -117,7 +117,7 @@ struct QuartetClassData {
     long long nq_all = 0, nq64 = 0, nq32 = 0;
     std::vector<int> n_all, n64;
     std::vector<long long> base_all, base64, base32;
-    DevBuf d_base_all, d_base64, d_base32, d_k032, d_out, d_raw;
+    DevBuf d_base_all, d_base64, d_base32, d_k032, d_pos, d_raw;
     int ntile = 0;
 };
 
@@ -133,10 +133,11 @@ struct sf_basis {
     double tau = 0.0, tau_mixed = 0.0;
     QuartetClassData qc[kNumQuartetClass];
     long long funcq = 0, primq = 0;
-    // digestion plan
-    DevBuf d_keys, d_vals, d_keys2, d_vals2, d_sort_tmp, d_segb, d_sege, d_tab;
+    // digestion plan: destination-sorted tile slots
+    DevBuf d_segb, d_sege, d_key_base, d_tiles;
     DevBuf d_dest[9];
     int ndest[9] = {0};
+    unsigned long long ntile_total = 0;
     DevBuf d_Ps, d_X, d_Y;
 };
 
@@ -318,7 +319,9 @@ extern "C" int sf_basis_create(int nshell, const int* L, const int* nprim, const
                     pp.Pz = (ea * A[2] + eb * B[2]) / p;
                     pp.K = b->coef[b->pstart[sa] + i] * b->coef[b->pstart[sb] + j] *
                            std::exp(-ea * eb / p * AB2);
-                    pp.pad0 = pp.pad1 = pp.pad2 = 0.0;
+                    pp.ip = 1.0 / p;
+                    pp.Kp = pp.K / p;
+                    pp.eb = eb;
                     b->prim.push_back(pp);
                 }
         }
@@ -484,7 +487,11 @@ extern "C" int sf_screen(sf_basis* b, double tau, double tau_mixed)
             SF_TRY(Q.d_base64.upload(Q.base64));
             SF_TRY(Q.d_base32.upload(Q.base32));
             SF_TRY(Q.d_k032.upload(Q.n64));
-            SF_TRY(Q.d_out.ensure((size_t)std::max(1LL, Q.nq_all) * Q.ntile * sizeof(double)));
+            SF_TRY(Q.d_pos.ensure((size_t)std::max(1LL, Q.nq_all) * 6 * sizeof(unsigned)));
+            if (class_split(c))
+                SF_TRY(Q.d_raw.ensure((size_t)std::max(1LL, Q.nq_all) * class_nint(c) * sizeof(double)));
+            else
+                Q.d_raw.release();
             total_entries += 6 * Q.nq_all;
             // function-level unique quartets and primitive quartets
             const int la = pair_la(pcb), lb = pair_lb(pcb), lc = pair_la(pck), ld = pair_lb(pck);
@@ -508,59 +515,81 @@ extern "C" int sf_screen(sf_basis* b, double tau, double tau_mixed)
         return kEUnsupported;
     }
     const size_t E = (size_t)std::max(1LL, total_entries);
-    SF_TRY(b->d_keys.ensure(E * sizeof(unsigned)));
-    SF_TRY(b->d_vals.ensure(E * sizeof(unsigned long long)));
-    SF_TRY(b->d_keys2.ensure(E * sizeof(unsigned)));
-    SF_TRY(b->d_vals2.ensure(E * sizeof(unsigned long long)));
+    DevBuf d_keys, d_vals, d_keys2, d_vals2, d_sort_tmp;
+    SF_TRY(d_keys.ensure(E * sizeof(unsigned)));
+    SF_TRY(d_vals.ensure(E * sizeof(unsigned long long)));
+    SF_TRY(d_keys2.ensure(E * sizeof(unsigned)));
+    SF_TRY(d_vals2.ensure(E * sizeof(unsigned long long)));
     long long ebase = 0;
     std::vector<ClassTab> tab(kNumQuartetClass);
     for (int c = 0; c < kNumQuartetClass; ++c) {
         QuartetClassData& Q = b->qc[c];
-        tab[c].out = Q.d_out.as<double>();
+        tab[c].pos = Q.d_pos.as<unsigned>();
         tab[c].qstride = Q.nq_all;
-        class_tile_offsets(c, tab[c].off);
-        tab[c].pad = 0;
         if (Q.nq_all == 0) continue;
         SF_TRY(launch_plan_entries(c, b->pc[Q.pcb].sorted_dev(), b->pc[Q.pck].sorted_dev(),
                                    Q.d_base_all.as<long long>(), Q.nq_all, ns,
-                                   b->d_keys.as<unsigned>() + ebase,
-                                   b->d_vals.as<unsigned long long>() + ebase, 0));
+                                   d_keys.as<unsigned>() + ebase,
+                                   d_vals.as<unsigned long long>() + ebase, 0));
         ebase += 6 * Q.nq_all;
     }
-    SF_TRY(b->d_tab.upload(tab));
+    DevBuf d_tab;
+    SF_TRY(d_tab.upload(tab));
     const int nbits = key_bits(nkeys);
     size_t tmp_bytes = 0;
-    SF_HIP(rocprim::radix_sort_pairs(nullptr, tmp_bytes, b->d_keys.as<unsigned>(),
-                                     b->d_keys2.as<unsigned>(), b->d_vals.as<unsigned long long>(),
-                                     b->d_vals2.as<unsigned long long>(), (size_t)total_entries, 0,
+    SF_HIP(rocprim::radix_sort_pairs(nullptr, tmp_bytes, d_keys.as<unsigned>(),
+                                     d_keys2.as<unsigned>(), d_vals.as<unsigned long long>(),
+                                     d_vals2.as<unsigned long long>(), (size_t)total_entries, 0,
                                      nbits, (hipStream_t)0));
-    SF_TRY(b->d_sort_tmp.ensure(tmp_bytes));
+    SF_TRY(d_sort_tmp.ensure(tmp_bytes));
     if (total_entries > 0)
-        SF_HIP(rocprim::radix_sort_pairs(b->d_sort_tmp.p, tmp_bytes, b->d_keys.as<unsigned>(),
-                                         b->d_keys2.as<unsigned>(),
-                                         b->d_vals.as<unsigned long long>(),
-                                         b->d_vals2.as<unsigned long long>(),
-                                         (size_t)total_entries, 0, nbits, (hipStream_t)0));
+        SF_HIP(rocprim::radix_sort_pairs(d_sort_tmp.p, tmp_bytes, d_keys.as<unsigned>(),
+                                         d_keys2.as<unsigned>(), d_vals.as<unsigned long long>(),
+                                         d_vals2.as<unsigned long long>(), (size_t)total_entries,
+                                         0, nbits, (hipStream_t)0));
+    d_sort_tmp.release();
+    d_keys.release();
+    d_vals.release();
     SF_TRY(b->d_segb.ensure(nkeys * sizeof(long long)));
     SF_TRY(b->d_sege.ensure(nkeys * sizeof(long long)));
     SF_HIP(hipMemset(b->d_segb.p, 0, nkeys * sizeof(long long)));
     SF_HIP(hipMemset(b->d_sege.p, 0, nkeys * sizeof(long long)));
-    SF_TRY(launch_segments(b->d_keys2.as<unsigned>(), total_entries, b->d_segb.as<long long>(),
+    SF_TRY(launch_segments(d_keys2.as<unsigned>(), total_entries, b->d_segb.as<long long>(),
                            b->d_sege.as<long long>(), 0));
     std::vector<long long> segb(nkeys), sege(nkeys);
     SF_HIP(hipMemcpy(segb.data(), b->d_segb.p, nkeys * sizeof(long long), hipMemcpyDeviceToHost));
     SF_HIP(hipMemcpy(sege.data(), b->d_sege.p, nkeys * sizeof(long long), hipMemcpyDeviceToHost));
-    // sorted values live in d_vals2; the unsorted buffers are no longer needed
-    b->d_keys.release();
-    b->d_vals.release();
-    b->d_sort_tmp.release();
+    // slot layout: destinations in key order, each a run of NE-double tiles
+    std::vector<unsigned long long> key_base(nkeys);
+    std::vector<int> key_ne(nkeys);
     std::vector<int> dest[9];
+    unsigned long long acc = 0;
+    const unsigned long long nsq = (unsigned long long)ns * ns;
     for (unsigned long long k = 0; k < nkeys; ++k) {
-        if (sege[k] <= segb[k]) continue;
-        const unsigned long long rem = k % ((unsigned long long)ns * ns);
+        const unsigned long long rem = k % nsq;
         const int s1 = (int)(rem / ns), s2 = (int)(rem % ns);
+        key_ne[k] = cart_n(b->L[s1]) * cart_n(b->L[s2]);
+        key_base[k] = acc;
+        if (sege[k] <= segb[k]) continue;
+        acc += (unsigned long long)(sege[k] - segb[k]) * key_ne[k];
         dest[3 * b->L[s1] + b->L[s2]].push_back((int)k);
     }
+    if (acc >= (1ULL << 32)) {
+        set_error(kEUnsupported, "digestion plan of %llu doubles exceeds 32-bit slots", acc);
+        return kEUnsupported;
+    }
+    b->ntile_total = acc;
+    SF_TRY(b->d_key_base.upload(key_base));
+    DevBuf d_key_ne;
+    SF_TRY(d_key_ne.upload(key_ne));
+    SF_TRY(launch_plan_positions(d_keys2.as<unsigned>(), d_vals2.as<unsigned long long>(),
+                                 total_entries, b->d_segb.as<long long>(),
+                                 b->d_key_base.as<unsigned long long>(), d_key_ne.as<int>(),
+                                 d_tab.as<ClassTab>(), 0));
+    SF_HIP(hipDeviceSynchronize());
+    d_keys2.release();
+    d_vals2.release();
+    SF_TRY(b->d_tiles.ensure(std::max<size_t>(1, acc) * sizeof(double)));
     for (int g = 0; g < 9; ++g) {
         b->ndest[g] = (int)dest[g].size();
         SF_TRY(b->d_dest[g].upload(dest[g]));
@@ -651,7 +680,10 @@ static ClassParams class_params(sf_basis* b, int c, const double* Ps)
     cp.same = Q.same;
     cp.nbf = b->nbf;
     cp.P = Ps;
-    cp.out = Q.d_out.as<double>();
+    cp.out = nullptr;
+    cp.raw = Q.d_raw.as<double>();
+    cp.pos = Q.d_pos.as<unsigned>();
+    cp.tiles = b->d_tiles.as<double>();
     return cp;
 }
 
@@ -675,13 +707,13 @@ extern "C" int sf_eri_dense(sf_basis* b, double* eri)
         QuartetClassData& Q = b->qc[c];
         if (Q.nq_all == 0) continue;
         const int nint = class_nint(c);
-        SF_TRY(Q.d_raw.ensure((size_t)Q.nq_all * nint * sizeof(double)));
+        DevBuf d_dense;
+        SF_TRY(d_dense.ensure((size_t)Q.nq_all * nint * sizeof(double)));
         ClassParams cp = class_params(b, c, nullptr);
-        cp.out = Q.d_raw.as<double>();
+        cp.out = d_dense.as<double>();
         SF_TRY(launch_eri_class(c, 0, kModeRaw, cp, 0));
         std::vector<double> raw((size_t)Q.nq_all * nint);
-        SF_HIP(hipMemcpy(raw.data(), Q.d_raw.p, raw.size() * sizeof(double), hipMemcpyDeviceToHost));
-        Q.d_raw.release();
+        SF_HIP(hipMemcpy(raw.data(), d_dense.p, raw.size() * sizeof(double), hipMemcpyDeviceToHost));
         const PairClassData& B = b->pc[Q.pcb];
         const PairClassData& K = b->pc[Q.pck];
         const int nA = cart_n(pair_la(Q.pcb)), nB = cart_n(pair_lb(Q.pcb));
@@ -770,7 +802,7 @@ static int run_jk(sf_basis* b, const double* P, double* J, double* K, int precis
     for (int g = 0; g < 9; ++g)
         SF_TRY(launch_reduce(nf[g / 3], nf[g % 3], b->d_dest[g].as<int>(), b->ndest[g],
                              b->d_segb.as<long long>(), b->d_sege.as<long long>(),
-                             b->d_vals2.as<unsigned long long>(), b->d_tab.as<ClassTab>(),
+                             b->d_key_base.as<unsigned long long>(), b->d_tiles.as<double>(),
                              b->d_foff.as<int>(), b->nshell, nbf, b->d_X.as<double>(),
                              b->d_Y.as<double>(), s));
     SF_TRY(launch_finalize(b->d_X.as<double>(), b->d_Y.as<double>(), b->d_fnorm.as<double>(), nbf,

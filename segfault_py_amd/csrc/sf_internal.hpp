@@ -63,17 +63,18 @@ struct ClassParams {
     int same;                  // bra and ket pair classes coincide
     int nbf;
     const double* P;           // density in the shell-normalised basis
-    double* out;               // SoA planes: out[k * qstride + q]
+    double* out;               // kModeRaw: SoA planes out[k * qstride + q]
+    double* raw;               // split classes: raw integral planes (NINT * qstride)
+    const unsigned* pos;       // kModeJK: pos[t * qstride + q] = slot of tile t in `tiles`
+    double* tiles;             // kModeJK: destination-sorted digestion tiles
 };
 
 enum EriMode : int { kModeJK = 0, kModeRaw = 1 };
 
-// Digestion tile plane offsets of one quartet class.
+// Where the plan-position kernel writes the slots of one quartet class.
 struct ClassTab {
-    const double* out;
+    unsigned* pos;       // pos[t * qstride + q]
     long long qstride;
-    int off[6];
-    int pad;
 };
 
 // Launchers (k_eri.hip).
@@ -82,6 +83,7 @@ int launch_schwarz(int pc, const int* sa, const int* sb, const int* poff, const 
                    const PrimPair* prim, const double* shell_xyz, double* Q, hipStream_t s);
 int class_ntile(int qcls);
 int class_nint(int qcls);
+int class_split(int qcls);
 void class_tile_offsets(int qcls, int off[6]);
 
 // Launchers (k_util.hip).
@@ -94,9 +96,13 @@ int launch_plan_entries(int qcls, const PairListDev& bra, const PairListDev& ket
                         unsigned int* keys, unsigned long long* vals, hipStream_t s);
 int launch_segments(const unsigned int* keys, long long n, long long* seg_begin,
                     long long* seg_end, hipStream_t s);
+int launch_plan_positions(const unsigned int* keys, const unsigned long long* vals, long long n,
+                          const long long* seg_begin, const unsigned long long* key_base,
+                          const int* key_ne, const ClassTab* tab, hipStream_t s);
 int launch_reduce(int n1, int n2, const int* dest_keys, int ndest, const long long* seg_begin,
-                  const long long* seg_end, const unsigned long long* vals, const ClassTab* tab,
-                  const int* shell_foff, int nshell, int nbf, double* X, double* Y, hipStream_t s);
+                  const long long* seg_end, const unsigned long long* key_base,
+                  const double* tiles, const int* shell_foff, int nshell, int nbf, double* X,
+                  double* Y, hipStream_t s);
 int launch_scale_density(const double* P, const double* fnorm, int nbf, double* Ps, hipStream_t s);
 int launch_finalize(const double* X, const double* Y, const double* fnorm, int nbf, double* J,
                     double* K, hipStream_t s);

@@ -9,8 +9,8 @@
 //    grid of exact values (X < 39.95) followed by downward recursion
 //    F_{m-1} = (2X F_m + e^-X)/(2m-1); for X >= 39.95, F_0 = sqrt(pi/X)/2
 //    (erfc(sqrt(40)) < 2e-19) and upward recursion.
-//  * F_0, F_1 for one-node quadrature: alternating Maclaurin series for
-//    X < 0.5, else F_0 = sqrt(pi) erf(sqrt X)/(2 sqrt X), F_1 = (F_0 - e^-X)/(2X).
+//  * F_0, F_1 for one-node quadrature: the same grid Taylor series for both
+//    (no exp) below X = 40, F_0 = sqrt(pi/X)/2, F_1 = F_0/(2X) above.
 //  * Rys nodes/weights for n = 1..5: unit-interval Chebyshev series of
 //    degree 11 (tables generated at 50 digits by gen_tables.py) below
 //    XMAX[n], the Gauss-Hermite large-X limit above.
@@ -44,12 +44,30 @@ constexpr double kPi = 3.14159265358979323846264338;
 constexpr double kSqrtPi = 1.77245385090551602729816748;
 
 // ---------------------------------------------------------------- Boys ----
-// 1/j! for the Taylor terms.
-SF_HD SF_INLINE double inv_fact(int j)
+// 1/(2m-1) for the downward recursion, m = 1..34.
+SF_HD SF_INLINE double inv_odd(int m)
 {
-    constexpr double t[10] = {1.0, 1.0, 0.5, 1.0 / 6, 1.0 / 24, 1.0 / 120, 1.0 / 720,
-                              1.0 / 5040, 1.0 / 40320, 1.0 / 362880};
-    return t[j];
+    constexpr double t[35] = {
+        -1.0,       1.0,        1.0 / 3,  1.0 / 5,  1.0 / 7,  1.0 / 9,  1.0 / 11, 1.0 / 13,
+        1.0 / 15,   1.0 / 17,   1.0 / 19, 1.0 / 21, 1.0 / 23, 1.0 / 25, 1.0 / 27, 1.0 / 29,
+        1.0 / 31,   1.0 / 33,   1.0 / 35, 1.0 / 37, 1.0 / 39, 1.0 / 41, 1.0 / 43, 1.0 / 45,
+        1.0 / 47,   1.0 / 49,   1.0 / 51, 1.0 / 53, 1.0 / 55, 1.0 / 57, 1.0 / 59, 1.0 / 61,
+        1.0 / 63,   1.0 / 65,   1.0 / 67};
+    return t[m];
+}
+
+// sum_{j=0}^{8} g[j] d^j / j!  (Horner; the 1/(j+1) factors are constants)
+SF_HD SF_INLINE double taylor9(const double* g, double d)
+{
+    double s = g[8];
+    s = g[7] + s * (d * (1.0 / 8));
+    s = g[6] + s * (d * (1.0 / 7));
+    s = g[5] + s * (d * (1.0 / 6));
+    s = g[4] + s * (d * (1.0 / 5));
+    s = g[3] + s * (d * (1.0 / 4));
+    s = g[2] + s * (d * (1.0 / 3));
+    s = g[1] + s * (d * (1.0 / 2));
+    return g[0] + s * d;
 }
 
 // F_0..F_mmax at X >= 0 (mmax <= SF_BOYS_MTAB - 10).
@@ -58,48 +76,34 @@ SF_HD SF_INLINE void boys_grid(int mmax, double X, double* F)
     if (X < 39.95) {
         const int k = (int)(X * 10.0 + 0.5);
         const double d = k * SF_BOYS_DX - X;  // |d| <= 0.05
-        const double* g = sf_boys_grid + k * SF_BOYS_MTAB + mmax;
-        double s = g[8];
-#pragma unroll
-        for (int j = 7; j >= 0; --j) s = g[j] + s * d * (1.0 / (j + 1));
-        // Horner form of sum_j g[j] d^j / j!
-        F[mmax] = s;
+        F[mmax] = taylor9(sf_boys_grid + k * SF_BOYS_MTAB + mmax, d);
         if (mmax > 0) {
             const double e = exp(-X);
-            for (int m = mmax; m > 0; --m) F[m - 1] = (2.0 * X * F[m] + e) / (2.0 * m - 1.0);
+            for (int m = mmax; m > 0; --m) F[m - 1] = (2.0 * X * F[m] + e) * inv_odd(m);
         }
     } else {
         const double e = exp(-X);
-        const double inv2x = 0.5 / X;
-        F[0] = 0.5 * kSqrtPi / sqrt(X);
-        for (int m = 0; m < mmax; ++m) F[m + 1] = ((2.0 * m + 1.0) * F[m] - e) * inv2x;
+        const double ix = 1.0 / X;
+        F[0] = 0.5 * kSqrtPi * sqrt(ix);
+        for (int m = 0; m < mmax; ++m) F[m + 1] = ((2.0 * m + 1.0) * F[m] - e) * (0.5 * ix);
     }
 }
 
-// F_0 and F_1 by closed form (no table loads).
+// F_0 and F_1 without exp or erf: grid Taylor series below X = 40, the
+// asymptotic forms above (e^-X < 4.3e-18 there, below double rounding of
+// F_0 and F_1).  When only F_0 is used the compiler drops the F_1 terms.
 SF_HD SF_INLINE void boys01(double X, double& F0, double& F1)
 {
-    if (X < 0.5) {
-        // F_m(X) = sum_k (-X)^k / (k! (2m+2k+1)), k = 0..15
-        double s0 = 0.0, s1 = 0.0, term = 1.0;
-        double pw[16];
-#pragma unroll
-        for (int k = 0; k < 16; ++k) {
-            pw[k] = term;
-            term *= -X / (k + 1);
-        }
-#pragma unroll
-        for (int k = 15; k >= 0; --k) {
-            s0 += pw[k] / (2 * k + 1);
-            s1 += pw[k] / (2 * k + 3);
-        }
-        F0 = s0;
-        F1 = s1;
+    if (X < 39.95) {
+        const int k = (int)(X * 10.0 + 0.5);
+        const double d = k * SF_BOYS_DX - X;
+        const double* g = sf_boys_grid + k * SF_BOYS_MTAB;
+        F0 = taylor9(g, d);
+        F1 = taylor9(g + 1, d);
     } else {
-        const double sx = sqrt(X);
-        const double e = exp(-X);
-        F0 = 0.5 * kSqrtPi * erf(sx) / sx;
-        F1 = (F0 - e) * (0.5 / X);
+        const double ix = 1.0 / X;
+        F0 = 0.5 * kSqrtPi * sqrt(ix);
+        F1 = 0.5 * F0 * ix;
     }
 }
 

@@ -29,6 +29,9 @@ def harness():
         H.h_eri.argtypes = [ctypes.c_int] * 4 + [_f64p, ctypes.c_int, _f64p, ctypes.c_int, _f64p,
                                                  _f64p, _f64p, _f64p, _f64p, ctypes.c_int]
         H.h_digest.argtypes = [ctypes.c_int] * 4 + [_f64p, _f64p, ctypes.c_int, _i32p, _f64p]
+        H.h_eri_split.argtypes = H.h_eri.argtypes
+        H.h_digest_tiles.argtypes = [ctypes.c_int] * 4 + [_f64p, ctypes.c_double, _f64p,
+                                                          ctypes.c_int, _i32p, _f64p]
         H.h_rys.argtypes = [ctypes.c_int, ctypes.c_double, _f64p, _f64p]
         H.h_boys.argtypes = [ctypes.c_int, ctypes.c_double, _f64p]
         H.h_boys01.argtypes = [ctypes.c_double, _f64p]
@@ -48,18 +51,26 @@ def prim_pairs(sa, sb):
         for b, cb in zip(sb.expns, sb.coefs):
             p = a + b
             P = (a * A + b * B) / p
-            out.append([p, *P, ca * cb * np.exp(-a * b / p * AB2), 0.0, 0.0, 0.0])
+            K = ca * cb * np.exp(-a * b / p * AB2)
+            out.append([p, *P, K, 1.0 / p, K / p, b])
     return np.ascontiguousarray(np.array(out).reshape(-1))
 
 
-def shell_quartet(shells, a, b, c, d, fp32=False):
+def shell_quartet(shells, a, b, c, d, fp32=False, split=False):
     S = [shells[a], shells[b], shells[c], shells[d]]
     out = np.zeros(int(np.prod([s.nfunc for s in S])))
-    rc = harness().h_eri(S[0].L, S[1].L, S[2].L, S[3].L, prim_pairs(S[0], S[1]),
+    fn = harness().h_eri_split if split else harness().h_eri
+    rc = fn(S[0].L, S[1].L, S[2].L, S[3].L, prim_pairs(S[0], S[1]),
                          S[0].nprim * S[1].nprim, prim_pairs(S[2], S[3]), S[2].nprim * S[3].nprim,
                          S[0].center, S[1].center, S[2].center, S[3].center, out, int(fp32))
     assert rc == 0
     return out
+
+
+def uses_split(la, lb, lc, ld):
+    """Classes the GPU computes one bra component pair per wavefront (more
+    than 81 integrals per quartet); the rule of QuartetShape::SPLIT."""
+    return ncart(la) * ncart(lb) * ncart(lc) * ncart(ld) > 81
 
 
 def canonical_pairs(shells):
@@ -135,14 +146,18 @@ def jk_model(mol, P, tau=1e-12, tau_mixed=0.0, fp32_below=False):
             for i, (a, b) in enumerate(bra):
                 for j in range(n_all[i]):
                     c, d = ket[j]
-                    v = shell_quartet(shells, a, b, c, d, fp32=fp32_below and j >= n64[i])
-                    fac = (0.5 if a == b else 1.0) * (0.5 if c == d else 1.0) * (0.5 if same and i == j else 1.0)
-                    v = np.ascontiguousarray(v * fac)
                     Ls = [shells[s].L for s in (a, b, c, d)]
+                    split = uses_split(*Ls)
+                    v = shell_quartet(shells, a, b, c, d, fp32=fp32_below and j >= n64[i], split=split)
+                    fac = (0.5 if a == b else 1.0) * (0.5 if c == d else 1.0) * (0.5 if same and i == j else 1.0)
                     nA, nB, nC, nD = [ncart(L) for L in Ls]
                     t = np.zeros(nA * nB + nC * nD + nA * nC + nA * nD + nB * nC + nB * nD)
                     f0 = np.array([shells[s].offset for s in (a, b, c, d)], dtype=np.int32)
-                    assert harness().h_digest(*Ls, v, Ps, n, f0, t) == 0
+                    if split:
+                        assert harness().h_digest_tiles(*Ls, np.ascontiguousarray(v), fac, Ps, n, f0, t) == 0
+                    else:
+                        v = np.ascontiguousarray(v * fac)
+                        assert harness().h_digest(*Ls, v, Ps, n, f0, t) == 0
                     o = 0
                     for (s1, s2, n1, n2, M) in ((a, b, nA, nB, X), (c, d, nC, nD, X), (a, c, nA, nC, Y),
                                                 (a, d, nA, nD, Y), (b, c, nB, nC, Y), (b, d, nB, nD, Y)):

@@ -39,6 +39,42 @@ static void run(const double* bra, int nb, const double* ket, int nk, const doub
 }
 
 template <int LA, int LB, int LC, int LD>
+static void run_split(const double* bra, int nb, const double* ket, int nk, const double* A,
+                      const double* B, const double* C, const double* D, double* out, int fp32)
+{
+    using S = QuartetShape<LA, LB, LC, LD>;
+    for (int s = 0; s < S::NA * S::NB; ++s) {
+        static_switch<0, S::NA * S::NB>(s, [&](auto sc) {
+            constexpr int FA = decltype(sc)::value / S::NB, FB = decltype(sc)::value % S::NB;
+            if (fp32) {
+                float acc[S::NC * S::ND] = {};
+                eri_bra_component<LA, LB, LC, LD, FA, FB, float>(
+                    (const PrimPair*)bra, nb, (const PrimPair*)ket, nk, A, B, C, D, acc);
+                for (int k = 0; k < S::NC * S::ND; ++k) out[s * S::NC * S::ND + k] = acc[k];
+            } else {
+                double acc[S::NC * S::ND] = {};
+                eri_bra_component<LA, LB, LC, LD, FA, FB, double>(
+                    (const PrimPair*)bra, nb, (const PrimPair*)ket, nk, A, B, C, D, acc);
+                for (int k = 0; k < S::NC * S::ND; ++k) out[s * S::NC * S::ND + k] = acc[k];
+            }
+        });
+    }
+}
+
+template <int LA, int LB, int LC, int LD>
+static void run_tiles(const double* v, double fac, const double* P, int nbf, const int* f0,
+                      double* t)
+{
+    using S = QuartetShape<LA, LB, LC, LD>;
+    digest_tile<LA, LB, LC, LD, 0>(v, 1, fac, P, nbf, f0[0], f0[1], f0[2], f0[3], t + S::T_AB);
+    digest_tile<LA, LB, LC, LD, 1>(v, 1, fac, P, nbf, f0[0], f0[1], f0[2], f0[3], t + S::T_CD);
+    digest_tile<LA, LB, LC, LD, 2>(v, 1, fac, P, nbf, f0[0], f0[1], f0[2], f0[3], t + S::T_AC);
+    digest_tile<LA, LB, LC, LD, 3>(v, 1, fac, P, nbf, f0[0], f0[1], f0[2], f0[3], t + S::T_AD);
+    digest_tile<LA, LB, LC, LD, 4>(v, 1, fac, P, nbf, f0[0], f0[1], f0[2], f0[3], t + S::T_BC);
+    digest_tile<LA, LB, LC, LD, 5>(v, 1, fac, P, nbf, f0[0], f0[1], f0[2], f0[3], t + S::T_BD);
+}
+
+template <int LA, int LB, int LC, int LD>
 static void run_digest(const double* v, const double* P, int nbf, const int* f0, double* t)
 {
     digest_quartet<LA, LB, LC, LD, double>(v, P, nbf, f0[0], f0[1], f0[2], f0[3], t);
@@ -69,6 +105,33 @@ extern "C" int h_digest(int la, int lb, int lc, int ld, const double* v, const d
 #define X(a, b, c, d)                                      \
     if (la == a && lb == b && lc == c && ld == d) {        \
         run_digest<a, b, c, d>(v, P, nbf, f0, t);          \
+        return 0;                                          \
+    }
+    SF_CLASSES(X)
+#undef X
+    return 1;
+}
+
+extern "C" int h_eri_split(int la, int lb, int lc, int ld, const double* bra, int nb,
+                           const double* ket, int nk, const double* A, const double* B,
+                           const double* C, const double* D, double* out, int fp32)
+{
+#define X(a, b, c, d)                                                          \
+    if (la == a && lb == b && lc == c && ld == d) {                            \
+        run_split<a, b, c, d>(bra, nb, ket, nk, A, B, C, D, out, fp32);        \
+        return 0;                                                              \
+    }
+    SF_CLASSES(X)
+#undef X
+    return 1;
+}
+
+extern "C" int h_digest_tiles(int la, int lb, int lc, int ld, const double* v, double fac,
+                              const double* P, int nbf, const int* f0, double* t)
+{
+#define X(a, b, c, d)                                      \
+    if (la == a && lb == b && lc == c && ld == d) {        \
+        run_tiles<a, b, c, d>(v, fac, P, nbf, f0, t);      \
         return 0;                                          \
     }
     SF_CLASSES(X)

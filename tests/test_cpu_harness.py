@@ -134,3 +134,44 @@ def test_ket_count_builder_prefix_and_caps():
     assert list(n64) == [1, 2, 2, 0]
     n_all, n64 = CM.ket_counts(Qb, Qk[:2], False, 1e-9, 0.0)
     assert list(n_all) == [2, 2, 2, 2] and list(n64) == list(n_all)
+
+
+def test_split_bra_component_path_equals_full(water_ccpvdz):
+    """eri_bra_component (one wavefront per bra component pair) reproduces
+    eri_shell_quartet for every class, FP64 and FP32."""
+    shells = water_ccpvdz.shells()
+    last = {}
+    for k, sh in enumerate(shells):
+        last[sh.L] = k
+    n = 0
+    for la in (0, 1, 2):
+        for lb in range(la + 1):
+            for lc in (0, 1, 2):
+                for ld in range(lc + 1):
+                    if (la, lb) < (lc, ld):
+                        continue
+                    q = (last[la], last[lb], last[lc], last[ld])
+                    full = CM.shell_quartet(shells, *q)
+                    split = CM.shell_quartet(shells, *q, split=True)
+                    assert np.max(np.abs(full - split)) <= 1e-14 * max(1.0, np.max(np.abs(full)))
+                    f32 = CM.shell_quartet(shells, *q, fp32=True, split=True)
+                    assert np.max(np.abs(f32 - full)) <= 1e-5 * max(1.0, np.max(np.abs(full)))
+                    n += 1
+    assert n == 21
+
+
+def test_digest_tiles_equal_digest_quartet():
+    rng = np.random.default_rng(9)
+    nbf = 40
+    P = rng.standard_normal((nbf, nbf))
+    P = np.ascontiguousarray(P + P.T)
+    H = CM.harness()
+    for ls in [(0, 0, 0, 0), (1, 0, 1, 0), (2, 1, 1, 0), (2, 2, 2, 1), (2, 2, 2, 2), (1, 1, 1, 1)]:
+        nA, nB, nC, nD = [CM.ncart(l) for l in ls]
+        v = rng.standard_normal(nA * nB * nC * nD)
+        f0 = np.array([0, 7, 15, 30], dtype=np.int32)
+        nt = nA * nB + nC * nD + nA * nC + nA * nD + nB * nC + nB * nD
+        t1, t2 = np.zeros(nt), np.zeros(nt)
+        assert H.h_digest(*ls, np.ascontiguousarray(v * 0.25), P, nbf, f0, t1) == 0
+        assert H.h_digest_tiles(*ls, v, 0.25, P, nbf, f0, t2) == 0
+        assert np.max(np.abs(t1 - t2)) < 1e-12
