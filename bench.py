@@ -38,7 +38,22 @@ def parse():
                     help="target host time of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", type=int, default=1, help="workload index (1 = 20 waters)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (for HBM counter scaling)")
     return ap.parse_args()
+
+
+def copy_calibration(mib: int):
+    """A plain device-to-device copy of known size, to scale FETCH_SIZE /
+    WRITE_SIZE counter readings of the ERI kernels."""
+    import torch
+    n = mib * (1 << 20) // 8
+    a = torch.arange(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": n * 8, "read_bytes": n * 8, "write_bytes": n * 8}))
 
 
 def synthetic_density(n: int, nocc: int, seed: int = 0) -> np.ndarray:
@@ -111,6 +126,9 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only "
                                    "(see DESIGN.md), --gpus must be 1"}))
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     import torch
     from segfault_py_amd import engine, metrics, workloads as W
 

@@ -84,8 +84,10 @@ int sf_screen(sf_basis *b, double tau, double tau_mixed);
  * quartet class c (0..20); *funcq = surviving function-level quartets. */
 int sf_quartet_counts(const sf_basis *b, long long *counts, long long *funcq,
                       long long *primq);
-/* Sorted significant pairs of pair class pc (0..5): shells and Q. */
-int sf_pair_list(const sf_basis *b, int pc, int *n, int *sa, int *sb, double *Q);
+/* Sorted significant pairs of pair class pc (0..5): shells, Q and the
+ * number of primitive pairs kept after primitive-pair screening (any output
+ * pointer may be NULL). */
+int sf_pair_list(const sf_basis *b, int pc, int *n, int *sa, int *sb, double *Q, int *nprim);
 /* Per-bra ket counts of quartet class c (all, FP64 part). */
 int sf_quartet_list(const sf_basis *b, int c, int *nbra, int *n_all, int *n_fp64);
 

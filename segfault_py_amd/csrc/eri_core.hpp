@@ -125,7 +125,7 @@ SF_HD SF_INLINE void eri_shell_quartet(const PrimPair* __restrict__ bra, int nbr
             const PrimPair pk = ket[ik];
             const double q = pk.p;
             const double pq = p + q;
-            const double rs = 1.0 / sqrt(pq);
+            const double rs = fast_rsqrt(pq);
             const double ipq = rs * rs;
             const double rho = p * q * ipq;
             const double PQ[3] = {pb.Px - pk.Px, pb.Py - pk.Py, pb.Pz - pk.Pz};
@@ -306,7 +306,7 @@ SF_HD SF_INLINE void eri_bra_component(const PrimPair* __restrict__ bra, int nbr
             const PrimPair pk = ket[ik];
             const double q = pk.p;
             const double pq = p + q;
-            const double rs = 1.0 / sqrt(pq);
+            const double rs = fast_rsqrt(pq);
             const double ipq = rs * rs;
             const double rho = p * q * ipq;
             const double PQ[3] = {pb.Px - pk.Px, pb.Py - pk.Py, pb.Pz - pk.Pz};

@@ -291,7 +291,29 @@ extern "C" int sf_basis_create(int nshell, const int* L, const int* nprim, const
         pst += nprim[s];
     }
     b->nbf = off;
-    // all shell pairs a >= b, canonical orientation, primitive pair data
+    // all shell pairs a >= b, canonical orientation, primitive pair data.
+    // Primitive-pair screening: with Kp = K/p, a primitive quartet is bounded
+    // by 2 pi^{5/2} |Kp_ab| |Kp_cd| / sqrt(p+q) times its polynomial factors;
+    // pairs with 2 pi^{5/2} |Kp_ab| M / sqrt(p_ab) < 1e-20 (M = largest |Kp|
+    // of the basis) are dropped, which changes no integral by more than ~1e-15.
+    std::vector<double> kp_all;
+    double kp_max = 0.0;
+    for (int a = 0; a < nshell; ++a)
+        for (int c = 0; c <= a; ++c) {
+            const double* A = &b->xyz[3 * a];
+            const double* B = &b->xyz[3 * c];
+            const double AB2 = (A[0] - B[0]) * (A[0] - B[0]) + (A[1] - B[1]) * (A[1] - B[1]) +
+                               (A[2] - B[2]) * (A[2] - B[2]);
+            for (int i = 0; i < nprim[a]; ++i)
+                for (int j = 0; j < nprim[c]; ++j) {
+                    const double ea = b->expn[b->pstart[a] + i], eb = b->expn[b->pstart[c] + j];
+                    const double p = ea + eb;
+                    const double K = b->coef[b->pstart[a] + i] * b->coef[b->pstart[c] + j] *
+                                     std::exp(-ea * eb / p * AB2);
+                    kp_max = std::max(kp_max, std::fabs(K / p));
+                }
+        }
+    const double kPrimTol = 1e-20;
     b->pair_slot.resize((size_t)nshell * (nshell + 1) / 2);
     for (int a = 0; a < nshell; ++a)
         for (int c = 0; c <= a; ++c) {
@@ -302,12 +324,12 @@ extern "C" int sf_basis_create(int nshell, const int* L, const int* nprim, const
             P.sa.push_back(sa);
             P.sb.push_back(sb);
             P.poff.push_back((int)b->prim.size());
-            P.np.push_back(nprim[sa] * nprim[sb]);
             b->pair_slot[(size_t)a * (a + 1) / 2 + c] = (pcl << 24) | (int)(P.sa.size() - 1);
             const double* A = &b->xyz[3 * sa];
             const double* B = &b->xyz[3 * sb];
             const double AB2 = (A[0] - B[0]) * (A[0] - B[0]) + (A[1] - B[1]) * (A[1] - B[1]) +
                                (A[2] - B[2]) * (A[2] - B[2]);
+            int kept = 0;
             for (int i = 0; i < nprim[sa]; ++i)
                 for (int j = 0; j < nprim[sb]; ++j) {
                     const double ea = b->expn[b->pstart[sa] + i], eb = b->expn[b->pstart[sb] + j];
@@ -322,8 +344,12 @@ extern "C" int sf_basis_create(int nshell, const int* L, const int* nprim, const
                     pp.ip = 1.0 / p;
                     pp.Kp = pp.K / p;
                     pp.eb = eb;
+                    if (34.986836655249725 * std::fabs(pp.Kp) * kp_max / std::sqrt(p) < kPrimTol)
+                        continue;
                     b->prim.push_back(pp);
+                    ++kept;
                 }
+            P.np.push_back(kept);
         }
     if (b->pair_slot.size() > (1u << 24)) {
         set_error(kEUnsupported, "too many shell pairs");
@@ -624,7 +650,8 @@ extern "C" int sf_quartet_counts(const sf_basis* b, long long* counts, long long
     return kOk;
 }
 
-extern "C" int sf_pair_list(const sf_basis* b, int pc, int* n, int* sa, int* sb, double* Q)
+extern "C" int sf_pair_list(const sf_basis* b, int pc, int* n, int* sa, int* sb, double* Q,
+                            int* nprim)
 {
     if (!b || pc < 0 || pc >= kNumPairClass || !n) {
         set_error(kEInval, "sf_pair_list: bad arguments");
@@ -640,6 +667,7 @@ extern "C" int sf_pair_list(const sf_basis* b, int pc, int* n, int* sa, int* sb,
         if (sa) sa[i] = P.s_sa[i];
         if (sb) sb[i] = P.s_sb[i];
         if (Q) Q[i] = P.s_Q[i];
+        if (nprim) nprim[i] = P.s_np[i];
     }
     return kOk;
 }

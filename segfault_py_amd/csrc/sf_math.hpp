@@ -43,6 +43,38 @@ namespace sf {
 constexpr double kPi = 3.14159265358979323846264338;
 constexpr double kSqrtPi = 1.77245385090551602729816748;
 
+// ------------------------------------------------- reciprocal, 1/sqrt ----
+// Hardware seeds (v_rcp_f64 / v_rsq_f64) refined by two Newton steps to
+// full double precision, for the positive normal arguments of the kernels
+// (p + q, X >= 39.95, F_0).  The host build seeds from the exact value with
+// a deliberate 3e-9 relative error so the CPU harness exercises the
+// refinement itself.
+SF_HD SF_INLINE double fast_rcp(double x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    double y = __builtin_amdgcn_rcp(x);
+#else
+    double y = (1.0 / x) * (1.0 + 3e-9);
+#endif
+    double e = fma(-x, y, 1.0);
+    y = fma(y, e, y);
+    e = fma(-x, y, 1.0);
+    return fma(y, e, y);
+}
+
+SF_HD SF_INLINE double fast_rsqrt(double x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    double y = __builtin_amdgcn_rsq(x);
+#else
+    double y = (1.0 / sqrt(x)) * (1.0 + 3e-9);
+#endif
+    double e = fma(-x * y, y, 1.0);  // 1 - x y^2
+    y = fma(0.5 * y, e, y);
+    e = fma(-x * y, y, 1.0);
+    return fma(0.5 * y, e, y);
+}
+
 // ---------------------------------------------------------------- Boys ----
 // 1/(2m-1) for the downward recursion, m = 1..34.
 SF_HD SF_INLINE double inv_odd(int m)
@@ -101,9 +133,8 @@ SF_HD SF_INLINE void boys01(double X, double& F0, double& F1)
         F0 = taylor9(g, d);
         F1 = taylor9(g + 1, d);
     } else {
-        const double ix = 1.0 / X;
-        F0 = 0.5 * kSqrtPi * sqrt(ix);
-        F1 = 0.5 * F0 * ix;
+        F0 = 0.5 * kSqrtPi * fast_rsqrt(X);
+        F1 = 0.5 * F0 * fast_rcp(X);
     }
 }
 
@@ -166,8 +197,8 @@ SF_HD SF_INLINE void rys_roots(double X, double* u, double* w)
             else w[v - N] = val;
         }
     } else {
-        const double ix = 1.0 / X;
-        const double isx = sqrt(ix);
+        const double ix = fast_rcp(X);
+        const double isx = fast_rsqrt(X);
 #pragma unroll
         for (int r = 0; r < N; ++r) {
             u[r] = herm_u<N>(r) * ix;
@@ -181,7 +212,7 @@ SF_HD SF_INLINE void rys_roots1_closed(double X, double* u, double* w)
 {
     double F0, F1;
     boys01(X, F0, F1);
-    u[0] = F1 / F0;
+    u[0] = F1 * fast_rcp(F0);
     w[0] = F0;
 }
 

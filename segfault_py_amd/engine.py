@@ -72,7 +72,7 @@ def lib():
     L.sf_screen.argtypes = [vp, ctypes.c_double, ctypes.c_double]
     L.sf_quartet_counts.argtypes = [vp, _i64p, ctypes.POINTER(ctypes.c_longlong),
                                     ctypes.POINTER(ctypes.c_longlong)]
-    L.sf_pair_list.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), vp, vp, vp]
+    L.sf_pair_list.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), vp, vp, vp, vp]
     L.sf_quartet_list.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), vp, vp]
     L.sf_eri_dense.argtypes = [vp, _f64p]
     L.sf_jk.argtypes = [vp, vp, vp, vp, ctypes.c_int, vp]
@@ -222,16 +222,21 @@ class Basis:
         check(lib().sf_quartet_counts(self._h, c, ctypes.byref(fq), ctypes.byref(pq)))
         return c.reshape(NUM_QUARTET_CLASSES, 3), fq.value, pq.value
 
-    def pair_list(self, pc: int):
+    def pair_list(self, pc: int, with_nprim: bool = False):
+        """Sorted significant pairs of pair class pc: (sa, sb, Q[, nprim])."""
         n = ctypes.c_int()
-        check(lib().sf_pair_list(self._h, pc, ctypes.byref(n), None, None, None))
+        check(lib().sf_pair_list(self._h, pc, ctypes.byref(n), None, None, None, None))
         sa = np.zeros(max(n.value, 1), dtype=np.int32)
         sb = np.zeros_like(sa)
+        npp = np.zeros_like(sa)
         Q = np.zeros(max(n.value, 1))
-        check(lib().sf_pair_list(self._h, pc, ctypes.byref(n), sa.ctypes.data_as(ctypes.c_void_p),
-                                 sb.ctypes.data_as(ctypes.c_void_p),
-                                 Q.ctypes.data_as(ctypes.c_void_p)))
+        vp = ctypes.c_void_p
+        check(lib().sf_pair_list(self._h, pc, ctypes.byref(n), sa.ctypes.data_as(vp),
+                                 sb.ctypes.data_as(vp), Q.ctypes.data_as(vp),
+                                 npp.ctypes.data_as(vp)))
         k = n.value
+        if with_nprim:
+            return sa[:k], sb[:k], Q[:k], npp[:k]
         return sa[:k], sb[:k], Q[:k]
 
     def quartet_list(self, c: int):

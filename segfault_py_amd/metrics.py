@@ -33,14 +33,14 @@ def flops_per_prim_quartet(la, lb, lc, ld) -> int:
 
 def class_work(basis, mol=None) -> list[dict]:
     """Per quartet class: shell / primitive / unique-function quartet counts
-    (all surviving, and the FP64 part) and algorithmic flops."""
-    shells = (mol or basis.mol).shells()
-    nprim = np.array([s.nprim for s in shells], dtype=np.int64)
+    (all surviving, and the FP64 part) and algorithmic flops.  Primitive
+    quartets count only the primitive pairs kept by primitive screening,
+    i.e. the ones the kernels evaluate."""
     out = []
     pairs = {}
     for pc in range(6):
-        sa, sb, _ = basis.pair_list(pc)
-        pairs[pc] = (sa, sb)
+        sa, sb, _, npp = basis.pair_list(pc, with_nprim=True)
+        pairs[pc] = (sa, sb, npp.astype(np.int64))
     for c in range(NUM_QUARTET_CLASSES):
         la, lb, lc, ld = quartet_class_ls(c)
         pcb = la * (la + 1) // 2 + lb
@@ -49,10 +49,8 @@ def class_work(basis, mol=None) -> list[dict]:
         rec = {"class": c, "ls": (la, lb, lc, ld), "shellq": 0, "shellq64": 0, "primq": 0,
                "primq64": 0, "funcq": 0, "funcq64": 0, "flops": 0.0, "flops64": 0.0}
         if len(n_all) and len(pairs[pck][0]):
-            sa, sb = pairs[pcb]
-            ka, kb = pairs[pck]
-            npb = nprim[sa] * nprim[sb]
-            npk = nprim[ka] * nprim[kb]
+            sa, sb, npb = pairs[pcb]
+            ka, kb, npk = pairs[pck]
             nA, nB, nC, nD = ncart(la), ncart(lb), ncart(lc), ncart(ld)
             pab = np.where(sa == sb, nA * (nA + 1) // 2, nA * nB).astype(np.int64)
             pcd = np.where(ka == kb, nC * (nC + 1) // 2, nC * nD).astype(np.int64)
