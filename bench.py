@@ -205,6 +205,15 @@ def main():
                      "kernel_ms": dom_ms,
                      "note": "FP64 vector-ALU bound; peak = MI355X FP64 (vector = matrix)"},
     }
+    # full SCF (DIIS, tol 1e-8) in FP64 and in mixed precision on the same basis
+    from segfault_py_amd import hf
+    r64 = hf.scf_diis(mol, basis=basis, precision=0)
+    rmx = hf.scf_diis(mol, basis=basis, precision=1)
+    line["scf"] = {"energy_fp64": r64.energy, "energy_mixed": rmx.energy,
+                   "dE_scf_mixed_vs_fp64": abs(rmx.energy - r64.energy),
+                   "converged": bool(r64.converged and rmx.converged),
+                   "iterations_fp64": r64.iterations, "iterations_mixed": rmx.iterations,
+                   "wall_s_fp64": r64.wall_s, "wall_s_mixed": rmx.wall_s}
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(basis, mol, work, args.cpu_seconds)
     print(json.dumps(line))

@@ -69,6 +69,14 @@ int sf_basis_create(int nshell, const int *L, const int *nprim, const double *ce
 int sf_basis_destroy(sf_basis *b);
 int sf_basis_nbf(const sf_basis *b, int *nbf);
 
+/* Point charges for the nuclear-attraction integrals: Z[natom], xyz[3*natom]
+ * (bohr).  Replaces the atom loop of hf.py:266-274 (mol.atoms[i].xyz, number). */
+int sf_basis_set_nuclei(sf_basis *b, int natom, const double *Z, const double *xyz);
+/* One-electron integrals S, T, V (device buffers, N x N row-major; V with the
+ * -Z_C factor), drop-in for mmd2.overlap / kinetic / potential summed as in
+ * hf.py:266-274 (mmd2.py:14-84). */
+int sf_one_electron(sf_basis *b, double *S, double *T, double *V, void *stream);
+
 /* Schwarz factors Q_ab = sqrt(max |(ij|ij)|), i in a, j in b, for every
  * shell pair a >= b, at Q[a*(a+1)/2 + b] (host buffer).  No counterpart in
  * the reference (which screens nothing). */

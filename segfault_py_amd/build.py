@@ -19,7 +19,8 @@ OUT = os.path.join(HERE, "libsfeng.so")
 # quartet class so the 21 classes build in parallel.
 UNITS = ([("k_eri.hip", [], "k_eri_dispatch.o")]
          + [("k_eri.hip", [f"-DSF_CLASS={c}"], f"k_eri_c{c:02d}.o") for c in range(21)]
-         + [("k_util.hip", [], "k_util.o"), ("sf_api.hip", [], "sf_api.o")])
+         + [("k_util.hip", [], "k_util.o"), ("k_onee.hip", [], "k_onee.o"),
+            ("sf_api.hip", [], "sf_api.o")])
 ARCH = os.environ.get("SF_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [

@@ -1,0 +1,72 @@
+// k_onee.hip -- one-electron integrals S, T, V on the GPU (McMurchie-Davidson).
+//
+// Reference: mmd2.overlap / kinetic / potential (src/hf/mmd2.py:14-84) over
+// the Fortran mmd module (core.f90:30-97), assembled per function pair in
+// Python loops at hf.py:266-274.  Here one thread computes one shell pair
+// (all components, all primitive pairs, all nuclei) and writes its block
+// and the transposed block: every matrix element has exactly one writer.
+//   S_ab = K (pi/p)^{3/2} E^x_0 E^y_0 E^z_0,
+//   T_ab = sum_k [-2 b^2 S^k_{a,b+2} + b (2 b_k + 1) S^k_{ab} - b_k(b_k-1)/2 S^k_{a,b-2}] prod_{k'!=k} S^k'
+//          (mmd2.kinetic_3d, mmd2.py:31-42),
+//   V_ab = sum_C -Z_C (2 pi / p) K sum_{tuv} E^x_t E^y_u E^z_v R_{tuv}(p, P - C)
+//          (mmd2.potential_3d, mmd2.py:53-76; the -Z_C factor of hf.py:85-88).
+#include "sf_internal.hpp"
+
+namespace sf {
+
+template <int LA, int LB>
+__global__ __launch_bounds__(128) void k_one_electron(const int* __restrict__ sa, const int* __restrict__ sb,
+                                                      const int* __restrict__ poff, const int* __restrict__ np,
+                                                      int n, const PrimPair* __restrict__ prim,
+                                                      const double* __restrict__ xyz,
+                                                      const int* __restrict__ foff,
+                                                      const double* __restrict__ fnorm, int natom,
+                                                      const double* __restrict__ Zc,
+                                                      const double* __restrict__ Cxyz, int nbf,
+                                                      double* __restrict__ S, double* __restrict__ T,
+                                                      double* __restrict__ V)
+{
+    constexpr int NA = cart_n(LA), NB = cart_n(LB);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int a = sa[i], b = sb[i];
+    const double A[3] = {xyz[3 * a], xyz[3 * a + 1], xyz[3 * a + 2]};
+    const double B[3] = {xyz[3 * b], xyz[3 * b + 1], xyz[3 * b + 2]};
+    double s[NA * NB], t[NA * NB], v[NA * NB];
+    one_electron_shell_pair<LA, LB>(prim + poff[i], np[i], A, B, natom, Zc, Cxyz, s, t, v);
+    const int r0 = foff[a], c0 = foff[b];
+#pragma unroll
+    for (int fa = 0; fa < NA; ++fa)
+#pragma unroll
+        for (int fb = 0; fb < NB; ++fb) {
+            const double nn = fnorm[r0 + fa] * fnorm[c0 + fb];
+            const long long ij = (long long)(r0 + fa) * nbf + c0 + fb;
+            const long long ji = (long long)(c0 + fb) * nbf + r0 + fa;
+            S[ij] = S[ji] = nn * s[fa * NB + fb];
+            T[ij] = T[ji] = nn * t[fa * NB + fb];
+            V[ij] = V[ji] = nn * v[fa * NB + fb];
+        }
+}
+
+int launch_one_electron(int pc, const int* sa, const int* sb, const int* poff, const int* np, int n,
+                        const PrimPair* prim, const double* xyz, const int* foff,
+                        const double* fnorm, int natom, const double* Z, const double* Cxyz, int nbf,
+                        double* S, double* T, double* V, hipStream_t s)
+{
+    if (n <= 0) return kOk;
+    dim3 g((n + 127) / 128), b(128);
+#define O_(id, la, lb)                                                                          \
+    case id:                                                                                    \
+        hipLaunchKernelGGL((k_one_electron<la, lb>), g, b, 0, s, sa, sb, poff, np, n, prim, xyz, \
+                           foff, fnorm, natom, Z, Cxyz, nbf, S, T, V);                          \
+        break;
+    switch (pc) {
+        O_(0, 0, 0) O_(1, 1, 0) O_(2, 1, 1) O_(3, 2, 0) O_(4, 2, 1) O_(5, 2, 2)
+    default: set_error(kEUnsupported, "pair class %d", pc); return kEUnsupported;
+    }
+#undef O_
+    SF_HIP(hipGetLastError());
+    return kOk;
+}
+
+}  // namespace sf

@@ -1,0 +1,170 @@
+// mmd.hpp -- McMurchie-Davidson building blocks (host+device):
+// Hermite expansion coefficients E^{ij}_t and Coulomb auxiliary R_{tuv}.
+//
+// Reference counterparts: mmd.hermite_gauss_e (src/hf/core/core.f90:30-61)
+// and mmd.coulomb_auxiliary_r (core.f90:64-97), there evaluated by
+// exponential-cost recursion per call; here by the same recurrences run
+// bottom-up once per primitive pair into small register tables.
+//   E^{00}_0 = 1 (the exp(-mu X_AB^2) factor is carried by K of the pair),
+//   E^{i+1,j}_t = E^{ij}_{t-1}/(2p) + X_PA E^{ij}_t + (t+1) E^{ij}_{t+1},
+//   E^{i,j+1}_t = E^{ij}_{t-1}/(2p) + X_PB E^{ij}_t + (t+1) E^{ij}_{t+1};
+//   R^n_{000} = (-2p)^n F_n(p R_PC^2),
+//   R^n_{t+1,u,v} = t R^{n+1}_{t-1,u,v} + X_PC R^{n+1}_{tuv} (same in u, v).
+#pragma once
+#include "eri_core.hpp"
+
+namespace sf {
+
+// E[i][j][t] for i <= IM, j <= JM, t <= i + j (entries with t > i+j are 0).
+template <int IM, int JM>
+SF_HD SF_INLINE void hermite_E(double ip2, double PA, double PB, double (&E)[IM + 1][JM + 1][IM + JM + 1])
+{
+    constexpr int NT = IM + JM + 1;
+#pragma unroll
+    for (int i = 0; i <= IM; ++i)
+#pragma unroll
+        for (int j = 0; j <= JM; ++j)
+#pragma unroll
+            for (int t = 0; t < NT; ++t) E[i][j][t] = 0.0;
+    E[0][0][0] = 1.0;
+#pragma unroll
+    for (int i = 0; i < IM; ++i)
+#pragma unroll
+        for (int t = 0; t <= i + 1; ++t) {
+            double v = PA * E[i][0][t];
+            if (t > 0) v += ip2 * E[i][0][t - 1];
+            if (t + 1 <= i) v += (t + 1) * E[i][0][t + 1];
+            E[i + 1][0][t] = v;
+        }
+#pragma unroll
+    for (int i = 0; i <= IM; ++i)
+#pragma unroll
+        for (int j = 0; j < JM; ++j)
+#pragma unroll
+            for (int t = 0; t <= i + j + 1; ++t) {
+                double v = PB * E[i][j][t];
+                if (t > 0) v += ip2 * E[i][j][t - 1];
+                if (t + 1 <= i + j) v += (t + 1) * E[i][j][t + 1];
+                E[i][j + 1][t] = v;
+            }
+}
+
+// R[t][u][v] = R^0_{tuv} for t + u + v <= L, from F_0..F_L at T = p R_PC^2.
+template <int L>
+SF_HD SF_INLINE void coulomb_R(double p, double X, double Y, double Z, const double* F,
+                               double (&R)[L + 1][L + 1][L + 1])
+{
+    // Rn[n][t][u][v]: build from n = L down to 0
+    double Rn[L + 1][L + 1][L + 1][L + 1];
+    double m2p = 1.0;
+#pragma unroll
+    for (int n = 0; n <= L; ++n) {
+#pragma unroll
+        for (int t = 0; t <= L; ++t)
+#pragma unroll
+            for (int u = 0; u <= L; ++u)
+#pragma unroll
+                for (int v = 0; v <= L; ++v) Rn[n][t][u][v] = 0.0;
+        Rn[n][0][0][0] = m2p * F[n];
+        m2p *= -2.0 * p;
+    }
+#pragma unroll
+    for (int n = L - 1; n >= 0; --n) {
+        // fill all t+u+v <= L-n from level n+1
+#pragma unroll
+        for (int s = 1; s <= L - n; ++s)
+#pragma unroll
+            for (int t = 0; t <= s; ++t)
+#pragma unroll
+                for (int u = 0; u <= s - t; ++u) {
+                    const int v = s - t - u;
+                    double r;
+                    if (t > 0) {
+                        r = X * Rn[n + 1][t - 1][u][v];
+                        if (t > 1) r += (t - 1) * Rn[n + 1][t - 2][u][v];
+                    } else if (u > 0) {
+                        r = Y * Rn[n + 1][t][u - 1][v];
+                        if (u > 1) r += (u - 1) * Rn[n + 1][t][u - 2][v];
+                    } else {
+                        r = Z * Rn[n + 1][t][u][v - 1];
+                        if (v > 1) r += (v - 1) * Rn[n + 1][t][u][v - 2];
+                    }
+                    Rn[n][t][u][v] = r;
+                }
+    }
+#pragma unroll
+    for (int t = 0; t <= L; ++t)
+#pragma unroll
+        for (int u = 0; u <= L; ++u)
+#pragma unroll
+            for (int v = 0; v <= L; ++v) R[t][u][v] = Rn[0][t][u][v];
+}
+
+// S, T, V blocks (shell-normalised components, row-major NA x NB) of one
+// shell pair over its primitive pairs and natom point charges Z at Cxyz
+// (reference: mmd2.py:14-84; the kinetic formula of kinetic_3d, :31-42).
+template <int LA, int LB>
+SF_HD SF_INLINE void one_electron_shell_pair(const PrimPair* __restrict__ prim, int np,
+                                             const double* A, const double* B, int natom,
+                                             const double* __restrict__ Zc,
+                                             const double* __restrict__ Cxyz, double* s, double* t,
+                                             double* v)
+{
+    constexpr int NA = cart_n(LA), NB = cart_n(LB), L = LA + LB;
+#pragma unroll
+    for (int e = 0; e < NA * NB; ++e) s[e] = t[e] = v[e] = 0.0;
+    for (int k = 0; k < np; ++k) {
+        const PrimPair pp = prim[k];
+        const double p = pp.p, eb = pp.eb;
+        const double P[3] = {pp.Px, pp.Py, pp.Pz};
+        double E[3][LA + 1][LB + 3][LA + LB + 3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) hermite_E<LA, LB + 2>(0.5 * pp.ip, P[d] - A[d], P[d] - B[d], E[d]);
+        const double sp = kPi * pp.ip;
+        const double pre_s = pp.K * sp * sqrt(sp);
+#pragma unroll
+        for (int fa = 0; fa < NA; ++fa)
+#pragma unroll
+            for (int fb = 0; fb < NB; ++fb) {
+                double S1[3], T1[3];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    const int ad = cart_l(LA, fa, d), bd = cart_l(LB, fb, d);
+                    S1[d] = E[d][ad][bd][0];
+                    double tk = -2.0 * eb * eb * E[d][ad][bd + 2][0] + eb * (2 * bd + 1) * S1[d];
+                    if (bd >= 2) tk -= 0.5 * bd * (bd - 1) * E[d][ad][bd - 2][0];
+                    T1[d] = tk;
+                }
+                s[fa * NB + fb] += pre_s * S1[0] * S1[1] * S1[2];
+                t[fa * NB + fb] += pre_s * (T1[0] * S1[1] * S1[2] + S1[0] * T1[1] * S1[2] +
+                                            S1[0] * S1[1] * T1[2]);
+            }
+        for (int c = 0; c < natom; ++c) {
+            const double PC[3] = {P[0] - Cxyz[3 * c], P[1] - Cxyz[3 * c + 1], P[2] - Cxyz[3 * c + 2]};
+            double F[SF_BOYS_MTAB];
+            boys_grid(L, p * (PC[0] * PC[0] + PC[1] * PC[1] + PC[2] * PC[2]), F);
+            double R[L + 1][L + 1][L + 1];
+            coulomb_R<L>(p, PC[0], PC[1], PC[2], F, R);
+            const double pre_v = -Zc[c] * 2.0 * kPi * pp.ip * pp.K;
+#pragma unroll
+            for (int fa = 0; fa < NA; ++fa)
+#pragma unroll
+                for (int fb = 0; fb < NB; ++fb) {
+                    const int ax = cart_l(LA, fa, 0), ay = cart_l(LA, fa, 1), az = cart_l(LA, fa, 2);
+                    const int bx = cart_l(LB, fb, 0), by = cart_l(LB, fb, 1), bz = cart_l(LB, fb, 2);
+                    double acc = 0.0;
+#pragma unroll
+                    for (int tt = 0; tt <= ax + bx; ++tt)
+#pragma unroll
+                        for (int uu = 0; uu <= ay + by; ++uu)
+#pragma unroll
+                            for (int vv = 0; vv <= az + bz; ++vv)
+                                acc += E[0][ax][bx][tt] * E[1][ay][by][uu] * E[2][az][bz][vv] *
+                                       R[tt][uu][vv];
+                    v[fa * NB + fb] += pre_v * acc;
+                }
+        }
+    }
+}
+
+}  // namespace sf

@@ -127,6 +127,9 @@ struct sf_basis {
     std::vector<double> xyz, expn, coef, fnorm;
     std::vector<PrimPair> prim;
     DevBuf d_prim, d_xyz, d_foff, d_fnorm, d_zero;
+    // nuclei (point charges) for the one-electron potential
+    int nnuc = 0;
+    DevBuf d_Z, d_nuc;
     PairClassData pc[kNumPairClass];
     std::vector<int> pair_slot;  // for pair (a>=b) index: (pc << 24) | position in pc arrays
     bool have_schwarz = false, screened = false;
@@ -864,4 +867,33 @@ extern "C" int sf_jk_profile(sf_basis* b, const double* P, double* J, double* K,
         return kEInval;
     }
     return run_jk(b, P, J, K, precision, (hipStream_t)stream, ms);
+}
+
+extern "C" int sf_basis_set_nuclei(sf_basis* b, int natom, const double* Z, const double* xyz)
+{
+    if (!b || natom < 0 || (natom > 0 && (!Z || !xyz))) {
+        set_error(kEInval, "sf_basis_set_nuclei: bad arguments");
+        return kEInval;
+    }
+    b->nnuc = natom;
+    SF_TRY(b->d_Z.upload(std::vector<double>(Z, Z + natom)));
+    SF_TRY(b->d_nuc.upload(std::vector<double>(xyz, xyz + 3 * natom)));
+    return kOk;
+}
+
+extern "C" int sf_one_electron(sf_basis* b, double* S, double* T, double* V, void* stream)
+{
+    if (!b || !S || !T || !V) {
+        set_error(kEInval, "sf_one_electron: null argument");
+        return kEInval;
+    }
+    for (int p = 0; p < kNumPairClass; ++p) {
+        PairClassData& P = b->pc[p];
+        SF_TRY(launch_one_electron(p, P.d_sa.as<int>(), P.d_sb.as<int>(), P.d_poff.as<int>(),
+                                   P.d_np.as<int>(), (int)P.sa.size(), b->d_prim.as<PrimPair>(),
+                                   b->d_xyz.as<double>(), b->d_foff.as<int>(),
+                                   b->d_fnorm.as<double>(), b->nnuc, b->d_Z.as<double>(),
+                                   b->d_nuc.as<double>(), b->nbf, S, T, V, (hipStream_t)stream));
+    }
+    return kOk;
 }

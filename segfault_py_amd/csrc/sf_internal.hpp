@@ -5,7 +5,7 @@
 
 #include <cstdint>
 
-#include "eri_core.hpp"
+#include "mmd.hpp"
 
 namespace sf {
 
@@ -107,6 +107,10 @@ int launch_scale_density(const double* P, const double* fnorm, int nbf, double* 
 int launch_finalize(const double* X, const double* Y, const double* fnorm, int nbf, double* J,
                     double* K, hipStream_t s);
 int launch_fill(double* p, long long n, double v, hipStream_t s);
+int launch_one_electron(int pc, const int* sa, const int* sb, const int* poff, const int* np, int n,
+                        const PrimPair* prim, const double* xyz, const int* foff,
+                        const double* fnorm, int natom, const double* Z, const double* Cxyz, int nbf,
+                        double* S, double* T, double* V, hipStream_t s);
 int launch_eri_components(int nf, const int* npgf, const int* poff, const double* coefs,
                           const double* expns, const double* norms, const int* shell,
                           const double* center, long long nq, const int* quartets, double* out,

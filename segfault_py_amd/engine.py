@@ -76,6 +76,8 @@ def lib():
     L.sf_quartet_list.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), vp, vp]
     L.sf_eri_dense.argtypes = [vp, _f64p]
     L.sf_jk.argtypes = [vp, vp, vp, vp, ctypes.c_int, vp]
+    L.sf_basis_set_nuclei.argtypes = [vp, ctypes.c_int, _f64p, _f64p]
+    L.sf_one_electron.argtypes = [vp, vp, vp, vp, vp]
     L.sf_jk_profile.argtypes = [vp, vp, vp, vp, ctypes.c_int, vp, _f64p]
     _LIB = L
     return L
@@ -91,7 +93,7 @@ def exported_symbols() -> list[str]:
     return ["sf_last_error", "sf_version", "sf_boys", "sf_rys_roots", "sf_eri_rys",
             "sf_eri_components", "sf_basis_create", "sf_basis_destroy", "sf_basis_nbf",
             "sf_schwarz", "sf_screen", "sf_quartet_counts", "sf_pair_list", "sf_quartet_list",
-            "sf_eri_dense", "sf_jk", "sf_jk_profile"]
+            "sf_eri_dense", "sf_jk", "sf_jk_profile", "sf_basis_set_nuclei", "sf_one_electron"]
 
 
 def _torch():
@@ -189,6 +191,9 @@ class Basis:
         n = ctypes.c_int()
         check(lib().sf_basis_nbf(self._h, ctypes.byref(n)))
         self.nbf = n.value
+        check(lib().sf_basis_set_nuclei(self._h, mol.natom,
+                                        np.ascontiguousarray(mol.numbers, dtype=np.float64),
+                                        np.ascontiguousarray(mol.xyz__, dtype=np.float64).reshape(-1)))
         self.tau = None
         self.tau_mixed = None
 
@@ -202,6 +207,17 @@ class Basis:
             self.close()
         except Exception:
             pass
+
+    def one_electron(self):
+        """S, T, V (float64 CUDA tensors, N x N)."""
+        torch = _torch()
+        S = torch.empty((self.nbf, self.nbf), dtype=torch.float64, device="cuda")
+        T = torch.empty_like(S)
+        V = torch.empty_like(S)
+        check(lib().sf_one_electron(self._h, ctypes.c_void_p(S.data_ptr()),
+                                    ctypes.c_void_p(T.data_ptr()), ctypes.c_void_p(V.data_ptr()),
+                                    _stream_ptr(torch)))
+        return S, T, V
 
     def schwarz(self) -> np.ndarray:
         """Q[a*(a+1)/2 + b] for shell pairs a >= b."""

@@ -35,6 +35,8 @@ def harness():
         H.h_rys.argtypes = [ctypes.c_int, ctypes.c_double, _f64p, _f64p]
         H.h_boys.argtypes = [ctypes.c_int, ctypes.c_double, _f64p]
         H.h_boys01.argtypes = [ctypes.c_double, _f64p]
+        H.h_one_electron.argtypes = [ctypes.c_int, ctypes.c_int, _f64p, ctypes.c_int, _f64p, _f64p,
+                                     ctypes.c_int, _f64p, _f64p, _f64p, _f64p, _f64p]
         _H = H
     return _H
 
@@ -166,3 +168,27 @@ def jk_model(mol, P, tau=1e-12, tau_mixed=0.0, fp32_below=False):
                         o += n1 * n2
     D = fn[:, None] * fn[None, :]
     return D * (X + X.T), D * (Y + Y.T)
+
+
+def one_electron_model(mol):
+    """S, T, V assembled from the kernels' per-shell-pair routine on the CPU."""
+    shells = mol.shells()
+    fn = mol.function_norms()
+    n = mol.nbf
+    S, T, V = np.zeros((n, n)), np.zeros((n, n)), np.zeros((n, n))
+    Z = np.ascontiguousarray(mol.numbers, dtype=np.float64)
+    C = np.ascontiguousarray(mol.xyz__, dtype=np.float64).reshape(-1)
+    for a in range(len(shells)):
+        for b in range(a + 1):
+            sa, sb = (a, b) if shells[a].L >= shells[b].L else (b, a)
+            A, B = shells[sa], shells[sb]
+            nA, nB = A.nfunc, B.nfunc
+            s, t, v = np.zeros(nA * nB), np.zeros(nA * nB), np.zeros(nA * nB)
+            assert harness().h_one_electron(A.L, B.L, prim_pairs(A, B), A.nprim * B.nprim, A.center,
+                                            B.center, len(Z), Z, C, s, t, v) == 0
+            r, c = A.offset, B.offset
+            for M, x in ((S, s), (T, t), (V, v)):
+                blk = x.reshape(nA, nB) * fn[r:r + nA, None] * fn[None, c:c + nB]
+                M[r:r + nA, c:c + nB] = blk
+                M[c:c + nB, r:r + nA] = blk.T
+    return S, T, V

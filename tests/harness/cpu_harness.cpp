@@ -1,7 +1,7 @@
 // CPU harness: compiles the HIP kernels' host+device math headers with g++
 // so that tests can check them against the oracle without a GPU.
 // Test infrastructure only.
-#include "../../segfault_py_amd/csrc/eri_core.hpp"
+#include "../../segfault_py_amd/csrc/mmd.hpp"
 
 using namespace sf;
 
@@ -136,5 +136,20 @@ extern "C" int h_digest_tiles(int la, int lb, int lc, int ld, const double* v, d
     }
     SF_CLASSES(X)
 #undef X
+    return 1;
+}
+
+extern "C" int h_one_electron(int la, int lb, const double* prim, int np, const double* A,
+                              const double* B, int natom, const double* Z, const double* C,
+                              double* s, double* t, double* v)
+{
+    const PrimPair* pp = (const PrimPair*)prim;
+#define O_(x, y)                                                                     \
+    if (la == x && lb == y) {                                                        \
+        one_electron_shell_pair<x, y>(pp, np, A, B, natom, Z, C, s, t, v);           \
+        return 0;                                                                    \
+    }
+    O_(0, 0) O_(1, 0) O_(1, 1) O_(2, 0) O_(2, 1) O_(2, 2)
+#undef O_
     return 1;
 }

@@ -175,3 +175,13 @@ def test_digest_tiles_equal_digest_quartet():
         assert H.h_digest(*ls, np.ascontiguousarray(v * 0.25), P, nbf, f0, t1) == 0
         assert H.h_digest_tiles(*ls, v, 0.25, P, nbf, f0, t2) == 0
         assert np.max(np.abs(t1 - t2)) < 1e-12
+
+
+@pytest.mark.parametrize("fixture", ["water_sp", "water_ccpvdz"])
+def test_one_electron_model_matches_oracle(oracle, request, fixture):
+    mol = request.getfixturevalue(fixture)
+    S, T, V = CM.one_electron_model(mol)
+    Sr, Tr, Vr = oracle.one_electron(mol, 8)
+    assert np.max(np.abs(S - Sr)) < 1e-13
+    assert np.max(np.abs(T - Tr)) < 1e-12
+    assert np.max(np.abs(V - Vr)) < 1e-12

@@ -176,3 +176,35 @@ def test_scf_energy_with_gpu_jk(gpu, oracle, name, group):
 
     E, _, conv, _ = oracle.scf_diis(S, T + V, int(mol.numbers.sum()) // 2, jk, mol.e_nuc)
     assert conv and abs(E - ref["etot"]) < 1e-7, (E, ref["etot"])
+
+
+def test_one_electron_kernel_matches_oracle(gpu, oracle, water_ccpvdz):
+    from segfault_py_amd import engine
+    S, T, V = engine.Basis(water_ccpvdz).one_electron()
+    Sr, Tr, Vr = oracle.one_electron(water_ccpvdz, 8)
+    assert np.max(np.abs(S.cpu().numpy() - Sr)) < 1e-13
+    assert np.max(np.abs(T.cpu().numpy() - Tr)) < 1e-12
+    assert np.max(np.abs(V.cpu().numpy() - Vr)) < 1e-12
+
+
+@pytest.mark.parametrize("name,group", [("water", "others"), ("methane", "alkane"),
+                                        ("ammonia", "others"), ("ethene", "alkene")])
+def test_gpu_scf_diis_matches_reference_energy(gpu, name, group):
+    """Everything on the GPU (S, T, V, J/K, eigh, DIIS) reproduces the
+    reference's published cc-pVDZ energy (molecule_test.json)."""
+    from segfault_py_amd import hf
+    from segfault_py_amd.molecule import Molecule
+    ref = _golden("molecule_test.json")[group][name]
+    mol = Molecule(os.path.join(GOLDEN, "molecules", f"{name}.xyz"),
+                   os.path.join(GOLDEN, "cc-pvdz-HCNO.dat"))
+    r = hf.scf_diis(mol)
+    assert r.converged and abs(r.energy - ref["etot"]) < 1e-7, (r.energy, ref["etot"])
+    # mixed precision stays within 1e-6 Eh of FP64
+    rm = hf.scf_diis(mol, precision=1)
+    assert rm.converged and abs(rm.energy - r.energy) < 1e-6
+
+
+def test_gpu_scf_variants_agree(gpu, water_sp):
+    from segfault_py_amd import hf
+    e = [f(water_sp, tol=1e-10).energy for f in (hf.scf, hf.scf_damping, hf.scf_diis)]
+    assert max(e) - min(e) < 1e-8
