@@ -29,6 +29,18 @@ sys.path.insert(0, ROOT)
 FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 (vector and matrix)
 
 
+def measured_traffic(kernel_ls):
+    """HBM bytes per launch of the dominant kernel from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summary (profiles/), if one exists."""
+    import re
+    name = "k_eri_class_" + "".join(map(str, kernel_ls)) + "_h2o20.md"
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"HBM traffic per launch = ([0-9.eE+]+) bytes", open(path).read())
+    return float(m.group(1)) if m else None
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -37,6 +49,7 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="target host time of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-scf", action="store_true", help="skip the FP64/mixed SCF energy check")
     ap.add_argument("--workload", type=int, default=1, help="workload index (1 = 20 waters)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (for HBM counter scaling)")
@@ -200,20 +213,22 @@ def main():
         "kernel_ms_fp64": kernel_ms,
         "setup_s": setup_s,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
+                     "traffic": measured_traffic(dom["ls"]),
                      "kernel": f"k_eri_class<{','.join(map(str, dom['ls']))},double,JK>",
                      "kernel_ms": dom_ms,
                      "note": "FP64 vector-ALU bound; peak = MI355X FP64 (vector = matrix)"},
     }
     # full SCF (DIIS, tol 1e-8) in FP64 and in mixed precision on the same basis
     from segfault_py_amd import hf
-    r64 = hf.scf_diis(mol, basis=basis, precision=0)
-    rmx = hf.scf_diis(mol, basis=basis, precision=1)
-    line["scf"] = {"energy_fp64": r64.energy, "energy_mixed": rmx.energy,
-                   "dE_scf_mixed_vs_fp64": abs(rmx.energy - r64.energy),
-                   "converged": bool(r64.converged and rmx.converged),
-                   "iterations_fp64": r64.iterations, "iterations_mixed": rmx.iterations,
-                   "wall_s_fp64": r64.wall_s, "wall_s_mixed": rmx.wall_s}
+    if not args.no_scf:
+        r64 = hf.scf_diis(mol, basis=basis, precision=0)
+        rmx = hf.scf_diis(mol, basis=basis, precision=1)
+        line["scf"] = {"energy_fp64": r64.energy, "energy_mixed": rmx.energy,
+                       "dE_scf_mixed_vs_fp64": abs(rmx.energy - r64.energy),
+                       "converged": bool(r64.converged and rmx.converged),
+                       "iterations_fp64": r64.iterations, "iterations_mixed": rmx.iterations,
+                       "wall_s_fp64": r64.wall_s, "wall_s_mixed": rmx.wall_s}
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(basis, mol, work, args.cpu_seconds)
     print(json.dumps(line))
