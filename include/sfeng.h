@@ -117,6 +117,15 @@ int sf_jk(sf_basis *b, const double *P, double *J, double *K, int precision, voi
 int sf_jk_profile(sf_basis *b, const double *P, double *J, double *K, int precision,
                   void *stream, double *ms);
 
+/* Nuclear gradient of the RHF electronic energy (host output grad[3*natom],
+ * hartree/bohr, atom order of the basis): sum P dH/dR - sum W dS/dR +
+ * 1/2 sum Gamma d(pq|rs)/dR for the density P and energy-weighted density
+ * W = 2 sum_occ e_i C_i C_i^T (device buffers, N x N).  Derivative ERIs
+ * over the screened quartet list of sf_screen, deterministic reductions.
+ * No counterpart in the reference (single-point energies only); nuclear
+ * repulsion is not included.  Requires sf_basis_set_nuclei for every atom. */
+int sf_grad(sf_basis *b, const double *P, const double *W, double *grad, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

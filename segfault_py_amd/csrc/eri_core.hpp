@@ -346,6 +346,194 @@ SF_HD SF_INLINE void eri_bra_component(const PrimPair* __restrict__ bra, int nbr
     }
 }
 
+// ---------------------------------------------------------- gradients ----
+// One Cartesian direction of the 2-D integrals around bra exponents (AK, BK)
+// for the nuclear derivatives d/dA, d/dB, d/dC:
+//   I0[ic][id]  = I(AK,   BK,   ic, id), ic <= LC+1,
+//   IAp / IAm   = I(AK+-1, BK,   ic, id), ic <= LC  (IAm unused when AK = 0),
+//   IBp / IBm   = I(AK,   BK+-1, ic, id), ic <= LC  (IBm unused when BK = 0).
+template <int AK, int BK, int LC, int LD, typename R>
+struct DirDeriv {
+    R I0[LC + 2][LD + 1];
+    R IAp[LC + 1][LD + 1], IAm[LC + 1][LD + 1];
+    R IBp[LC + 1][LD + 1], IBm[LC + 1][LD + 1];
+};
+
+template <int AK, int BK, int LC, int LD, typename R>
+SF_HD SF_INLINE void dir_deriv(R g00, R c00, R c00p, R b10, R b01, R b00, R ab, const R* CDp,
+                               DirDeriv<AK, BK, LC, LD, R>& o)
+{
+    constexpr int NN = AK + BK + 2, NM = LC + LD + 2;
+    R G[NN][NM];
+    G[0][0] = g00;
+    G[1][0] = c00 * g00;
+#pragma unroll
+    for (int n = 1; n < NN - 1; ++n) G[n + 1][0] = (R)n * b10 * G[n - 1][0] + c00 * G[n][0];
+#pragma unroll
+    for (int n = 0; n < NN; ++n) {
+        G[n][1] = c00p * G[n][0];
+        if (n > 0) G[n][1] += (R)n * b00 * G[n - 1][0];
+    }
+#pragma unroll
+    for (int m = 1; m < NM - 1; ++m)
+#pragma unroll
+        for (int n = 0; n < NN; ++n) {
+            R g = (R)m * b01 * G[n][m - 1] + c00p * G[n][m];
+            if (n > 0) g += (R)n * b00 * G[n - 1][m];
+            G[n][m + 1] = g;
+        }
+    // ket transfer on every row n, ic <= LC+1
+    R H[NN][LC + 2][LD + 1];
+#pragma unroll
+    for (int n = 0; n < NN; ++n)
+#pragma unroll
+        for (int ic = 0; ic <= LC + 1; ++ic)
+#pragma unroll
+            for (int id = 0; id <= LD; ++id) {
+                R s = (R)0;
+#pragma unroll
+                for (int m = 0; m <= id; ++m)
+                    if (ic + m < NM) s += (R)binom_c(id, m) * CDp[id - m] * G[n][ic + m];
+                H[n][ic][id] = s;
+            }
+    R abp[BK + 2];
+    abp[0] = (R)1;
+#pragma unroll
+    for (int e = 1; e <= BK + 1; ++e) abp[e] = abp[e - 1] * ab;
+    // bra transfer T(ia, ib) = sum_n C(ib,n) AB^(ib-n) H[ia+n]
+#pragma unroll
+    for (int ic = 0; ic <= LC + 1; ++ic)
+#pragma unroll
+        for (int id = 0; id <= LD; ++id) {
+            R s = (R)0;
+#pragma unroll
+            for (int n = 0; n <= BK; ++n) s += (R)binom_c(BK, n) * abp[BK - n] * H[AK + n][ic][id];
+            o.I0[ic][id] = s;
+        }
+#pragma unroll
+    for (int ic = 0; ic <= LC; ++ic)
+#pragma unroll
+        for (int id = 0; id <= LD; ++id) {
+            R sp = (R)0, sm = (R)0, tp = (R)0, tm = (R)0;
+#pragma unroll
+            for (int n = 0; n <= BK; ++n) {
+                sp += (R)binom_c(BK, n) * abp[BK - n] * H[AK + 1 + n][ic][id];
+                if (AK > 0) sm += (R)binom_c(BK, n) * abp[BK - n] * H[(AK > 0 ? AK - 1 : 0) + n][ic][id];
+            }
+#pragma unroll
+            for (int n = 0; n <= BK + 1; ++n) tp += (R)binom_c(BK + 1, n) * abp[BK + 1 - n] * H[AK + n][ic][id];
+            if (BK > 0) {
+#pragma unroll
+                for (int n = 0; n <= (BK > 0 ? BK - 1 : 0); ++n)
+                    tm += (R)binom_c(BK - 1, n) * abp[BK - 1 - n] * H[AK + n][ic][id];
+            }
+            o.IAp[ic][id] = sp;
+            o.IAm[ic][id] = sm;
+            o.IBp[ic][id] = tp;
+            o.IBm[ic][id] = tm;
+        }
+}
+
+// Nuclear-derivative contraction for ONE bra component pair (FA, FB) of a
+// shell quartet: g[0..8] += sum_{fc,fd} D[fc*ND+fd] * d(fa fb|fc fd)/d(A, B, C)
+// (x, y, z each), with d/dA of a primitive 2a (a+1 b| - a_k (a-1 b|, and the
+// same for B (exponent b) and C (exponent c); the D-derivative follows from
+// translational invariance, -(dA + dB + dC).  D is the density factor of
+// the component quartet (see k_grad_split).
+template <int LA, int LB, int LC, int LD, int FA, int FB, typename R>
+SF_HD SF_INLINE void grad_bra_component(const PrimPair* __restrict__ bra, int nbra,
+                                        const PrimPair* __restrict__ ket, int nket,
+                                        const double* A, const double* B, const double* C,
+                                        const double* D, const double* __restrict__ Dfac,
+                                        double* __restrict__ g)
+{
+    using S = QuartetShape<LA, LB, LC, LD>;
+    constexpr int NR = (S::L + 1) / 2 + 1;
+    constexpr int ax = cart_l(LA, FA, 0), ay = cart_l(LA, FA, 1), az = cart_l(LA, FA, 2);
+    constexpr int bx = cart_l(LB, FB, 0), by = cart_l(LB, FB, 1), bz = cart_l(LB, FB, 2);
+    R CDp[3][LD + 1];
+    R AB[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const R cd = (R)(C[k] - D[k]);
+        AB[k] = (R)(A[k] - B[k]);
+        CDp[k][0] = (R)1;
+#pragma unroll
+        for (int e = 1; e <= LD; ++e) CDp[k][e] = CDp[k][e - 1] * cd;
+    }
+    for (int ib = 0; ib < nbra; ++ib) {
+        const PrimPair pb = bra[ib];
+        const double p = pb.p;
+        const R ta = (R)(2.0 * (p - pb.eb)), tb = (R)(2.0 * pb.eb);
+        const double PA[3] = {pb.Px - A[0], pb.Py - A[1], pb.Pz - A[2]};
+        for (int ik = 0; ik < nket; ++ik) {
+            const PrimPair pk = ket[ik];
+            const double q = pk.p;
+            const R tc = (R)(2.0 * (q - pk.eb));
+            const double pq = p + q;
+            const double rs = fast_rsqrt(pq);
+            const double ipq = rs * rs;
+            const double rho = p * q * ipq;
+            const double PQ[3] = {pb.Px - pk.Px, pb.Py - pk.Py, pb.Pz - pk.Pz};
+            const double X = rho * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]);
+            const double pref = 34.986836655249725 * pb.Kp * pk.Kp * rs;
+            double u[NR], w[NR];
+            rys_nodes<NR>(X, u, w);
+            const double QC[3] = {pk.Px - C[0], pk.Py - C[1], pk.Pz - C[2]};
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                const double t2 = u[r];
+                const R b00 = (R)(0.5 * ipq * t2);
+                const R b10 = (R)(0.5 * pb.ip * (1.0 - q * ipq * t2));
+                const R b01 = (R)(0.5 * pk.ip * (1.0 - p * ipq * t2));
+                R c00[3], c00p[3];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    c00[k] = (R)(PA[k] - q * ipq * PQ[k] * t2);
+                    c00p[k] = (R)(QC[k] + p * ipq * PQ[k] * t2);
+                }
+                DirDeriv<ax, bx, LC, LD, R> Dx;
+                DirDeriv<ay, by, LC, LD, R> Dy;
+                DirDeriv<az, bz, LC, LD, R> Dz;
+                dir_deriv<ax, bx, LC, LD, R>((R)(pref * w[r]), c00[0], c00p[0], b10, b01, b00, AB[0],
+                                             CDp[0], Dx);
+                dir_deriv<ay, by, LC, LD, R>((R)1, c00[1], c00p[1], b10, b01, b00, AB[1], CDp[1], Dy);
+                dir_deriv<az, bz, LC, LD, R>((R)1, c00[2], c00p[2], b10, b01, b00, AB[2], CDp[2], Dz);
+#pragma unroll
+                for (int fc = 0; fc < S::NC; ++fc)
+#pragma unroll
+                    for (int fd = 0; fd < S::ND; ++fd) {
+                        const int cx = cart_l(LC, fc, 0), cy = cart_l(LC, fc, 1), cz = cart_l(LC, fc, 2);
+                        const int dx = cart_l(LD, fd, 0), dy = cart_l(LD, fd, 1), dz = cart_l(LD, fd, 2);
+                        const R f = (R)Dfac[fc * S::ND + fd];
+                        const R x0 = Dx.I0[cx][dx], y0 = Dy.I0[cy][dy], z0 = Dz.I0[cz][dz];
+                        // d/dA
+                        const R dax = ta * Dx.IAp[cx][dx] - (R)ax * Dx.IAm[cx][dx];
+                        const R day = ta * Dy.IAp[cy][dy] - (R)ay * Dy.IAm[cy][dy];
+                        const R daz = ta * Dz.IAp[cz][dz] - (R)az * Dz.IAm[cz][dz];
+                        // d/dB
+                        const R dbx = tb * Dx.IBp[cx][dx] - (R)bx * Dx.IBm[cx][dx];
+                        const R dby = tb * Dy.IBp[cy][dy] - (R)by * Dy.IBm[cy][dy];
+                        const R dbz = tb * Dz.IBp[cz][dz] - (R)bz * Dz.IBm[cz][dz];
+                        // d/dC (ket component exponent c_k)
+                        const R dcx = tc * Dx.I0[cx + 1][dx] - (cx > 0 ? (R)cx * Dx.I0[cx > 0 ? cx - 1 : 0][dx] : (R)0);
+                        const R dcy = tc * Dy.I0[cy + 1][dy] - (cy > 0 ? (R)cy * Dy.I0[cy > 0 ? cy - 1 : 0][dy] : (R)0);
+                        const R dcz = tc * Dz.I0[cz + 1][dz] - (cz > 0 ? (R)cz * Dz.I0[cz > 0 ? cz - 1 : 0][dz] : (R)0);
+                        g[0] += (double)(f * dax * y0 * z0);
+                        g[1] += (double)(f * x0 * day * z0);
+                        g[2] += (double)(f * x0 * y0 * daz);
+                        g[3] += (double)(f * dbx * y0 * z0);
+                        g[4] += (double)(f * x0 * dby * z0);
+                        g[5] += (double)(f * x0 * y0 * dbz);
+                        g[6] += (double)(f * dcx * y0 * z0);
+                        g[7] += (double)(f * x0 * dcy * z0);
+                        g[8] += (double)(f * x0 * y0 * dcz);
+                    }
+            }
+        }
+    }
+}
+
 // One digestion tile of a shell quartet from its raw integrals v (plane
 // stride vs): T = 0 X_ab, 1 X_cd, 2 Y_ac, 3 Y_ad, 4 Y_bc, 5 Y_bd (see
 // digest_quartet).  fac is the degeneracy factor.

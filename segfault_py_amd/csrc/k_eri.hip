@@ -32,7 +32,8 @@ using SchwarzLaunch = int (*)(const int*, const int*, const int*, const int*, in
 #define X(id, a, b, c, d)                                                                       \
     int launch_class_##id(int fp32, int mode, const ClassParams& cp, hipStream_t s);           \
     int launch_schwarz_##id(const int*, const int*, const int*, const int*, int,               \
-                            const PrimPair*, const double*, double*, hipStream_t);
+                            const PrimPair*, const double*, double*, hipStream_t);             \
+    int launch_grad_##id(const ClassParams& cp, double* gout, hipStream_t s);
 SF_FOR_EACH_CLASS(X)
 #undef X
 
@@ -183,6 +184,74 @@ __global__ __launch_bounds__(256) void k_digest_split(const ClassParams cp,
     });
 }
 
+// Nuclear gradient, two-electron part: one wavefront per (64 quartets, bra
+// component pair).  Each thread contracts d(fa fb|fc fd)/d(A,B,C) with the
+// density factor fac * [4 P_ab P_cd - P_ac P_bd - P_ad P_bc] (the 8 images of
+// 1/2 sum Gamma_pqrs (pq|rs)^x with Gamma = P_pq P_rs - P_pr P_qs / 2) and
+// writes its 9 partial derivatives to gout[(s*9 + k) * qstride + q].
+template <int LA, int LB, int LC, int LD>
+__global__ __launch_bounds__(256) void k_grad_split(const ClassParams cp, double* __restrict__ gout)
+{
+    using S = QuartetShape<LA, LB, LC, LD>;
+    constexpr int NS = S::NA * S::NB, NK = S::NC * S::ND;
+    const long long gw = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int s = __builtin_amdgcn_readfirstlane((int)(gw % NS));
+    const long long q = (gw / NS) * 64 + (threadIdx.x & 63);
+    if (q >= cp.nq) return;
+    const int i = find_bra(cp.seg_base, cp.bra.n, q);
+    const int j = cp.seg_k0[i] + (int)(q - cp.seg_base[i]);
+    const long long qa = cp.qall_base[i] + j;
+    const int a = cp.bra.sa[i], b = cp.bra.sb[i];
+    const int c = cp.ket.sa[j], d = cp.ket.sb[j];
+    const double A[3] = {cp.shell_xyz[3 * a], cp.shell_xyz[3 * a + 1], cp.shell_xyz[3 * a + 2]};
+    const double B[3] = {cp.shell_xyz[3 * b], cp.shell_xyz[3 * b + 1], cp.shell_xyz[3 * b + 2]};
+    const double C[3] = {cp.shell_xyz[3 * c], cp.shell_xyz[3 * c + 1], cp.shell_xyz[3 * c + 2]};
+    const double D[3] = {cp.shell_xyz[3 * d], cp.shell_xyz[3 * d + 1], cp.shell_xyz[3 * d + 2]};
+    const double fac = ((a == b) ? 0.5 : 1.0) * ((c == d) ? 0.5 : 1.0) *
+                       ((cp.same && i == j) ? 0.5 : 1.0);
+    const int fa0 = cp.shell_foff[a], fb0 = cp.shell_foff[b];
+    const int fc0 = cp.shell_foff[c], fd0 = cp.shell_foff[d];
+    const double* P = cp.P;
+    const long n = cp.nbf;
+    double g[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) g[k] = 0.0;
+    static_switch<0, NS>(s, [&](auto sc) {
+        constexpr int FA = decltype(sc)::value / S::NB, FB = decltype(sc)::value % S::NB;
+        const double pab = P[(fa0 + FA) * n + fb0 + FB];
+        double Df[NK];
+#pragma unroll
+        for (int fc = 0; fc < S::NC; ++fc)
+#pragma unroll
+            for (int fd = 0; fd < S::ND; ++fd)
+                Df[fc * S::ND + fd] =
+                    fac * (4.0 * pab * P[(fc0 + fc) * n + fd0 + fd] -
+                           P[(fa0 + FA) * n + fc0 + fc] * P[(fb0 + FB) * n + fd0 + fd] -
+                           P[(fa0 + FA) * n + fd0 + fd] * P[(fb0 + FB) * n + fc0 + fc]);
+        grad_bra_component<LA, LB, LC, LD, FA, FB, double>(cp.prim + cp.bra.poff[i], cp.bra.np[i],
+                                                           cp.prim + cp.ket.poff[j], cp.ket.np[j],
+                                                           A, B, C, D, Df, g);
+    });
+#pragma unroll
+    for (int k = 0; k < 9; ++k) gout[(long long)(s * 9 + k) * cp.qstride + qa] = g[k];
+}
+
+template <int LA, int LB, int LC, int LD>
+static int launch_grad_t(const ClassParams& cp, double* gout, hipStream_t s)
+{
+    using S = QuartetShape<LA, LB, LC, LD>;
+    if (cp.nq <= 0) return kOk;
+    const long long waves = (cp.nq + 63) / 64;
+    const long long nb = (waves * S::NA * S::NB + 3) / 4;
+    if (nb > 0x7fffffffLL) {
+        set_error(kEUnsupported, "quartet segment too large (%lld)", cp.nq);
+        return kEUnsupported;
+    }
+    hipLaunchKernelGGL((k_grad_split<LA, LB, LC, LD>), dim3((unsigned)nb), dim3(256), 0, s, cp, gout);
+    SF_HIP(hipGetLastError());
+    return kOk;
+}
+
 template <int LA, int LB, int LC, int LD>
 static int launch_split(int fp32, int mode, const ClassParams& cp, hipStream_t s)
 {
@@ -263,6 +332,10 @@ static int launch_schwarz_t(const int* sa, const int* sb, const int* poff, const
                             const PrimPair* prim, const double* xyz, double* Q, hipStream_t s)  \
     {                                                                                          \
         return launch_schwarz_t<a, b, c, d>(sa, sb, poff, np, n, prim, xyz, Q, s);             \
+    }                                                                                          \
+    int launch_grad_##id(const ClassParams& cp, double* gout, hipStream_t s)                   \
+    {                                                                                          \
+        return launch_grad_t<a, b, c, d>(cp, gout, s);                                         \
     }
 #define SF_PICK(id, a, b, c, d) SF_PICK_##id(id, a, b, c, d)
 #if SF_CLASS == 0
@@ -394,6 +467,29 @@ int class_ntile(int qcls)
     switch (qcls) {
 #define X(id, a, b, c, d) \
     case id: return QuartetShape<a, b, c, d>::NTILE;
+        SF_FOR_EACH_CLASS(X)
+#undef X
+    }
+    return 0;
+}
+
+int launch_grad_class(int qcls, const ClassParams& cp, double* gout, hipStream_t s)
+{
+    switch (qcls) {
+#define X(id, a, b, c, d) \
+    case id: return launch_grad_##id(cp, gout, s);
+        SF_FOR_EACH_CLASS(X)
+#undef X
+    }
+    set_error(kEUnsupported, "quartet class %d", qcls);
+    return kEUnsupported;
+}
+
+int class_nslice(int qcls)
+{
+    switch (qcls) {
+#define X(id, a, b, c, d) \
+    case id: return QuartetShape<a, b, c, d>::NA * QuartetShape<a, b, c, d>::NB;
         SF_FOR_EACH_CLASS(X)
 #undef X
     }

@@ -69,4 +69,62 @@ int launch_one_electron(int pc, const int* sa, const int* sb, const int* poff, c
     return kOk;
 }
 
+// Gradient of the one-electron energy: one thread per shell pair writes its
+// row of 3*natom derivatives (one_electron_grad_pair); rows are summed by
+// k_grad_rows in a fixed order.
+template <int LA, int LB>
+__global__ __launch_bounds__(64) void k_grad_onee(const int* __restrict__ sa, const int* __restrict__ sb,
+                                                  const int* __restrict__ poff, const int* __restrict__ np,
+                                                  int n, const PrimPair* __restrict__ prim,
+                                                  const double* __restrict__ xyz,
+                                                  const int* __restrict__ foff,
+                                                  const int* __restrict__ shell_atom, int natom,
+                                                  const double* __restrict__ Zc,
+                                                  const double* __restrict__ Cxyz, int nbf,
+                                                  const double* __restrict__ Ps,
+                                                  const double* __restrict__ Ws,
+                                                  double* __restrict__ rows)
+{
+    constexpr int NA = cart_n(LA), NB = cart_n(LB);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int a = sa[i], b = sb[i];
+    const double A[3] = {xyz[3 * a], xyz[3 * a + 1], xyz[3 * a + 2]};
+    const double B[3] = {xyz[3 * b], xyz[3 * b + 1], xyz[3 * b + 2]};
+    double Pb[NA * NB], Wb[NA * NB];
+    const int r0 = foff[a], c0 = foff[b];
+#pragma unroll
+    for (int fa = 0; fa < NA; ++fa)
+#pragma unroll
+        for (int fb = 0; fb < NB; ++fb) {
+            Pb[fa * NB + fb] = Ps[(long long)(r0 + fa) * nbf + c0 + fb];
+            Wb[fa * NB + fb] = Ws[(long long)(r0 + fa) * nbf + c0 + fb];
+        }
+    one_electron_grad_pair<LA, LB>(prim + poff[i], np[i], A, B, shell_atom[a], shell_atom[b], natom,
+                                   Zc, Cxyz, Pb, Wb, (a == b) ? 1.0 : 2.0,
+                                   rows + (long long)i * 3 * natom);
+}
+
+int launch_grad_one_electron(int pc, const int* sa, const int* sb, const int* poff, const int* np,
+                             int n, const PrimPair* prim, const double* xyz, const int* foff,
+                             const int* shell_atom, int natom, const double* Z, const double* Cxyz,
+                             int nbf, const double* Ps, const double* Ws, double* rows,
+                             hipStream_t s)
+{
+    if (n <= 0) return kOk;
+    dim3 g((n + 63) / 64), b(64);
+#define O_(id, la, lb)                                                                            \
+    case id:                                                                                      \
+        hipLaunchKernelGGL((k_grad_onee<la, lb>), g, b, 0, s, sa, sb, poff, np, n, prim, xyz, foff, \
+                           shell_atom, natom, Z, Cxyz, nbf, Ps, Ws, rows);                        \
+        break;
+    switch (pc) {
+        O_(0, 0, 0) O_(1, 1, 0) O_(2, 1, 1) O_(3, 2, 0) O_(4, 2, 1) O_(5, 2, 2)
+    default: set_error(kEUnsupported, "pair class %d", pc); return kEUnsupported;
+    }
+#undef O_
+    SF_HIP(hipGetLastError());
+    return kOk;
+}
+
 }  // namespace sf

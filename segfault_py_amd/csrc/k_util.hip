@@ -279,6 +279,154 @@ int launch_reduce(int n1, int n2, const int* dest_keys, int ndest, const long lo
     return kEUnsupported;
 }
 
+// ------------------------------------------- gradient reductions ---------
+// Bra side: one wavefront per bra pair i of a class sums d/dA and d/dB over
+// its kets (the contiguous quartets [base[i], base[i+1])) and all bra
+// component slices, in a fixed order.
+__global__ __launch_bounds__(256) void k_grad_reduce_bra(const double* __restrict__ gout, int ns,
+                                                         long long qstride,
+                                                         const long long* __restrict__ base,
+                                                         int nbra, double* __restrict__ part)
+{
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= nbra) return;
+    const int lane = threadIdx.x & 63;
+    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (long long q = base[i] + lane; q < base[i + 1]; q += 64)
+        for (int s = 0; s < ns; ++s)
+#pragma unroll
+            for (int k = 0; k < 6; ++k) acc[k] += gout[(long long)(s * 9 + k) * qstride + q];
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) acc[k] += __shfl_xor(acc[k], o, 64);
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < 6; ++k) part[6 * (long long)i + k] = acc[k];
+}
+
+// Ket side: one wavefront per ket pair j.  The bras holding ket j are the
+// i in [i0, i1) with n_i = base[i+1] - base[i] > j (i0 = j when bra and ket
+// lists coincide, else 0); n_i > j holds on a prefix of that range because
+// the lists are sorted by Schwarz factor.  Sums d/dC and d/dD = -(A+B+C).
+__global__ __launch_bounds__(256) void k_grad_reduce_ket(const double* __restrict__ gout, int ns,
+                                                         long long qstride,
+                                                         const long long* __restrict__ base,
+                                                         int nbra, int nket, int same,
+                                                         double* __restrict__ part)
+{
+    const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (j >= nket) return;
+    const int lane = threadIdx.x & 63;
+    const int i0 = same ? j : 0;
+    int lo = i0, hi = nbra;  // first i >= i0 with n_i <= j
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (base[mid + 1] - base[mid] > j) lo = mid + 1;
+        else hi = mid;
+    }
+    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int i = i0 + lane; i < lo; i += 64) {
+        const long long q = base[i] + j;
+        for (int s = 0; s < ns; ++s) {
+            double g[9];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) g[k] = gout[(long long)(s * 9 + k) * qstride + q];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                acc[k] += g[6 + k];
+                acc[3 + k] -= g[k] + g[3 + k] + g[6 + k];
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) acc[k] += __shfl_xor(acc[k], o, 64);
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < 6; ++k) part[6 * (long long)j + k] = acc[k];
+}
+
+// Per atom: fixed-order sum of the 3-vectors listed for it (CSR).
+__global__ __launch_bounds__(64) void k_grad_atoms(const double* __restrict__ part,
+                                                   const long long* __restrict__ off,
+                                                   const long long* __restrict__ idx, int natom,
+                                                   double* __restrict__ grad)
+{
+    const int a = blockIdx.x;
+    if (a >= natom) return;
+    const int lane = threadIdx.x;
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (long long e = off[a] + lane; e < off[a + 1]; e += 64) {
+        const double* v = part + idx[e];
+        acc[0] += v[0];
+        acc[1] += v[1];
+        acc[2] += v[2];
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) acc[k] += __shfl_xor(acc[k], o, 64);
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) grad[3 * a + k] = acc[k];
+}
+
+int launch_grad_reduce_bra(const double* gout, int nslice, long long qstride, const long long* base,
+                           int nbra, double* part, hipStream_t s)
+{
+    if (nbra <= 0) return kOk;
+    hipLaunchKernelGGL(k_grad_reduce_bra, dim3((nbra + 3) / 4), dim3(256), 0, s, gout, nslice,
+                       qstride, base, nbra, part);
+    SF_HIP(hipGetLastError());
+    return kOk;
+}
+
+int launch_grad_reduce_ket(const double* gout, int nslice, long long qstride, const long long* base,
+                           int nbra, int nket, int same, double* part, hipStream_t s)
+{
+    if (nket <= 0) return kOk;
+    hipLaunchKernelGGL(k_grad_reduce_ket, dim3((nket + 3) / 4), dim3(256), 0, s, gout, nslice,
+                       qstride, base, nbra, nket, same, part);
+    SF_HIP(hipGetLastError());
+    return kOk;
+}
+
+int launch_grad_atoms(const double* part, const long long* csr_off, const long long* csr_idx,
+                      int natom, double* grad, hipStream_t s)
+{
+    if (natom <= 0) return kOk;
+    hipLaunchKernelGGL(k_grad_atoms, dim3(natom), dim3(64), 0, s, part, csr_off, csr_idx, natom,
+                       grad);
+    SF_HIP(hipGetLastError());
+    return kOk;
+}
+
+// Column sums of the per-shell-pair one-electron gradient rows, plus `add`.
+__global__ __launch_bounds__(64) void k_grad_rows(const double* __restrict__ rows, long long nrows,
+                                                  int ncol, const double* __restrict__ add,
+                                                  double* __restrict__ out)
+{
+    const int k = blockIdx.x;
+    if (k >= ncol) return;
+    const int lane = threadIdx.x;
+    double acc = 0.0;
+    for (long long r = lane; r < nrows; r += 64) acc += rows[r * ncol + k];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if (lane == 0) out[k] = acc + (add ? add[k] : 0.0);
+}
+
+int launch_grad_rows(const double* rows, long long nrows, int ncol, const double* add, double* out,
+                     hipStream_t s)
+{
+    if (ncol <= 0) return kOk;
+    hipLaunchKernelGGL(k_grad_rows, dim3(ncol), dim3(64), 0, s, rows, nrows, ncol, add, out);
+    SF_HIP(hipGetLastError());
+    return kOk;
+}
+
 // --------------------------------------------------- small dense kernels --
 __global__ void k_scale_density(const double* __restrict__ P, const double* __restrict__ f, int n,
                                 double* __restrict__ Ps)

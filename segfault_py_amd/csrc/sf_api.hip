@@ -129,7 +129,11 @@ struct sf_basis {
     DevBuf d_prim, d_xyz, d_foff, d_fnorm, d_zero;
     // nuclei (point charges) for the one-electron potential
     int nnuc = 0;
-    DevBuf d_Z, d_nuc;
+    DevBuf d_Z, d_nuc, d_atom;
+    // nuclear gradient scratch (built on first use after each sf_screen)
+    bool grad_ready = false;
+    std::vector<long long> gpart_bra, gpart_ket;  // per class offsets into d_gpart
+    DevBuf d_gpart, d_gout, d_gcsr_off, d_gcsr_idx, d_grows, d_Ws, d_g2, d_gtot;
     PairClassData pc[kNumPairClass];
     std::vector<int> pair_slot;  // for pair (a>=b) index: (pc << 24) | position in pc arrays
     bool have_schwarz = false, screened = false;
@@ -360,6 +364,7 @@ extern "C" int sf_basis_create(int nshell, const int* L, const int* nprim, const
         return kEUnsupported;
     }
     int rc = b->d_prim.upload(b->prim);
+    if (!rc) rc = b->d_atom.upload(b->atom);
     if (!rc) rc = b->d_xyz.upload(b->xyz);
     if (!rc) rc = b->d_foff.upload(b->foff);
     if (!rc) rc = b->d_fnorm.upload(b->fnorm);
@@ -446,6 +451,7 @@ extern "C" int sf_screen(sf_basis* b, double tau, double tau_mixed)
     }
     SF_TRY(compute_schwarz(b));
     b->screened = false;
+    b->grad_ready = false;
     b->tau = tau;
     b->tau_mixed = tau_mixed;
     double qmax = 0.0;
@@ -895,5 +901,108 @@ extern "C" int sf_one_electron(sf_basis* b, double* S, double* T, double* V, voi
                                    b->d_fnorm.as<double>(), b->nnuc, b->d_Z.as<double>(),
                                    b->d_nuc.as<double>(), b->nbf, S, T, V, (hipStream_t)stream));
     }
+    return kOk;
+}
+
+// Layout of the gradient partials and the per-atom lists that sum them.
+static int prepare_grad(sf_basis* b)
+{
+    if (b->grad_ready) return kOk;
+    const int natom = b->natom;
+    b->gpart_bra.assign(kNumQuartetClass, -1);
+    b->gpart_ket.assign(kNumQuartetClass, -1);
+    long long off = 0;
+    size_t gout = 1;
+    std::vector<std::vector<long long>> lists(natom);
+    for (int c = 0; c < kNumQuartetClass; ++c) {
+        const QuartetClassData& Q = b->qc[c];
+        if (Q.nq_all == 0) continue;
+        const PairClassData& B = b->pc[Q.pcb];
+        const PairClassData& K = b->pc[Q.pck];
+        b->gpart_bra[c] = off;
+        for (int i = 0; i < Q.nbra; ++i) {
+            lists[b->atom[B.s_sa[i]]].push_back(off + 6LL * i);
+            lists[b->atom[B.s_sb[i]]].push_back(off + 6LL * i + 3);
+        }
+        off += 6LL * Q.nbra;
+        b->gpart_ket[c] = off;
+        const int nket = (int)K.s_sa.size();
+        for (int j = 0; j < nket; ++j) {
+            lists[b->atom[K.s_sa[j]]].push_back(off + 6LL * j);
+            lists[b->atom[K.s_sb[j]]].push_back(off + 6LL * j + 3);
+        }
+        off += 6LL * nket;
+        gout = std::max(gout, (size_t)class_nslice(c) * 9 * (size_t)Q.nq_all);
+    }
+    std::vector<long long> csr_off(natom + 1, 0), csr_idx;
+    for (int a = 0; a < natom; ++a) {
+        csr_off[a + 1] = csr_off[a] + (long long)lists[a].size();
+        csr_idx.insert(csr_idx.end(), lists[a].begin(), lists[a].end());
+    }
+    SF_TRY(b->d_gpart.ensure(std::max<long long>(1, off) * sizeof(double)));
+    SF_TRY(b->d_gout.ensure(gout * sizeof(double)));
+    SF_TRY(b->d_gcsr_off.upload(csr_off));
+    SF_TRY(b->d_gcsr_idx.upload(csr_idx.empty() ? std::vector<long long>(1, 0) : csr_idx));
+    long long npair = 0;
+    for (auto& P : b->pc) npair += (long long)P.sa.size();
+    SF_TRY(b->d_grows.ensure(std::max<long long>(1, npair) * 3 * std::max(1, natom) * sizeof(double)));
+    const size_t nn = (size_t)b->nbf * b->nbf;
+    SF_TRY(b->d_Ws.ensure(nn * sizeof(double)));
+    SF_TRY(b->d_g2.ensure(3 * std::max(1, natom) * sizeof(double)));
+    SF_TRY(b->d_gtot.ensure(3 * std::max(1, natom) * sizeof(double)));
+    b->grad_ready = true;
+    return kOk;
+}
+
+extern "C" int sf_grad(sf_basis* b, const double* P, const double* W, double* grad, void* stream)
+{
+    if (!b || !P || !W || !grad) {
+        set_error(kEInval, "sf_grad: null argument");
+        return kEInval;
+    }
+    if (!b->screened) {
+        set_error(kEState, "sf_grad: call sf_screen first");
+        return kEState;
+    }
+    if (b->nnuc != b->natom || b->natom <= 0) {
+        set_error(kEState, "sf_grad: nuclei (sf_basis_set_nuclei) must match the %d atoms", b->natom);
+        return kEState;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    SF_TRY(prepare_grad(b));
+    const int nbf = b->nbf, natom = b->natom;
+    SF_TRY(launch_scale_density(P, b->d_fnorm.as<double>(), nbf, b->d_Ps.as<double>(), s));
+    SF_TRY(launch_scale_density(W, b->d_fnorm.as<double>(), nbf, b->d_Ws.as<double>(), s));
+    for (int c = 0; c < kNumQuartetClass; ++c) {
+        QuartetClassData& Q = b->qc[c];
+        if (Q.nq_all == 0) continue;
+        ClassParams cp = class_params(b, c, b->d_Ps.as<double>());
+        SF_TRY(launch_grad_class(c, cp, b->d_gout.as<double>(), s));
+        SF_TRY(launch_grad_reduce_bra(b->d_gout.as<double>(), class_nslice(c), Q.nq_all,
+                                      Q.d_base_all.as<long long>(), Q.nbra,
+                                      b->d_gpart.as<double>() + b->gpart_bra[c], s));
+        SF_TRY(launch_grad_reduce_ket(b->d_gout.as<double>(), class_nslice(c), Q.nq_all,
+                                      Q.d_base_all.as<long long>(), Q.nbra,
+                                      (int)b->pc[Q.pck].s_sa.size(), Q.same,
+                                      b->d_gpart.as<double>() + b->gpart_ket[c], s));
+    }
+    SF_TRY(launch_grad_atoms(b->d_gpart.as<double>(), b->d_gcsr_off.as<long long>(),
+                             b->d_gcsr_idx.as<long long>(), natom, b->d_g2.as<double>(), s));
+    long long row = 0;
+    for (int p = 0; p < kNumPairClass; ++p) {
+        PairClassData& Pc = b->pc[p];
+        SF_TRY(launch_grad_one_electron(p, Pc.d_sa.as<int>(), Pc.d_sb.as<int>(), Pc.d_poff.as<int>(),
+                                        Pc.d_np.as<int>(), (int)Pc.sa.size(),
+                                        b->d_prim.as<PrimPair>(), b->d_xyz.as<double>(),
+                                        b->d_foff.as<int>(), b->d_atom.as<int>(), natom,
+                                        b->d_Z.as<double>(), b->d_nuc.as<double>(), nbf,
+                                        b->d_Ps.as<double>(), b->d_Ws.as<double>(),
+                                        b->d_grows.as<double>() + row * 3 * natom, s));
+        row += (long long)Pc.sa.size();
+    }
+    SF_TRY(launch_grad_rows(b->d_grows.as<double>(), row, 3 * natom, b->d_g2.as<double>(),
+                            b->d_gtot.as<double>(), s));
+    SF_HIP(hipMemcpyAsync(grad, b->d_gtot.p, 3 * natom * sizeof(double), hipMemcpyDeviceToHost, s));
+    SF_HIP(hipStreamSynchronize(s));
     return kOk;
 }

@@ -84,6 +84,22 @@ int launch_schwarz(int pc, const int* sa, const int* sb, const int* poff, const 
 int class_ntile(int qcls);
 int class_nint(int qcls);
 int class_split(int qcls);
+int class_nslice(int qcls);
+int launch_grad_class(int qcls, const ClassParams& cp, double* gout, hipStream_t s);
+int launch_grad_reduce_bra(const double* gout, int nslice, long long qstride,
+                           const long long* base, int nbra, double* part, hipStream_t s);
+int launch_grad_reduce_ket(const double* gout, int nslice, long long qstride,
+                           const long long* base, int nbra, int nket, int same, double* part,
+                           hipStream_t s);
+int launch_grad_atoms(const double* part, const long long* csr_off, const long long* csr_idx,
+                      int natom, double* grad, hipStream_t s);
+int launch_grad_one_electron(int pc, const int* sa, const int* sb, const int* poff, const int* np,
+                             int n, const PrimPair* prim, const double* xyz, const int* foff,
+                             const int* shell_atom, int natom, const double* Z, const double* Cxyz,
+                             int nbf, const double* Ps, const double* Ws, double* rows,
+                             hipStream_t s);
+int launch_grad_rows(const double* rows, long long nrows, int ncol, const double* add, double* out,
+                     hipStream_t s);
 void class_tile_offsets(int qcls, int off[6]);
 
 // Launchers (k_util.hip).

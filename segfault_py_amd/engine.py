@@ -78,6 +78,7 @@ def lib():
     L.sf_jk.argtypes = [vp, vp, vp, vp, ctypes.c_int, vp]
     L.sf_basis_set_nuclei.argtypes = [vp, ctypes.c_int, _f64p, _f64p]
     L.sf_one_electron.argtypes = [vp, vp, vp, vp, vp]
+    L.sf_grad.argtypes = [vp, vp, vp, _f64p, vp]
     L.sf_jk_profile.argtypes = [vp, vp, vp, vp, ctypes.c_int, vp, _f64p]
     _LIB = L
     return L
@@ -93,7 +94,8 @@ def exported_symbols() -> list[str]:
     return ["sf_last_error", "sf_version", "sf_boys", "sf_rys_roots", "sf_eri_rys",
             "sf_eri_components", "sf_basis_create", "sf_basis_destroy", "sf_basis_nbf",
             "sf_schwarz", "sf_screen", "sf_quartet_counts", "sf_pair_list", "sf_quartet_list",
-            "sf_eri_dense", "sf_jk", "sf_jk_profile", "sf_basis_set_nuclei", "sf_one_electron"]
+            "sf_eri_dense", "sf_jk", "sf_jk_profile", "sf_basis_set_nuclei", "sf_one_electron",
+            "sf_grad"]
 
 
 def _torch():
@@ -218,6 +220,17 @@ class Basis:
                                     ctypes.c_void_p(T.data_ptr()), ctypes.c_void_p(V.data_ptr()),
                                     _stream_ptr(torch)))
         return S, T, V
+
+    def gradient(self, P, W) -> np.ndarray:
+        """Electronic nuclear gradient (natom x 3, Eh/bohr) for density P and
+        energy-weighted density W (float64 CUDA tensors)."""
+        torch = _torch()
+        P = P.contiguous()
+        W = W.contiguous()
+        g = np.zeros(3 * self.mol.natom)
+        check(lib().sf_grad(self._h, ctypes.c_void_p(P.data_ptr()), ctypes.c_void_p(W.data_ptr()), g,
+                            _stream_ptr(torch)))
+        return g.reshape(-1, 3)
 
     def schwarz(self) -> np.ndarray:
         """Q[a*(a+1)/2 + b] for shell pairs a >= b."""

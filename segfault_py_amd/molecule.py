@@ -200,6 +200,7 @@ class Molecule:
             with open(bas_filename) as fh:
                 basis_text = fh.read()
         self.bas = bas_filename
+        self.basis_text = basis_text
         self.symbols = list(symbols)
         self.numbers = np.array([ATOMIC_NUMBER[s] for s in self.symbols], dtype=np.int64)
         self.xyz = np.asarray(xyz_angstrom, dtype=np.float64)

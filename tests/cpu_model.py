@@ -35,6 +35,11 @@ def harness():
         H.h_rys.argtypes = [ctypes.c_int, ctypes.c_double, _f64p, _f64p]
         H.h_boys.argtypes = [ctypes.c_int, ctypes.c_double, _f64p]
         H.h_boys01.argtypes = [ctypes.c_double, _f64p]
+        H.h_grad.argtypes = [ctypes.c_int] * 4 + [_f64p, ctypes.c_int, _f64p, ctypes.c_int, _f64p,
+                                                  _f64p, _f64p, _f64p, _f64p, _f64p]
+        H.h_grad_onee.argtypes = [ctypes.c_int, ctypes.c_int, _f64p, ctypes.c_int, _f64p, _f64p,
+                                  ctypes.c_int, ctypes.c_int, ctypes.c_int, _f64p, _f64p, _f64p,
+                                  _f64p, ctypes.c_double, _f64p]
         H.h_one_electron.argtypes = [ctypes.c_int, ctypes.c_int, _f64p, ctypes.c_int, _f64p, _f64p,
                                      ctypes.c_int, _f64p, _f64p, _f64p, _f64p, _f64p]
         _H = H
@@ -192,3 +197,76 @@ def one_electron_model(mol):
                 M[r:r + nA, c:c + nB] = blk
                 M[c:c + nB, r:r + nA] = blk.T
     return S, T, V
+
+
+def quartet_grad(shells, a, b, c, d, Dfac):
+    """sum_{ijkl} Dfac * d(ij|kl)/d(A,B,C) from the kernels' derivative code."""
+    S = [shells[x] for x in (a, b, c, d)]
+    g = np.zeros(9)
+    assert harness().h_grad(S[0].L, S[1].L, S[2].L, S[3].L, prim_pairs(S[0], S[1]),
+                            S[0].nprim * S[1].nprim, prim_pairs(S[2], S[3]), S[2].nprim * S[3].nprim,
+                            S[0].center, S[1].center, S[2].center, S[3].center,
+                            np.ascontiguousarray(Dfac, dtype=np.float64), g) == 0
+    return g
+
+
+def one_electron_grad_model(mol, P, W):
+    """d/dR [sum P (T + V) - sum W S] per atom (natom x 3), from the kernels'
+    per-pair routine on the CPU."""
+    shells = mol.shells()
+    fn = mol.function_norms()
+    Ps = P * fn[:, None] * fn[None, :]
+    Ws = W * fn[:, None] * fn[None, :]
+    Z = np.ascontiguousarray(mol.numbers, dtype=np.float64)
+    C = np.ascontiguousarray(mol.xyz__, dtype=np.float64).reshape(-1)
+    g = np.zeros(3 * mol.natom)
+    for a in range(len(shells)):
+        for b in range(a + 1):
+            sa, sb = (a, b) if shells[a].L >= shells[b].L else (b, a)
+            A, B = shells[sa], shells[sb]
+            r, c = A.offset, B.offset
+            Pb = np.ascontiguousarray(Ps[r:r + A.nfunc, c:c + B.nfunc])
+            Wb = np.ascontiguousarray(Ws[r:r + A.nfunc, c:c + B.nfunc])
+            row = np.zeros(3 * mol.natom)
+            assert harness().h_grad_onee(A.L, B.L, prim_pairs(A, B), A.nprim * B.nprim, A.center,
+                                         B.center, A.atom, B.atom, mol.natom, Z, C, Pb, Wb,
+                                         1.0 if sa == sb else 2.0, row) == 0
+            g += row
+    return g.reshape(-1, 3)
+
+
+def two_electron_grad_model(mol, P):
+    """d/dR of E2 = 1/2 sum P_pq P_rs [(pq|rs) - 1/2 (pr|qs)] per atom, over
+    the unique shell quartets (unscreened) with the kernels' derivative code."""
+    shells = mol.shells()
+    fn = mol.function_norms()
+    Ps = P * fn[:, None] * fn[None, :]
+    groups = canonical_pairs(shells)
+    pairs = [p for pc in range(6) for p in groups[pc]]
+    g = np.zeros((mol.natom, 3))
+    for ib, (a, b) in enumerate(pairs):
+        for (c, d) in pairs[:ib + 1]:
+            la, lb, lc, ld = (shells[x].L for x in (a, b, c, d))
+            # canonical class order: bra pair class >= ket pair class
+            if (la * (la + 1) // 2 + lb) < (lc * (lc + 1) // 2 + ld):
+                A_, B_, C_, D_ = c, d, a, b
+            else:
+                A_, B_, C_, D_ = a, b, c, d
+            sh = [shells[x] for x in (A_, B_, C_, D_)]
+            fac = (0.5 if A_ == B_ else 1.0) * (0.5 if C_ == D_ else 1.0) * (0.5 if {A_, B_} == {C_, D_} and (A_, B_) == (C_, D_) else 1.0)
+            o = [x.offset for x in sh]
+            n = [x.nfunc for x in sh]
+            Pab = Ps[o[0]:o[0] + n[0], o[1]:o[1] + n[1]]
+            Pcd = Ps[o[2]:o[2] + n[2], o[3]:o[3] + n[3]]
+            Pac = Ps[o[0]:o[0] + n[0], o[2]:o[2] + n[2]]
+            Pbd = Ps[o[1]:o[1] + n[1], o[3]:o[3] + n[3]]
+            Pad = Ps[o[0]:o[0] + n[0], o[3]:o[3] + n[3]]
+            Pbc = Ps[o[1]:o[1] + n[1], o[2]:o[2] + n[2]]
+            Dfac = fac * (4 * np.einsum('ab,cd->abcd', Pab, Pcd) - np.einsum('ac,bd->abcd', Pac, Pbd)
+                          - np.einsum('ad,bc->abcd', Pad, Pbc))
+            gq = quartet_grad(shells, A_, B_, C_, D_, Dfac.reshape(-1))
+            gD = -(gq[0:3] + gq[3:6] + gq[6:9])
+            for k, x in enumerate((A_, B_, C_)):
+                g[shells[x].atom] += gq[3 * k:3 * k + 3]
+            g[shells[D_].atom] += gD
+    return g

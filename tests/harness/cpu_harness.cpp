@@ -153,3 +153,50 @@ extern "C" int h_one_electron(int la, int lb, const double* prim, int np, const 
 #undef O_
     return 1;
 }
+
+template <int LA, int LB, int LC, int LD>
+static void run_grad(const double* bra, int nb, const double* ket, int nk, const double* A,
+                     const double* B, const double* C, const double* D, const double* Dfac,
+                     double* g)
+{
+    using S = QuartetShape<LA, LB, LC, LD>;
+    for (int k = 0; k < 9; ++k) g[k] = 0.0;
+    for (int s = 0; s < S::NA * S::NB; ++s) {
+        static_switch<0, S::NA * S::NB>(s, [&](auto sc) {
+            constexpr int FA = decltype(sc)::value / S::NB, FB = decltype(sc)::value % S::NB;
+            grad_bra_component<LA, LB, LC, LD, FA, FB, double>(
+                (const PrimPair*)bra, nb, (const PrimPair*)ket, nk, A, B, C, D,
+                Dfac + s * S::NC * S::ND, g);
+        });
+    }
+}
+
+extern "C" int h_grad(int la, int lb, int lc, int ld, const double* bra, int nb, const double* ket,
+                      int nk, const double* A, const double* B, const double* C, const double* D,
+                      const double* Dfac, double* g)
+{
+#define X(a, b, c, d)                                                          \
+    if (la == a && lb == b && lc == c && ld == d) {                            \
+        run_grad<a, b, c, d>(bra, nb, ket, nk, A, B, C, D, Dfac, g);           \
+        return 0;                                                              \
+    }
+    SF_CLASSES(X)
+#undef X
+    return 1;
+}
+
+extern "C" int h_grad_onee(int la, int lb, const double* prim, int np, const double* A,
+                           const double* B, int atomA, int atomB, int natom, const double* Z,
+                           const double* C, const double* Pb, const double* Wb, double mult,
+                           double* row)
+{
+    const PrimPair* pp = (const PrimPair*)prim;
+#define O_(x, y)                                                                              \
+    if (la == x && lb == y) {                                                                 \
+        one_electron_grad_pair<x, y>(pp, np, A, B, atomA, atomB, natom, Z, C, Pb, Wb, mult, row); \
+        return 0;                                                                             \
+    }
+    O_(0, 0) O_(1, 0) O_(1, 1) O_(2, 0) O_(2, 1) O_(2, 2)
+#undef O_
+    return 1;
+}

@@ -185,3 +185,87 @@ def test_one_electron_model_matches_oracle(oracle, request, fixture):
     assert np.max(np.abs(S - Sr)) < 1e-13
     assert np.max(np.abs(T - Tr)) < 1e-12
     assert np.max(np.abs(V - Vr)) < 1e-12
+
+
+def _shifted(shell, axis, h):
+    import copy
+    s2 = copy.copy(shell)
+    s2.center = shell.center.copy()
+    s2.center[axis] += h
+    return s2
+
+
+@pytest.mark.parametrize("ls", [(0, 0, 0, 0), (1, 0, 0, 0), (1, 0, 1, 0), (1, 1, 1, 1),
+                                (2, 0, 1, 0), (2, 1, 1, 0), (2, 2, 1, 1), (2, 1, 2, 1)])
+def test_derivative_integrals_match_finite_differences(water_ccpvdz, ls):
+    """grad_bra_component (the gradient kernel's math) vs central differences
+    of the contracted integrals w.r.t. each centre (h = 1e-5 bohr)."""
+    shells = water_ccpvdz.shells()
+    byL = {}
+    for k, sh in enumerate(shells):
+        byL.setdefault(sh.L, []).append(k)
+    # distinct shells on different atoms where possible
+    pick = [byL[ls[0]][-1], byL[ls[1]][0], byL[ls[2]][len(byL[ls[2]]) // 2], byL[ls[3]][-1]]
+    rng = np.random.default_rng(sum(ls))
+    nint = int(np.prod([CM.ncart(l) for l in ls]))
+    Dfac = rng.standard_normal(nint)
+    g = CM.quartet_grad(shells, *pick, Dfac)
+    h = 1e-5
+    base = [shells[x] for x in pick]
+    fd = np.zeros(12)
+    for ctr in range(4):
+        for ax in range(3):
+            vals = []
+            for sgn in (1, -1):
+                sh = list(base)
+                sh[ctr] = _shifted(base[ctr], ax, sgn * h)
+                v = CM.shell_quartet(sh, 0, 1, 2, 3)
+                vals.append(float(np.dot(Dfac, v)))
+            fd[3 * ctr + ax] = (vals[0] - vals[1]) / (2 * h)
+    got = np.concatenate([g, -(g[0:3] + g[3:6] + g[6:9])])
+    scale = max(1.0, np.max(np.abs(fd)))
+    assert np.max(np.abs(got - fd)) < 2e-7 * scale, (got, fd)
+
+
+def _displaced(mol, atom, axis, h_bohr):
+    from segfault_py_amd.molecule import Molecule, BOHR_ANGSTROM
+    xyz = mol.xyz.copy()
+    xyz[atom, axis] += h_bohr * BOHR_ANGSTROM
+    return Molecule.from_geometry(mol.symbols, xyz, _basis_text(mol))
+
+
+def _basis_text(mol):
+    from segfault_py_amd import workloads as W
+    return W.even_tempered_text(W.BASIS_SP)
+
+
+def test_one_and_two_electron_gradient_models_match_fd(oracle, water_sp):
+    """Per-pair one-electron derivative routine and per-quartet derivative
+    ERIs, assembled on the CPU, vs central differences (h = 1e-4 bohr) of
+    E1 = sum P(T+V) - sum W S and E2 = 1/2 sum P P [(pq|rs) - (pr|qs)/2]
+    from the oracle at displaced geometries (P, W held fixed)."""
+    mol = water_sp
+    rng = np.random.default_rng(21)
+    C = rng.standard_normal((mol.nbf, 5))
+    P = C @ C.T / 5
+    Wm = rng.standard_normal((mol.nbf, mol.nbf))
+    Wm = (Wm + Wm.T) / 10
+    g1 = CM.one_electron_grad_model(mol, P, Wm)
+    g2 = CM.two_electron_grad_model(mol, P)
+    h = 1e-4
+    fd1 = np.zeros_like(g1)
+    fd2 = np.zeros_like(g2)
+    for at in range(mol.natom):
+        for ax in range(3):
+            e1, e2 = [], []
+            for sgn in (1, -1):
+                m = _displaced(mol, at, ax, sgn * h)
+                S, T, V = oracle.one_electron(m, 8)
+                e1.append(np.sum(P * (T + V)) - np.sum(Wm * S))
+                eri = oracle.eri_tensor(m, 8)
+                J, K = oracle.jk_dense(eri, P)
+                e2.append(0.5 * np.sum(P * (J - 0.5 * K)))
+            fd1[at, ax] = (e1[0] - e1[1]) / (2 * h)
+            fd2[at, ax] = (e2[0] - e2[1]) / (2 * h)
+    assert np.max(np.abs(g1 - fd1)) < 1e-6, (g1, fd1)
+    assert np.max(np.abs(g2 - fd2)) < 1e-6, (g2, fd2)
