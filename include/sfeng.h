@@ -69,12 +69,40 @@ int sf_basis_create(int nshell, const int *L, const int *nprim, const double *ce
 int sf_basis_destroy(sf_basis *b);
 int sf_basis_nbf(const sf_basis *b, int *nbf);
 
+/* ---- Batches of independent molecules -----------------------------------
+ * One handle for nmol molecules solved together (no reference counterpart:
+ * the reference runs one molecule per scf_diis call, src/hf/hf.py:236-353).
+ * Molecule m owns shells [mol_shell_begin[m], mol_shell_begin[m+1]) and
+ * nuclei [mol_atom_begin[m], mol_atom_begin[m+1]) of the sf_basis_set_nuclei
+ * list; shell arrays as in sf_basis_create.  No shell pair or quartet
+ * crosses molecules; screening, pair significance and primitive-pair
+ * screening are per molecule, so each molecule's integrals, J and K are
+ * those of a single-molecule handle.  Every N x N matrix argument (P, J, K,
+ * S, T, V, W) then holds one block per molecule: molecule m's n_m x n_m
+ * block starts at element block_off[m] with leading dimension block_ld[m]
+ * (default: blocks packed back to back, ld = n_m; set e.g. a padded
+ * (nmol, nmax, nmax) layout with sf_basis_set_layout).  Elements outside
+ * the blocks are never read or written.  For nmol = 1 this is exactly the
+ * sf_basis_create handle.  sf_schwarz, sf_eri_dense and sf_grad need
+ * nmol = 1. */
+int sf_basis_create_batch(int nmol, const int *mol_shell_begin, int nshell, const int *L,
+                          const int *nprim, const double *center, const double *expns,
+                          const double *coefs, int natom, const int *mol_atom_begin,
+                          const int *shell_atom, sf_basis **out);
+/* Blocks must not overlap: [block_off[m], block_off[m] + n_m * block_ld[m]). */
+int sf_basis_set_layout(sf_basis *b, const long long *block_off, const int *block_ld);
+/* Current layout; *extent = elements a matrix buffer must span.  Output
+ * arrays (nmol entries) may be NULL. */
+int sf_basis_layout(const sf_basis *b, int *nmol, int *mol_nbf, long long *block_off,
+                    int *block_ld, long long *extent);
+
 /* Point charges for the nuclear-attraction integrals: Z[natom], xyz[3*natom]
  * (bohr).  Replaces the atom loop of hf.py:266-274 (mol.atoms[i].xyz, number). */
 int sf_basis_set_nuclei(sf_basis *b, int natom, const double *Z, const double *xyz);
-/* One-electron integrals S, T, V (device buffers, N x N row-major; V with the
- * -Z_C factor), drop-in for mmd2.overlap / kinetic / potential summed as in
- * hf.py:266-274 (mmd2.py:14-84). */
+/* One-electron integrals S, T, V (device buffers in the handle's layout:
+ * N x N row-major for one molecule; V with the -Z_C factor of the
+ * molecule's own nuclei), drop-in for mmd2.overlap / kinetic / potential
+ * summed as in hf.py:266-274 (mmd2.py:14-84). */
 int sf_one_electron(sf_basis *b, double *S, double *T, double *V, void *stream);
 
 /* Schwarz factors Q_ab = sqrt(max |(ij|ij)|), i in a, j in b, for every
@@ -103,7 +131,8 @@ int sf_quartet_list(const sf_basis *b, int c, int *nbra, int *n_all, int *n_fp64
  * (hf.py:33-61) after sf_screen(b, 0, 0). */
 int sf_eri_dense(sf_basis *b, double *eri);
 
-/* Direct-SCF J and K (device buffers, N x N row-major) for a symmetric
+/* Direct-SCF J and K (device buffers in the handle's layout: N x N
+ * row-major for one molecule, per-molecule blocks for a batch) for a symmetric
  * density P: J_pq = sum_rs (pq|rs) P_rs, K_pq = sum_rs (pr|qs) P_rs,
  * drop-in for the two einsums of hf.py:300 (G = J - K/2).
  * precision 0 = all FP64, 1 = mixed (FP32 below tau_mixed, FP64
@@ -116,6 +145,26 @@ int sf_jk(sf_basis *b, const double *P, double *J, double *K, int precision, voi
  * scaling, ms[43] = deterministic plan reduction and finalisation. */
 int sf_jk_profile(sf_basis *b, const double *P, double *J, double *K, int precision,
                   void *stream, double *ms);
+/* sf_jk through a captured hipGraph: the first call (and the first after
+ * the arguments, sf_screen, sf_autotune, sf_set_tuning or
+ * sf_basis_set_layout change) captures the whole build -- density scaling,
+ * every quartet-class kernel, the plan reduction and finalisation, kernel
+ * nodes only -- on a private stream; every call replays it on `stream`.
+ * Same results as sf_jk, bit for bit.  For SCF loops that rebuild J/K for
+ * fixed buffers (the per-iteration line hf.py:300). */
+int sf_jk_graph(sf_basis *b, const double *P, double *J, double *K, int precision,
+                void *stream);
+/* Launch-shape autotuner (no reference counterpart).  For every quartet
+ * class and precision segment (FP64 / FP32 part), times 64/128/256-thread
+ * blocks with one thread per quartet or persistent grids of 2 or 8 blocks
+ * per compute unit on the current screened lists (density P, device) and
+ * keeps the fastest; sf_jk and sf_jk_graph use the choice.
+ * shapes[4c + 2seg + {0,1}] receives (block, grid; 0 = one per quartet),
+ * ms[2c + seg] the chosen shape's time; either may be NULL. */
+int sf_autotune(sf_basis *b, const double *P, int precision, void *stream, int *shapes,
+                double *ms);
+/* Set the shapes directly, in the layout sf_autotune reports. */
+int sf_set_tuning(sf_basis *b, const int *shapes);
 
 /* Nuclear gradient of the RHF electronic energy (host output grad[3*natom],
  * hartree/bohr, atom order of the basis): sum P dH/dR - sum W dS/dR +

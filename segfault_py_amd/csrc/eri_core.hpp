@@ -60,6 +60,16 @@ struct PrimPair {
     double eb;   // exponent b_j of the second shell
 };
 
+// Where the four shells (a, b, c, d) of a quartet sit in an N x N matrix of
+// their molecule: component x of shell k and component y of shell k' are at
+// row[k] + x * ld + col[k'] + y.  For one molecule of nbf functions,
+// row = foff * nbf, col = foff, ld = nbf.
+struct QuartetMap {
+    long long row[4];
+    int col[4];
+    int ld;
+};
+
 template <int LA, int LB, int LC, int LD>
 struct QuartetShape {
     static constexpr int L = LA + LB + LC + LD;
@@ -539,8 +549,8 @@ SF_HD SF_INLINE void grad_bra_component(const PrimPair* __restrict__ bra, int nb
 // digest_quartet).  fac is the degeneracy factor.
 template <int LA, int LB, int LC, int LD, int T>
 SF_HD SF_INLINE void digest_tile(const double* __restrict__ v, long long vs, double fac,
-                                 const double* __restrict__ P, int nbf, int fa0, int fb0, int fc0,
-                                 int fd0, double* __restrict__ t)
+                                 const double* __restrict__ P, const QuartetMap& qm,
+                                 double* __restrict__ t)
 {
     using S = QuartetShape<LA, LB, LC, LD>;
     constexpr int N1 = (T == 0 || T == 2 || T == 3) ? S::NA : (T == 1) ? S::NC : S::NB;
@@ -548,13 +558,15 @@ SF_HD SF_INLINE void digest_tile(const double* __restrict__ v, long long vs, dou
     // density block contracted against: T0 P_cd, T1 P_ab, T2 P_bd, T3 P_bc, T4 P_ad, T5 P_ac
     constexpr int M1 = (T == 0) ? S::NC : (T == 1) ? S::NA : (T == 2 || T == 3) ? S::NB : S::NA;
     constexpr int M2 = (T == 0) ? S::ND : (T == 1) ? S::NB : (T == 2) ? S::ND : (T == 3) ? S::NC : (T == 4) ? S::ND : S::NC;
-    const int r0 = (T == 0) ? fc0 : (T == 1) ? fa0 : (T == 2 || T == 3) ? fb0 : fa0;
-    const int c0 = (T == 0) ? fd0 : (T == 1) ? fb0 : (T == 2) ? fd0 : (T == 3) ? fc0 : (T == 4) ? fd0 : fc0;
+    // shells (0 a, 1 b, 2 c, 3 d) of the density block's rows and columns
+    constexpr int RS = (T == 0) ? 2 : (T == 1) ? 0 : (T == 2 || T == 3) ? 1 : 0;
+    constexpr int CS = (T == 0) ? 3 : (T == 1) ? 1 : (T == 2) ? 3 : (T == 3) ? 2 : (T == 4) ? 3 : 2;
+    const double* Pr = P + qm.row[RS] + qm.col[CS];
     double Pm[M1][M2];
 #pragma unroll
     for (int x = 0; x < M1; ++x)
 #pragma unroll
-        for (int y = 0; y < M2; ++y) Pm[x][y] = P[(long)(r0 + x) * nbf + c0 + y];
+        for (int y = 0; y < M2; ++y) Pm[x][y] = Pr[(long long)x * qm.ld + y];
 #pragma unroll
     for (int e = 0; e < N1 * N2; ++e) t[e] = 0.0;
 #pragma unroll
@@ -583,32 +595,35 @@ SF_HD SF_INLINE void digest_tile(const double* __restrict__ v, long long vs, dou
 // so that over all unique quartets J = X + X^T and K = Y + Y^T reproduce
 // einsum('pqrs,rs') and einsum('prqs,rs') of hf.py:300.
 template <int LA, int LB, int LC, int LD, typename VT>
-SF_HD SF_INLINE void digest_quartet(const VT* __restrict__ v, const double* __restrict__ P, int nbf,
-                                    int fa0, int fb0, int fc0, int fd0, double* __restrict__ t)
+SF_HD SF_INLINE void digest_quartet(const VT* __restrict__ v, const double* __restrict__ P,
+                                    const QuartetMap& qm, double* __restrict__ t)
 {
     using S = QuartetShape<LA, LB, LC, LD>;
     double Pab[S::NA][S::NB], Pcd[S::NC][S::ND], Pac[S::NA][S::NC], Pad[S::NA][S::ND],
         Pbc[S::NB][S::NC], Pbd[S::NB][S::ND];
+    const long long ld = qm.ld;
 #pragma unroll
     for (int a = 0; a < S::NA; ++a) {
+        const double* Pa = P + qm.row[0] + a * ld;
 #pragma unroll
-        for (int b = 0; b < S::NB; ++b) Pab[a][b] = P[(long)(fa0 + a) * nbf + fb0 + b];
+        for (int b = 0; b < S::NB; ++b) Pab[a][b] = Pa[qm.col[1] + b];
 #pragma unroll
-        for (int c = 0; c < S::NC; ++c) Pac[a][c] = P[(long)(fa0 + a) * nbf + fc0 + c];
+        for (int c = 0; c < S::NC; ++c) Pac[a][c] = Pa[qm.col[2] + c];
 #pragma unroll
-        for (int d = 0; d < S::ND; ++d) Pad[a][d] = P[(long)(fa0 + a) * nbf + fd0 + d];
+        for (int d = 0; d < S::ND; ++d) Pad[a][d] = Pa[qm.col[3] + d];
     }
 #pragma unroll
     for (int b = 0; b < S::NB; ++b) {
+        const double* Pb = P + qm.row[1] + b * ld;
 #pragma unroll
-        for (int c = 0; c < S::NC; ++c) Pbc[b][c] = P[(long)(fb0 + b) * nbf + fc0 + c];
+        for (int c = 0; c < S::NC; ++c) Pbc[b][c] = Pb[qm.col[2] + c];
 #pragma unroll
-        for (int d = 0; d < S::ND; ++d) Pbd[b][d] = P[(long)(fb0 + b) * nbf + fd0 + d];
+        for (int d = 0; d < S::ND; ++d) Pbd[b][d] = Pb[qm.col[3] + d];
     }
 #pragma unroll
     for (int c = 0; c < S::NC; ++c)
 #pragma unroll
-        for (int d = 0; d < S::ND; ++d) Pcd[c][d] = P[(long)(fc0 + c) * nbf + fd0 + d];
+        for (int d = 0; d < S::ND; ++d) Pcd[c][d] = P[qm.row[2] + c * ld + qm.col[3] + d];
 #pragma unroll
     for (int i = 0; i < S::NTILE; ++i) t[i] = 0.0;
 #pragma unroll

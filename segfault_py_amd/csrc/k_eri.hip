@@ -30,7 +30,8 @@ namespace sf {
 using SchwarzLaunch = int (*)(const int*, const int*, const int*, const int*, int, const PrimPair*,
                               const double*, double*, hipStream_t);
 #define X(id, a, b, c, d)                                                                       \
-    int launch_class_##id(int fp32, int mode, const ClassParams& cp, hipStream_t s);           \
+    int launch_class_##id(int fp32, int mode, const ClassParams& cp, const Tune& tn,           \
+                          hipStream_t s);                                                      \
     int launch_schwarz_##id(const int*, const int*, const int*, const int*, int,               \
                             const PrimPair*, const double*, double*, hipStream_t);             \
     int launch_grad_##id(const ClassParams& cp, double* gout, hipStream_t s);
@@ -38,6 +39,12 @@ SF_FOR_EACH_CLASS(X)
 #undef X
 
 #ifdef SF_CLASS
+
+#define SF_TRY_K(x)                   \
+    do {                              \
+        const int rc_ = (x);          \
+        if (rc_ != kOk) return rc_;   \
+    } while (0)
 
 __device__ __forceinline__ int find_bra(const long long* __restrict__ base, int n, long long q)
 {
@@ -50,15 +57,49 @@ __device__ __forceinline__ int find_bra(const long long* __restrict__ base, int 
     return lo;
 }
 
+// Flat segment index q -> bra i, ket j (index in the ket list) and the
+// class-wide quartet index qa.
+struct QuartetIdx {
+    int i, j;
+    long long qa;
+};
+
+__device__ __forceinline__ QuartetIdx locate(const ClassParams& cp, long long q)
+{
+    QuartetIdx r;
+    r.i = find_bra(cp.seg_base, cp.bra.n, q);
+    const int jr = cp.seg_k0[r.i] + (int)(q - cp.seg_base[r.i]);
+    r.j = cp.ket0[r.i] + jr;
+    r.qa = cp.qall_base[r.i] + jr;
+    return r;
+}
+
+__device__ __forceinline__ QuartetMap quartet_map(const ShellMap& m, int a, int b, int c, int d)
+{
+    QuartetMap qm;
+    qm.row[0] = m.row[a];
+    qm.row[1] = m.row[b];
+    qm.row[2] = m.row[c];
+    qm.row[3] = m.row[d];
+    qm.col[0] = m.col[a];
+    qm.col[1] = m.col[b];
+    qm.col[2] = m.col[c];
+    qm.col[3] = m.col[d];
+    qm.ld = m.ld[a];
+    return qm;
+}
+
+// One thread per shell quartet; the blocks stride over the segment, so a
+// capped grid (Tune::grid) makes them persistent.
 template <int LA, int LB, int LC, int LD, typename R, int MODE>
 __global__ __launch_bounds__(256) void k_eri_class(const ClassParams cp)
 {
     using S = QuartetShape<LA, LB, LC, LD>;
-    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= cp.nq) return;
-    const int i = find_bra(cp.seg_base, cp.bra.n, q);
-    const int j = cp.seg_k0[i] + (int)(q - cp.seg_base[i]);
-    const long long qa = cp.qall_base[i] + j;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < cp.nq; q += stride) {
+    const QuartetIdx ix = locate(cp, q);
+    const int i = ix.i, j = ix.j;
+    const long long qa = ix.qa;
     const int a = cp.bra.sa[i], b = cp.bra.sb[i];
     const int c = cp.ket.sa[j], d = cp.ket.sb[j];
     const double A[3] = {cp.shell_xyz[3 * a], cp.shell_xyz[3 * a + 1], cp.shell_xyz[3 * a + 2]};
@@ -80,8 +121,7 @@ __global__ __launch_bounds__(256) void k_eri_class(const ClassParams cp)
 #pragma unroll
         for (int k = 0; k < S::NINT; ++k) v[k] = fac * (double)acc[k];
         double t[S::NTILE];
-        digest_quartet<LA, LB, LC, LD, double>(v, cp.P, cp.nbf, cp.shell_foff[a], cp.shell_foff[b],
-                                               cp.shell_foff[c], cp.shell_foff[d], t);
+        digest_quartet<LA, LB, LC, LD, double>(v, cp.P, quartet_map(cp.map, a, b, c, d), t);
         // each tile goes to its destination-sorted slot (contiguous elements)
 #pragma unroll
         for (int T = 0; T < 6; ++T) {
@@ -89,6 +129,7 @@ __global__ __launch_bounds__(256) void k_eri_class(const ClassParams cp)
 #pragma unroll
             for (int e = 0; e < S::tile_n1(T) * S::tile_n2(T); ++e) dst[e] = t[S::tile_off(T) + e];
         }
+    }
     }
 }
 
@@ -129,13 +170,16 @@ __global__ __launch_bounds__(256) void k_eri_split(const ClassParams cp, double*
 {
     using S = QuartetShape<LA, LB, LC, LD>;
     constexpr int NS = S::NA * S::NB, NK = S::NC * S::ND;
-    const long long gw = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const long long nwork = ((cp.nq + 63) >> 6) * NS;
+    const long long wstride = ((long long)gridDim.x * blockDim.x) >> 6;
+    for (long long gw = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6; gw < nwork;
+         gw += wstride) {
     const int s = __builtin_amdgcn_readfirstlane((int)(gw % NS));
     const long long q = (gw / NS) * 64 + (threadIdx.x & 63);
-    if (q >= cp.nq) return;
-    const int i = find_bra(cp.seg_base, cp.bra.n, q);
-    const int j = cp.seg_k0[i] + (int)(q - cp.seg_base[i]);
-    const long long qa = cp.qall_base[i] + j;
+    if (q >= cp.nq) continue;
+    const QuartetIdx ix = locate(cp, q);
+    const int i = ix.i, j = ix.j;
+    const long long qa = ix.qa;
     const int a = cp.bra.sa[i], b = cp.bra.sb[i];
     const int c = cp.ket.sa[j], d = cp.ket.sb[j];
     const double A[3] = {cp.shell_xyz[3 * a], cp.shell_xyz[3 * a + 1], cp.shell_xyz[3 * a + 2]};
@@ -153,6 +197,7 @@ __global__ __launch_bounds__(256) void k_eri_split(const ClassParams cp, double*
     });
 #pragma unroll
     for (int k = 0; k < NK; ++k) raw[(long long)(s * NK + k) * cp.qstride + qa] = (double)acc[k];
+    }
 }
 
 // Split classes, pass 2: one wavefront per (64 quartets, digestion tile).
@@ -161,27 +206,31 @@ __global__ __launch_bounds__(256) void k_digest_split(const ClassParams cp,
                                                       const double* __restrict__ raw)
 {
     using S = QuartetShape<LA, LB, LC, LD>;
-    const long long gw = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const long long nwork = ((cp.nq + 63) >> 6) * 6;
+    const long long wstride = ((long long)gridDim.x * blockDim.x) >> 6;
+    for (long long gw = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6; gw < nwork;
+         gw += wstride) {
     const int t = __builtin_amdgcn_readfirstlane((int)(gw % 6));
     const long long q = (gw / 6) * 64 + (threadIdx.x & 63);
-    if (q >= cp.nq) return;
-    const int i = find_bra(cp.seg_base, cp.bra.n, q);
-    const int j = cp.seg_k0[i] + (int)(q - cp.seg_base[i]);
-    const long long qa = cp.qall_base[i] + j;
+    if (q >= cp.nq) continue;
+    const QuartetIdx ix = locate(cp, q);
+    const int i = ix.i, j = ix.j;
+    const long long qa = ix.qa;
     const int a = cp.bra.sa[i], b = cp.bra.sb[i];
     const int c = cp.ket.sa[j], d = cp.ket.sb[j];
     const double fac = ((a == b) ? 0.5 : 1.0) * ((c == d) ? 0.5 : 1.0) *
                        ((cp.same && i == j) ? 0.5 : 1.0);
+    const QuartetMap qm = quartet_map(cp.map, a, b, c, d);
     static_switch<0, 6>(t, [&](auto tc) {
         constexpr int T = decltype(tc)::value;
         constexpr int NE = S::tile_n1(T) * S::tile_n2(T);
         double tile[NE];
-        digest_tile<LA, LB, LC, LD, T>(raw + qa, cp.qstride, fac, cp.P, cp.nbf, cp.shell_foff[a],
-                                       cp.shell_foff[b], cp.shell_foff[c], cp.shell_foff[d], tile);
+        digest_tile<LA, LB, LC, LD, T>(raw + qa, cp.qstride, fac, cp.P, qm, tile);
         double* dst = cp.tiles + cp.pos[T * cp.qstride + qa];
 #pragma unroll
         for (int e = 0; e < NE; ++e) dst[e] = tile[e];
     });
+    }
 }
 
 // Nuclear gradient, two-electron part: one wavefront per (64 quartets, bra
@@ -198,9 +247,9 @@ __global__ __launch_bounds__(256) void k_grad_split(const ClassParams cp, double
     const int s = __builtin_amdgcn_readfirstlane((int)(gw % NS));
     const long long q = (gw / NS) * 64 + (threadIdx.x & 63);
     if (q >= cp.nq) return;
-    const int i = find_bra(cp.seg_base, cp.bra.n, q);
-    const int j = cp.seg_k0[i] + (int)(q - cp.seg_base[i]);
-    const long long qa = cp.qall_base[i] + j;
+    const QuartetIdx ix = locate(cp, q);
+    const int i = ix.i, j = ix.j;
+    const long long qa = ix.qa;
     const int a = cp.bra.sa[i], b = cp.bra.sb[i];
     const int c = cp.ket.sa[j], d = cp.ket.sb[j];
     const double A[3] = {cp.shell_xyz[3 * a], cp.shell_xyz[3 * a + 1], cp.shell_xyz[3 * a + 2]};
@@ -209,25 +258,26 @@ __global__ __launch_bounds__(256) void k_grad_split(const ClassParams cp, double
     const double D[3] = {cp.shell_xyz[3 * d], cp.shell_xyz[3 * d + 1], cp.shell_xyz[3 * d + 2]};
     const double fac = ((a == b) ? 0.5 : 1.0) * ((c == d) ? 0.5 : 1.0) *
                        ((cp.same && i == j) ? 0.5 : 1.0);
-    const int fa0 = cp.shell_foff[a], fb0 = cp.shell_foff[b];
-    const int fc0 = cp.shell_foff[c], fd0 = cp.shell_foff[d];
+    const QuartetMap qm = quartet_map(cp.map, a, b, c, d);
     const double* P = cp.P;
-    const long n = cp.nbf;
+    const long long n = qm.ld;
     double g[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) g[k] = 0.0;
     static_switch<0, NS>(s, [&](auto sc) {
         constexpr int FA = decltype(sc)::value / S::NB, FB = decltype(sc)::value % S::NB;
-        const double pab = P[(fa0 + FA) * n + fb0 + FB];
+        const double* Pa = P + qm.row[0] + FA * n;   // row fa of P
+        const double* Pb = P + qm.row[1] + FB * n;   // row fb of P
+        const double pab = Pa[qm.col[1] + FB];
         double Df[NK];
 #pragma unroll
         for (int fc = 0; fc < S::NC; ++fc)
 #pragma unroll
             for (int fd = 0; fd < S::ND; ++fd)
                 Df[fc * S::ND + fd] =
-                    fac * (4.0 * pab * P[(fc0 + fc) * n + fd0 + fd] -
-                           P[(fa0 + FA) * n + fc0 + fc] * P[(fb0 + FB) * n + fd0 + fd] -
-                           P[(fa0 + FA) * n + fd0 + fd] * P[(fb0 + FB) * n + fc0 + fc]);
+                    fac * (4.0 * pab * P[qm.row[2] + fc * n + qm.col[3] + fd] -
+                           Pa[qm.col[2] + fc] * Pb[qm.col[3] + fd] -
+                           Pa[qm.col[3] + fd] * Pb[qm.col[2] + fc]);
         grad_bra_component<LA, LB, LC, LD, FA, FB, double>(cp.prim + cp.bra.poff[i], cp.bra.np[i],
                                                            cp.prim + cp.ket.poff[j], cp.ket.np[j],
                                                            A, B, C, D, Df, g);
@@ -252,30 +302,53 @@ static int launch_grad_t(const ClassParams& cp, double* gout, hipStream_t s)
     return kOk;
 }
 
+// Blocks for `items` work items of `per_block` each, capped by the tuned
+// persistent grid.  Fails when the launch would exceed the grid limit.
+static int grid_blocks(long long items, long long per_block, const Tune& tn, unsigned* out)
+{
+    long long nb = (items + per_block - 1) / per_block;
+    if (tn.grid > 0 && nb > tn.grid) nb = tn.grid;
+    if (nb > 0x7fffffffLL) {
+        set_error(kEUnsupported, "quartet segment too large (%lld work items)", items);
+        return kEUnsupported;
+    }
+    *out = (unsigned)(nb > 0 ? nb : 1);
+    return kOk;
+}
+
+static int check_block(const Tune& tn)
+{
+    if (tn.block < 64 || tn.block > 256 || tn.block % 64 != 0 || tn.grid < 0) {
+        set_error(kEInval, "launch shape block=%d grid=%d (block 64..256, multiple of 64)", tn.block,
+                  tn.grid);
+        return kEInval;
+    }
+    return kOk;
+}
+
 template <int LA, int LB, int LC, int LD>
-static int launch_split(int fp32, int mode, const ClassParams& cp, hipStream_t s)
+static int launch_split(int fp32, int mode, const ClassParams& cp, const Tune& tn, hipStream_t s)
 {
     using S = QuartetShape<LA, LB, LC, LD>;
     const long long waves = (cp.nq + 63) / 64;
-    const long long nb1 = (waves * S::NA * S::NB + 3) / 4, nb2 = (waves * 6 + 3) / 4;
-    if (nb1 > 0x7fffffffLL) {
-        set_error(kEUnsupported, "quartet segment too large (%lld)", cp.nq);
-        return kEUnsupported;
-    }
+    const int wpb = tn.block / 64;
+    unsigned nb1 = 0, nb2 = 0;
+    SF_TRY_K(grid_blocks(waves * S::NA * S::NB, wpb, tn, &nb1));
+    SF_TRY_K(grid_blocks(waves * 6, wpb, tn, &nb2));
     double* raw = (mode == kModeRaw) ? cp.out : cp.raw;
     if (!raw) {
         set_error(kEState, "split class without a raw buffer");
         return kEState;
     }
     if (fp32 && mode != kModeRaw)
-        hipLaunchKernelGGL((k_eri_split<LA, LB, LC, LD, float>), dim3((unsigned)nb1), dim3(256), 0, s,
+        hipLaunchKernelGGL((k_eri_split<LA, LB, LC, LD, float>), dim3(nb1), dim3(tn.block), 0, s,
                            cp, raw);
     else
-        hipLaunchKernelGGL((k_eri_split<LA, LB, LC, LD, double>), dim3((unsigned)nb1), dim3(256), 0,
-                           s, cp, raw);
+        hipLaunchKernelGGL((k_eri_split<LA, LB, LC, LD, double>), dim3(nb1), dim3(tn.block), 0, s,
+                           cp, raw);
     SF_HIP(hipGetLastError());
     if (mode == kModeJK) {
-        hipLaunchKernelGGL((k_digest_split<LA, LB, LC, LD>), dim3((unsigned)nb2), dim3(256), 0, s, cp,
+        hipLaunchKernelGGL((k_digest_split<LA, LB, LC, LD>), dim3(nb2), dim3(tn.block), 0, s, cp,
                            (const double*)raw);
         SF_HIP(hipGetLastError());
     }
@@ -283,27 +356,24 @@ static int launch_split(int fp32, int mode, const ClassParams& cp, hipStream_t s
 }
 
 template <int LA, int LB, int LC, int LD>
-static int launch_class(int fp32, int mode, const ClassParams& cp, hipStream_t s)
+static int launch_class(int fp32, int mode, const ClassParams& cp, const Tune& tn, hipStream_t s)
 {
     if (cp.nq <= 0) return kOk;
+    SF_TRY_K(check_block(tn));
     if constexpr (QuartetShape<LA, LB, LC, LD>::SPLIT) {
-        return launch_split<LA, LB, LC, LD>(fp32, mode, cp, s);
+        return launch_split<LA, LB, LC, LD>(fp32, mode, cp, tn, s);
     } else {
-    const int bs = 256;
-    const long long nb = (cp.nq + bs - 1) / bs;
-    if (nb > 0x7fffffffLL) {
-        set_error(kEUnsupported, "quartet segment too large (%lld)", cp.nq);
-        return kEUnsupported;
-    }
-    dim3 grid((unsigned)nb), block(bs);
-    if (mode == kModeRaw)
-        hipLaunchKernelGGL((k_eri_class<LA, LB, LC, LD, double, kModeRaw>), grid, block, 0, s, cp);
-    else if (fp32)
-        hipLaunchKernelGGL((k_eri_class<LA, LB, LC, LD, float, kModeJK>), grid, block, 0, s, cp);
-    else
-        hipLaunchKernelGGL((k_eri_class<LA, LB, LC, LD, double, kModeJK>), grid, block, 0, s, cp);
-    SF_HIP(hipGetLastError());
-    return kOk;
+        unsigned nb = 0;
+        SF_TRY_K(grid_blocks(cp.nq, tn.block, tn, &nb));
+        dim3 grid(nb), block(tn.block);
+        if (mode == kModeRaw)
+            hipLaunchKernelGGL((k_eri_class<LA, LB, LC, LD, double, kModeRaw>), grid, block, 0, s, cp);
+        else if (fp32)
+            hipLaunchKernelGGL((k_eri_class<LA, LB, LC, LD, float, kModeJK>), grid, block, 0, s, cp);
+        else
+            hipLaunchKernelGGL((k_eri_class<LA, LB, LC, LD, double, kModeJK>), grid, block, 0, s, cp);
+        SF_HIP(hipGetLastError());
+        return kOk;
     }
 }
 
@@ -324,9 +394,10 @@ static int launch_schwarz_t(const int* sa, const int* sb, const int* poff, const
 }
 
 #define SF_INST(id, a, b, c, d)                                                                 \
-    int launch_class_##id(int fp32, int mode, const ClassParams& cp, hipStream_t s)            \
+    int launch_class_##id(int fp32, int mode, const ClassParams& cp, const Tune& tn,           \
+                          hipStream_t s)                                                       \
     {                                                                                          \
-        return launch_class<a, b, c, d>(fp32, mode, cp, s);                                    \
+        return launch_class<a, b, c, d>(fp32, mode, cp, tn, s);                                \
     }                                                                                          \
     int launch_schwarz_##id(const int* sa, const int* sb, const int* poff, const int* np, int n, \
                             const PrimPair* prim, const double* xyz, double* Q, hipStream_t s)  \
@@ -450,11 +521,12 @@ SF_FOR_EACH_CLASS(SF_PICK)
 
 static_assert(quartet_class(pair_class(2, 1), pair_class(1, 1)) == 12, "class numbering");
 
-int launch_eri_class(int qcls, int fp32, int mode, const ClassParams& cp, hipStream_t s)
+int launch_eri_class(int qcls, int fp32, int mode, const ClassParams& cp, const Tune& tn,
+                     hipStream_t s)
 {
     switch (qcls) {
 #define X(id, a, b, c, d) \
-    case id: return launch_class_##id(fp32, mode, cp, s);
+    case id: return launch_class_##id(fp32, mode, cp, tn, s);
         SF_FOR_EACH_CLASS(X)
 #undef X
     }

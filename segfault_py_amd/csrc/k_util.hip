@@ -72,10 +72,11 @@ int launch_rys(long long n, int nroots, const double* X, double* u, double* w, h
 }
 
 // ------------------------------------------------------ quartet list ------
-// For bra i (Q sorted descending in both lists): the surviving kets are the
-// prefix j < cut with Qbra[i] * Qket[j] >= tau (the product is monotone in
-// Qket), limited to j <= i when bra and ket are the same pair class.  The
-// FP64 part is the prefix that also meets tau_mixed.
+// For bra i (Q sorted descending within each molecule in both lists): the
+// surviving kets are the prefix of its molecule's kets, ket0[i] + j with
+// j < cut and Qbra[i] * Qket[ket0[i] + j] >= tau (the product is monotone in
+// Qket), limited to ket0[i] + j <= i when bra and ket are the same pair
+// class.  The FP64 part is the prefix that also meets tau_mixed.
 __device__ __forceinline__ int count_prefix(double qi, const double* __restrict__ Qk, int n, double tau)
 {
     int lo = 0, hi = n;
@@ -88,29 +89,34 @@ __device__ __forceinline__ int count_prefix(double qi, const double* __restrict_
 }
 
 __global__ void k_count_kets(const double* __restrict__ Qb, int nb, const double* __restrict__ Qk,
-                             int nk, int same, double tau, double tau_mixed, int* __restrict__ n_all,
+                             const int* __restrict__ ket0, const int* __restrict__ nket, int same,
+                             double tau, double tau_mixed, int* __restrict__ n_all,
                              int* __restrict__ n_fp64)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nb) return;
     const double qi = Qb[i];
-    int n = count_prefix(qi, Qk, nk, tau);
-    if (same && n > i + 1) n = i + 1;
+    const double* Qm = Qk + ket0[i];
+    const int nk = nket[i];
+    int n = count_prefix(qi, Qm, nk, tau);
+    const int own = i - ket0[i];  // the bra's own position among its molecule's kets
+    if (same && n > own + 1) n = own + 1;
     int n64 = n;
     if (tau_mixed > tau) {
-        n64 = count_prefix(qi, Qk, nk, tau_mixed);
+        n64 = count_prefix(qi, Qm, nk, tau_mixed);
         if (n64 > n) n64 = n;
     }
     n_all[i] = n;
     n_fp64[i] = n64;
 }
 
-int launch_count_kets(const double* Qbra, int nbra, const double* Qket, int nket, int same,
-                      double tau, double tau_mixed, int* n_all, int* n_fp64, hipStream_t s)
+int launch_count_kets(const double* Qbra, int nbra, const double* Qket, const int* ket0,
+                      const int* nket, int same, double tau, double tau_mixed, int* n_all,
+                      int* n_fp64, hipStream_t s)
 {
     if (nbra <= 0) return kOk;
     hipLaunchKernelGGL(k_count_kets, dim3((nbra + 255) / 256), dim3(256), 0, s, Qbra, nbra, Qket,
-                       nket, same, tau, tau_mixed, n_all, n_fp64);
+                       ket0, nket, same, tau, tau_mixed, n_all, n_fp64);
     SF_HIP(hipGetLastError());
     return kOk;
 }
@@ -127,21 +133,28 @@ __device__ __forceinline__ int find_base(const long long* __restrict__ base, int
     return lo;
 }
 
+// Six plan entries per quartet: destination key of each digestion tile
+// (molecule-local dense numbering, see KeyMap) and the code
+// class | tile | quartet that k_plan_positions decodes.
 __global__ __launch_bounds__(256) void k_plan_entries(int qcls, const PairListDev bra,
                                                       const PairListDev ket,
+                                                      const int* __restrict__ ket0,
                                                       const long long* __restrict__ qall_base,
-                                                      long long nq, int ns,
+                                                      long long nq, const KeyMap km,
                                                       unsigned int* __restrict__ keys,
                                                       unsigned long long* __restrict__ vals)
 {
     const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= nq) return;
     const int i = find_base(qall_base, bra.n, q);
-    const int j = (int)(q - qall_base[i]);
-    const unsigned a = bra.sa[i], b = bra.sb[i], c = ket.sa[j], d = ket.sb[j];
-    const unsigned n2 = (unsigned)ns * (unsigned)ns;
-    const unsigned k[6] = {a * ns + b, c * ns + d, n2 + a * ns + c, n2 + a * ns + d,
-                           n2 + b * ns + c, n2 + b * ns + d};
+    const int j = ket0[i] + (int)(q - qall_base[i]);
+    const int m = km.sh_mol[bra.sa[i]];
+    const unsigned ns = (unsigned)km.mol_ns[m], kb = km.mol_kbase[m];
+    const unsigned a = km.sh_loc[bra.sa[i]], b = km.sh_loc[bra.sb[i]];
+    const unsigned c = km.sh_loc[ket.sa[j]], d = km.sh_loc[ket.sb[j]];
+    const unsigned n2 = ns * ns;
+    const unsigned k[6] = {kb + a * ns + b,      kb + c * ns + d,      kb + n2 + a * ns + c,
+                           kb + n2 + a * ns + d, kb + n2 + b * ns + c, kb + n2 + b * ns + d};
 #pragma unroll
     for (int t = 0; t < 6; ++t) {
         keys[6 * q + t] = k[t];
@@ -150,13 +163,13 @@ __global__ __launch_bounds__(256) void k_plan_entries(int qcls, const PairListDe
     }
 }
 
-int launch_plan_entries(int qcls, const PairListDev& bra, const PairListDev& ket,
-                        const long long* qall_base, long long nq, int nshell, unsigned int* keys,
-                        unsigned long long* vals, hipStream_t s)
+int launch_plan_entries(int qcls, const PairListDev& bra, const PairListDev& ket, const int* ket0,
+                        const long long* qall_base, long long nq, const KeyMap& km,
+                        unsigned int* keys, unsigned long long* vals, hipStream_t s)
 {
     if (nq <= 0) return kOk;
     hipLaunchKernelGGL(k_plan_entries, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, s, qcls,
-                       bra, ket, qall_base, nq, nshell, keys, vals);
+                       bra, ket, ket0, qall_base, nq, km, keys, vals);
     SF_HIP(hipGetLastError());
     return kOk;
 }
@@ -221,23 +234,19 @@ int launch_plan_positions(const unsigned int* keys, const unsigned long long* va
 // them coalesced; a fixed strided assignment of tiles to lanes and a fixed
 // xor butterfly make the sum bitwise reproducible without atomics.
 template <int N1, int N2>
-__global__ __launch_bounds__(256) void k_reduce(const int* __restrict__ dest_keys, int ndest,
+__global__ __launch_bounds__(256) void k_reduce(const DestList dl,
                                                 const long long* __restrict__ seg_begin,
                                                 const long long* __restrict__ seg_end,
                                                 const unsigned long long* __restrict__ key_base,
-                                                const double* __restrict__ tiles,
-                                                const int* __restrict__ foff, int ns, int nbf,
+                                                const double* __restrict__ tiles, const ShellMap map,
                                                 double* __restrict__ X, double* __restrict__ Y)
 {
     constexpr int NE = N1 * N2;
     const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (w >= ndest) return;
+    if (w >= dl.n) return;
     const int lane = threadIdx.x & 63;
-    const int key = dest_keys[w];
-    const int n2 = ns * ns;
-    const int m = key / n2;
-    const int rem = key - m * n2;
-    const int s1 = rem / ns, s2 = rem - (rem / ns) * ns;
+    const int key = dl.key[w];
+    const int s1 = dl.s1[w], s2 = dl.s2[w];
     const long long cnt = seg_end[key] - seg_begin[key];
     const double* src = tiles + key_base[key];
     double acc[NE];
@@ -253,23 +262,23 @@ __global__ __launch_bounds__(256) void k_reduce(const int* __restrict__ dest_key
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) acc[k] += __shfl_xor(acc[k], o, 64);
     if (lane == 0) {
-        double* dst = (m == 0) ? X : Y;
+        double* dst = (dl.xy[w] == 0 ? X : Y) + map.row[s1] + map.col[s2];
+        const long long ld = map.ld[s1];
 #pragma unroll
-        for (int k = 0; k < NE; ++k)
-            dst[(long long)(foff[s1] + k / N2) * nbf + foff[s2] + k % N2] = acc[k];
+        for (int k = 0; k < NE; ++k) dst[(k / N2) * ld + k % N2] = acc[k];
     }
 }
 
-int launch_reduce(int n1, int n2, const int* dest_keys, int ndest, const long long* seg_begin,
-                  const long long* seg_end, const unsigned long long* key_base, const double* tiles,
-                  const int* shell_foff, int nshell, int nbf, double* X, double* Y, hipStream_t s)
+int launch_reduce(int n1, int n2, const DestList& dl, const long long* seg_begin,
+                  const long long* seg_end, const unsigned long long* key_base,
+                  const double* tiles, const ShellMap& map, double* X, double* Y, hipStream_t s)
 {
-    if (ndest <= 0) return kOk;
-    dim3 g((ndest + 3) / 4), b(256);
+    if (dl.n <= 0) return kOk;
+    dim3 g((dl.n + 3) / 4), b(256);
 #define R_(a, c)                                                                              \
     if (n1 == a && n2 == c) {                                                                 \
-        hipLaunchKernelGGL((k_reduce<a, c>), g, b, 0, s, dest_keys, ndest, seg_begin, seg_end,  \
-                           key_base, tiles, shell_foff, nshell, nbf, X, Y);                   \
+        hipLaunchKernelGGL((k_reduce<a, c>), g, b, 0, s, dl, seg_begin, seg_end, key_base,      \
+                           tiles, map, X, Y);                                                 \
         SF_HIP(hipGetLastError());                                                            \
         return kOk;                                                                           \
     }
@@ -428,43 +437,68 @@ int launch_grad_rows(const double* rows, long long nrows, int ncol, const double
 }
 
 // --------------------------------------------------- small dense kernels --
-__global__ void k_scale_density(const double* __restrict__ P, const double* __restrict__ f, int n,
-                                double* __restrict__ Ps)
+// Per-molecule element loops: blockIdx.y = molecule, x over its n^2 elements.
+static int mol_grid(const MolLayout& ml, dim3* g)
 {
-    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (long long)n * n) return;
-    const int i = (int)(idx / n), j = (int)(idx % n);
-    Ps[idx] = f[i] * f[j] * P[idx];
+    const long long per = (long long)ml.nmax * ml.nmax;
+    if (ml.nmol <= 0 || ml.nmol > 65535 || per <= 0) {
+        set_error(kEUnsupported, "layout of %d molecules (at most 65535)", ml.nmol);
+        return kEUnsupported;
+    }
+    *g = dim3((unsigned)((per + 255) / 256), (unsigned)ml.nmol);
+    return kOk;
 }
 
-int launch_scale_density(const double* P, const double* fnorm, int nbf, double* Ps, hipStream_t s)
+__global__ void k_scale_density(const double* __restrict__ P, const double* __restrict__ f,
+                                const MolLayout ml, double* __restrict__ Ps)
 {
-    const long long n = (long long)nbf * nbf;
-    hipLaunchKernelGGL(k_scale_density, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, P, fnorm,
-                       nbf, Ps);
+    const int m = blockIdx.y;
+    const int n = ml.n[m];
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long long)n * n) return;
+    const int i = (int)(e / n), j = (int)(e % n);
+    const double* fm = f + ml.fbase[m];
+    const long long idx = ml.off[m] + (long long)i * ml.ld[m] + j;
+    Ps[idx] = fm[i] * fm[j] * P[idx];
+}
+
+int launch_scale_density(const double* P, const double* fnorm, const MolLayout& ml, double* Ps,
+                         hipStream_t s)
+{
+    dim3 g;
+    const int rc = mol_grid(ml, &g);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_scale_density, g, dim3(256), 0, s, P, fnorm, ml, Ps);
     SF_HIP(hipGetLastError());
     return kOk;
 }
 
+// J = D (X + X^T) D, K = D (Y + Y^T) D per molecule block (D = per-component
+// normalisation factors).
 __global__ void k_finalize(const double* __restrict__ X, const double* __restrict__ Y,
-                           const double* __restrict__ f, int n, double* __restrict__ J,
+                           const double* __restrict__ f, const MolLayout ml, double* __restrict__ J,
                            double* __restrict__ K)
 {
-    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (long long)n * n) return;
-    const int i = (int)(idx / n), j = (int)(idx % n);
-    const long long tr = (long long)j * n + i;
-    const double s = f[i] * f[j];
-    J[idx] = s * (X[idx] + X[tr]);
-    K[idx] = s * (Y[idx] + Y[tr]);
+    const int m = blockIdx.y;
+    const int n = ml.n[m];
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long long)n * n) return;
+    const int i = (int)(e / n), j = (int)(e % n);
+    const double* fm = f + ml.fbase[m];
+    const long long idx = ml.off[m] + (long long)i * ml.ld[m] + j;
+    const long long tr = ml.off[m] + (long long)j * ml.ld[m] + i;
+    const double sc = fm[i] * fm[j];
+    J[idx] = sc * (X[idx] + X[tr]);
+    K[idx] = sc * (Y[idx] + Y[tr]);
 }
 
-int launch_finalize(const double* X, const double* Y, const double* fnorm, int nbf, double* J,
-                    double* K, hipStream_t s)
+int launch_finalize(const double* X, const double* Y, const double* fnorm, const MolLayout& ml,
+                    double* J, double* K, hipStream_t s)
 {
-    const long long n = (long long)nbf * nbf;
-    hipLaunchKernelGGL(k_finalize, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, X, Y, fnorm,
-                       nbf, J, K);
+    dim3 g;
+    const int rc = mol_grid(ml, &g);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_finalize, g, dim3(256), 0, s, X, Y, fnorm, ml, J, K);
     SF_HIP(hipGetLastError());
     return kOk;
 }

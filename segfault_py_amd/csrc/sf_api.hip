@@ -1,5 +1,7 @@
-// sf_api.hip -- the C-ABI of libsfeng.so: basis handles, Schwarz screening,
-// quartet lists, the deterministic digestion plan and the J/K driver.
+// sf_api.hip -- the C-ABI of libsfeng.so: basis handles (one molecule or a
+// batch of independent molecules), Schwarz screening, quartet lists, the
+// deterministic digestion plan, the J/K driver with autotuned launch shapes
+// and graph replay, one-electron integrals and the nuclear gradient.
 //
 // The reference's call sites replaced here: rys.eri_rys via f2py
 // (src/hf/hf.py:26-30, :100-104), build_eri_parallel (hf.py:33-61) and the
@@ -92,12 +94,15 @@ static double dfact2(int n)
 using namespace sf;
 
 struct PairClassData {
-    // every pair of the class, canonical orientation (L(sa) >= L(sb))
-    std::vector<int> sa, sb, poff, np;
+    // every pair of the class, canonical orientation (L(sa) >= L(sb)), both
+    // shells in one molecule, molecule by molecule: molecule m's pairs are
+    // [mbeg[m], mbeg[m+1])
+    std::vector<int> sa, sb, poff, np, mbeg;
     std::vector<double> Q;
     DevBuf d_sa, d_sb, d_poff, d_np, d_Q;
-    // significant pairs, sorted by Q descending
-    std::vector<int> s_sa, s_sb, s_poff, s_np;
+    // significant pairs, molecule by molecule, sorted by Q descending inside
+    // each molecule; molecule m's are [s_mbeg[m], s_mbeg[m+1])
+    std::vector<int> s_sa, s_sb, s_poff, s_np, s_mol, s_mbeg;
     std::vector<double> s_Q;
     DevBuf ds_sa, ds_sb, ds_poff, ds_np, ds_Q;
     PairListDev sorted_dev() const
@@ -115,18 +120,36 @@ struct PairClassData {
 struct QuartetClassData {
     int pcb = 0, pck = 0, same = 0, nbra = 0;
     long long nq_all = 0, nq64 = 0, nq32 = 0;
-    std::vector<int> n_all, n64;
+    // per bra: surviving kets, FP64 kets, first ket and ket count of its molecule
+    std::vector<int> n_all, n64, ket0, nket;
     std::vector<long long> base_all, base64, base32;
-    DevBuf d_base_all, d_base64, d_base32, d_k032, d_pos, d_raw;
+    DevBuf d_base_all, d_base64, d_base32, d_k032, d_ket0, d_nket, d_pos, d_raw;
     int ntile = 0;
 };
 
+// A captured J/K build (sf_jk_graph) and the arguments it was captured for.
+struct JKGraph {
+    hipGraphExec_t exec = nullptr;
+    const double* P = nullptr;
+    double* J = nullptr;
+    double* K = nullptr;
+    int precision = -1;
+};
+
 struct sf_basis {
-    int nshell = 0, nbf = 0, natom = 0;
+    int nshell = 0, nbf = 0, natom = 0, nmol = 1;
     std::vector<int> L, nprim, pstart, foff, atom;
+    std::vector<int> sh_mol, sh_loc;       // molecule of each shell, index inside it
+    std::vector<int> mol_shell, mol_atom;  // nmol+1 ranges of shells and of nuclei
+    std::vector<int> mol_nbf, mol_fbase;   // functions per molecule, first global function
+    std::vector<long long> mol_off;        // layout: first element of each molecule's block
+    std::vector<int> mol_ld;               //         and its leading dimension
+    long long extent = 0;                  // elements spanned by the layout
     std::vector<double> xyz, expn, coef, fnorm;
     std::vector<PrimPair> prim;
     DevBuf d_prim, d_xyz, d_foff, d_fnorm, d_zero;
+    DevBuf d_row, d_col, d_ld, d_sh_mol, d_sh_loc, d_mol_ns, d_mol_kbase;
+    DevBuf d_mol_off, d_mol_ld, d_mol_n, d_mol_fbase, d_mol_nuc;
     // nuclei (point charges) for the one-electron potential
     int nnuc = 0;
     DevBuf d_Z, d_nuc, d_atom;
@@ -135,21 +158,55 @@ struct sf_basis {
     std::vector<long long> gpart_bra, gpart_ket;  // per class offsets into d_gpart
     DevBuf d_gpart, d_gout, d_gcsr_off, d_gcsr_idx, d_grows, d_Ws, d_g2, d_gtot;
     PairClassData pc[kNumPairClass];
-    std::vector<int> pair_slot;  // for pair (a>=b) index: (pc << 24) | position in pc arrays
+    std::vector<int> pair_slot;  // one molecule: pair (a>=b) -> (pc << 24) | position
     bool have_schwarz = false, screened = false;
     double tau = 0.0, tau_mixed = 0.0;
     QuartetClassData qc[kNumQuartetClass];
     long long funcq = 0, primq = 0;
     // digestion plan: destination-sorted tile slots
     DevBuf d_segb, d_sege, d_key_base, d_tiles;
-    DevBuf d_dest[9];
+    DevBuf d_dkey[9], d_ds1[9], d_ds2[9], d_dxy[9];
     int ndest[9] = {0};
     unsigned long long ntile_total = 0;
     DevBuf d_Ps, d_X, d_Y;
+    // launch shapes per quartet class and precision segment (sf_autotune)
+    Tune tune[kNumQuartetClass][2];
+    JKGraph graph;
+    hipStream_t cap_stream = nullptr;
+
+    ~sf_basis()
+    {
+        drop_graph();
+        if (cap_stream) (void)hipStreamDestroy(cap_stream);
+    }
+    void drop_graph()
+    {
+        if (graph.exec) (void)hipGraphExecDestroy(graph.exec);
+        graph = JKGraph();
+    }
+    ShellMap map() const
+    {
+        ShellMap m;
+        m.row = d_row.as<long long>();
+        m.col = d_col.as<int>();
+        m.ld = d_ld.as<int>();
+        return m;
+    }
+    MolLayout layout() const
+    {
+        MolLayout ml;
+        ml.off = d_mol_off.as<long long>();
+        ml.ld = d_mol_ld.as<int>();
+        ml.n = d_mol_n.as<int>();
+        ml.fbase = d_mol_fbase.as<int>();
+        ml.nmol = nmol;
+        ml.nmax = *std::max_element(mol_nbf.begin(), mol_nbf.end());
+        return ml;
+    }
 };
 
 extern "C" const char* sf_last_error(void) { return g_err; }
-extern "C" int sf_version(void) { return 100; }
+extern "C" int sf_version(void) { return 200; }
 
 // ------------------------------------------------------------ batches ----
 extern "C" int sf_boys(long long n, int mmax, const double* T, double* F, void* stream)
@@ -249,15 +306,68 @@ extern "C" int sf_eri_rys(const int npgf[4], const double* coefs, const double* 
 }
 
 // -------------------------------------------------------------- basis ----
-extern "C" int sf_basis_create(int nshell, const int* L, const int* nprim, const double* center,
-                               const double* expns, const double* coefs, int natom,
-                               const int* shell_atom, sf_basis** out)
+// Per-shell matrix positions and per-molecule layout arrays on the device,
+// internal buffers sized to the layout.  Called after creation and after
+// every sf_basis_set_layout.
+static int apply_layout(sf_basis* b)
 {
-    if (!out || nshell <= 0 || !L || !nprim || !center || !expns || !coefs || natom < 0) {
+    std::vector<long long> row(b->nshell);
+    std::vector<int> col(b->nshell), ld(b->nshell);
+    long long extent = 0;
+    for (int m = 0; m < b->nmol; ++m)
+        extent = std::max(extent, b->mol_off[m] + (long long)b->mol_nbf[m] * b->mol_ld[m]);
+    for (int s = 0; s < b->nshell; ++s) {
+        const int m = b->sh_mol[s];
+        const int lf = b->foff[s] - b->mol_fbase[m];
+        row[s] = b->mol_off[m] + (long long)lf * b->mol_ld[m];
+        col[s] = lf;
+        ld[s] = b->mol_ld[m];
+    }
+    b->extent = extent;
+    SF_TRY(b->d_row.upload(row));
+    SF_TRY(b->d_col.upload(col));
+    SF_TRY(b->d_ld.upload(ld));
+    SF_TRY(b->d_mol_off.upload(b->mol_off));
+    SF_TRY(b->d_mol_ld.upload(b->mol_ld));
+    SF_TRY(b->d_Ps.ensure((size_t)std::max(1LL, extent) * sizeof(double)));
+    SF_TRY(b->d_X.ensure((size_t)std::max(1LL, extent) * sizeof(double)));
+    SF_TRY(b->d_Y.ensure((size_t)std::max(1LL, extent) * sizeof(double)));
+    SF_TRY(b->d_Ws.ensure((size_t)std::max(1LL, extent) * sizeof(double)));
+    b->drop_graph();
+    return kOk;
+}
+
+// Shared by sf_basis_create (one molecule) and sf_basis_create_batch.
+static int create_basis(int nmol, const int* mol_shell, int nshell, const int* L, const int* nprim,
+                        const double* center, const double* expns, const double* coefs,
+                        const int* mol_atom, int natom, const int* shell_atom, sf_basis** out)
+{
+    if (!out || nshell <= 0 || nmol <= 0 || !mol_shell || !L || !nprim || !center || !expns ||
+        !coefs || natom < 0) {
         set_error(kEInval, "sf_basis_create: bad arguments");
         return kEInval;
     }
     *out = nullptr;
+    if (mol_shell[0] != 0 || mol_shell[nmol] != nshell) {
+        set_error(kEInval, "molecule shell ranges must start at 0 and end at nshell");
+        return kEInval;
+    }
+    for (int m = 0; m < nmol; ++m)
+        if (mol_shell[m + 1] <= mol_shell[m]) {
+            set_error(kEInval, "molecule %d has no shells", m);
+            return kEInval;
+        }
+    if (mol_atom) {
+        if (mol_atom[0] < 0 || mol_atom[nmol] > natom) {
+            set_error(kEInval, "molecule atom ranges outside [0, %d]", natom);
+            return kEInval;
+        }
+        for (int m = 0; m < nmol; ++m)
+            if (mol_atom[m + 1] < mol_atom[m]) {
+                set_error(kEInval, "molecule %d: decreasing atom range", m);
+                return kEInval;
+            }
+    }
     sf_basis* b = new (std::nothrow) sf_basis();
     if (!b) {
         set_error(kENoMem, "sf_basis_create: host allocation");
@@ -265,109 +375,153 @@ extern "C" int sf_basis_create(int nshell, const int* L, const int* nprim, const
     }
     b->nshell = nshell;
     b->natom = natom;
+    b->nmol = nmol;
+    b->mol_shell.assign(mol_shell, mol_shell + nmol + 1);
+    if (mol_atom) b->mol_atom.assign(mol_atom, mol_atom + nmol + 1);
+    else b->mol_atom = {0, natom};
     int off = 0, pst = 0;
-    for (int s = 0; s < nshell; ++s) {
-        if (L[s] < 0 || L[s] > kMaxL) {
-            set_error(kEUnsupported, "shell %d: L = %d (s, p, d supported)", s, L[s]);
+    for (int m = 0; m < nmol; ++m) {
+        b->mol_fbase.push_back(off);
+        for (int s = mol_shell[m]; s < mol_shell[m + 1]; ++s) {
+            if (L[s] < 0 || L[s] > kMaxL) {
+                set_error(kEUnsupported, "shell %d: L = %d (s, p, d supported)", s, L[s]);
+                delete b;
+                return kEUnsupported;
+            }
+            if (nprim[s] <= 0 || nprim[s] > 64) {
+                set_error(kEInval, "shell %d: nprim = %d", s, nprim[s]);
+                delete b;
+                return kEInval;
+            }
+            b->L.push_back(L[s]);
+            b->nprim.push_back(nprim[s]);
+            b->pstart.push_back(pst);
+            b->foff.push_back(off);
+            b->atom.push_back(shell_atom ? shell_atom[s] : 0);
+            b->sh_mol.push_back(m);
+            b->sh_loc.push_back(s - mol_shell[m]);
+            for (int k = 0; k < 3; ++k) b->xyz.push_back(center[3 * s + k]);
+            for (int i = 0; i < nprim[s]; ++i) {
+                b->expn.push_back(expns[pst + i]);
+                b->coef.push_back(coefs[pst + i]);
+            }
+            // per-component angular factors, reference component order
+            for (int lx = L[s]; lx >= 0; --lx)
+                for (int ly = L[s] - lx; ly >= 0; --ly) {
+                    const int lz = L[s] - lx - ly;
+                    b->fnorm.push_back(1.0 / std::sqrt(dfact2(2 * lx - 1) * dfact2(2 * ly - 1) *
+                                                       dfact2(2 * lz - 1)));
+                }
+            off += (L[s] + 1) * (L[s] + 2) / 2;
+            pst += nprim[s];
+        }
+        b->mol_nbf.push_back(off - b->mol_fbase[m]);
+    }
+    b->nbf = off;
+    // plan keys: 2 ns^2 per molecule (X and Y destinations)
+    std::vector<unsigned> kbase(nmol);
+    std::vector<int> mol_ns(nmol);
+    unsigned long long kacc = 0;
+    for (int m = 0; m < nmol; ++m) {
+        mol_ns[m] = mol_shell[m + 1] - mol_shell[m];
+        kbase[m] = (unsigned)kacc;
+        kacc += 2ULL * mol_ns[m] * mol_ns[m];
+        if (kacc >= (1ULL << 31)) {
+            set_error(kEUnsupported, "too many shell pairs for the digestion plan");
             delete b;
             return kEUnsupported;
         }
-        if (nprim[s] <= 0 || nprim[s] > 64) {
-            set_error(kEInval, "shell %d: nprim = %d", s, nprim[s]);
-            delete b;
-            return kEInval;
-        }
-        b->L.push_back(L[s]);
-        b->nprim.push_back(nprim[s]);
-        b->pstart.push_back(pst);
-        b->foff.push_back(off);
-        b->atom.push_back(shell_atom ? shell_atom[s] : 0);
-        for (int k = 0; k < 3; ++k) b->xyz.push_back(center[3 * s + k]);
-        for (int i = 0; i < nprim[s]; ++i) {
-            b->expn.push_back(expns[pst + i]);
-            b->coef.push_back(coefs[pst + i]);
-        }
-        // per-component angular factors, reference component order
-        for (int lx = L[s]; lx >= 0; --lx)
-            for (int ly = L[s] - lx; ly >= 0; --ly) {
-                const int lz = L[s] - lx - ly;
-                b->fnorm.push_back(1.0 / std::sqrt(dfact2(2 * lx - 1) * dfact2(2 * ly - 1) *
-                                                   dfact2(2 * lz - 1)));
-            }
-        off += (L[s] + 1) * (L[s] + 2) / 2;
-        pst += nprim[s];
     }
-    b->nbf = off;
-    // all shell pairs a >= b, canonical orientation, primitive pair data.
-    // Primitive-pair screening: with Kp = K/p, a primitive quartet is bounded
-    // by 2 pi^{5/2} |Kp_ab| |Kp_cd| / sqrt(p+q) times its polynomial factors;
-    // pairs with 2 pi^{5/2} |Kp_ab| M / sqrt(p_ab) < 1e-20 (M = largest |Kp|
-    // of the basis) are dropped, which changes no integral by more than ~1e-15.
-    std::vector<double> kp_all;
-    double kp_max = 0.0;
-    for (int a = 0; a < nshell; ++a)
-        for (int c = 0; c <= a; ++c) {
-            const double* A = &b->xyz[3 * a];
-            const double* B = &b->xyz[3 * c];
-            const double AB2 = (A[0] - B[0]) * (A[0] - B[0]) + (A[1] - B[1]) * (A[1] - B[1]) +
-                               (A[2] - B[2]) * (A[2] - B[2]);
-            for (int i = 0; i < nprim[a]; ++i)
-                for (int j = 0; j < nprim[c]; ++j) {
-                    const double ea = b->expn[b->pstart[a] + i], eb = b->expn[b->pstart[c] + j];
-                    const double p = ea + eb;
-                    const double K = b->coef[b->pstart[a] + i] * b->coef[b->pstart[c] + j] *
-                                     std::exp(-ea * eb / p * AB2);
-                    kp_max = std::max(kp_max, std::fabs(K / p));
-                }
-        }
+    // Shell pairs of each molecule, canonical orientation, primitive pair
+    // data.  Primitive-pair screening: with Kp = K/p, a primitive quartet is
+    // bounded by 2 pi^{5/2} |Kp_ab| |Kp_cd| / sqrt(p+q) times its polynomial
+    // factors; pairs with 2 pi^{5/2} |Kp_ab| M / sqrt(p_ab) < 1e-20 (M =
+    // largest |Kp| of the molecule) are dropped, which changes no integral by
+    // more than ~1e-15.
     const double kPrimTol = 1e-20;
-    b->pair_slot.resize((size_t)nshell * (nshell + 1) / 2);
-    for (int a = 0; a < nshell; ++a)
-        for (int c = 0; c <= a; ++c) {
-            int sa = a, sb = c;
-            if (L[sb] > L[sa]) std::swap(sa, sb);
-            const int pcl = pair_class(L[sa], L[sb]);
-            PairClassData& P = b->pc[pcl];
-            P.sa.push_back(sa);
-            P.sb.push_back(sb);
-            P.poff.push_back((int)b->prim.size());
-            b->pair_slot[(size_t)a * (a + 1) / 2 + c] = (pcl << 24) | (int)(P.sa.size() - 1);
-            const double* A = &b->xyz[3 * sa];
-            const double* B = &b->xyz[3 * sb];
-            const double AB2 = (A[0] - B[0]) * (A[0] - B[0]) + (A[1] - B[1]) * (A[1] - B[1]) +
-                               (A[2] - B[2]) * (A[2] - B[2]);
-            int kept = 0;
-            for (int i = 0; i < nprim[sa]; ++i)
-                for (int j = 0; j < nprim[sb]; ++j) {
-                    const double ea = b->expn[b->pstart[sa] + i], eb = b->expn[b->pstart[sb] + j];
-                    const double p = ea + eb;
-                    PrimPair pp;
-                    pp.p = p;
-                    pp.Px = (ea * A[0] + eb * B[0]) / p;
-                    pp.Py = (ea * A[1] + eb * B[1]) / p;
-                    pp.Pz = (ea * A[2] + eb * B[2]) / p;
-                    pp.K = b->coef[b->pstart[sa] + i] * b->coef[b->pstart[sb] + j] *
-                           std::exp(-ea * eb / p * AB2);
-                    pp.ip = 1.0 / p;
-                    pp.Kp = pp.K / p;
-                    pp.eb = eb;
-                    if (34.986836655249725 * std::fabs(pp.Kp) * kp_max / std::sqrt(p) < kPrimTol)
-                        continue;
-                    b->prim.push_back(pp);
-                    ++kept;
-                }
-            P.np.push_back(kept);
-        }
-    if (b->pair_slot.size() > (1u << 24)) {
-        set_error(kEUnsupported, "too many shell pairs");
+    if (nmol == 1) b->pair_slot.resize((size_t)nshell * (nshell + 1) / 2);
+    for (auto& P : b->pc) P.mbeg.push_back(0);
+    for (int m = 0; m < nmol; ++m) {
+        const int s0 = mol_shell[m], s1 = mol_shell[m + 1];
+        double kp_max = 0.0;
+        for (int a = s0; a < s1; ++a)
+            for (int c = s0; c <= a; ++c) {
+                const double* A = &b->xyz[3 * a];
+                const double* B = &b->xyz[3 * c];
+                const double AB2 = (A[0] - B[0]) * (A[0] - B[0]) + (A[1] - B[1]) * (A[1] - B[1]) +
+                                   (A[2] - B[2]) * (A[2] - B[2]);
+                for (int i = 0; i < nprim[a]; ++i)
+                    for (int j = 0; j < nprim[c]; ++j) {
+                        const double ea = b->expn[b->pstart[a] + i], eb = b->expn[b->pstart[c] + j];
+                        const double p = ea + eb;
+                        const double K = b->coef[b->pstart[a] + i] * b->coef[b->pstart[c] + j] *
+                                         std::exp(-ea * eb / p * AB2);
+                        kp_max = std::max(kp_max, std::fabs(K / p));
+                    }
+            }
+        for (int a = s0; a < s1; ++a)
+            for (int c = s0; c <= a; ++c) {
+                int sa = a, sb = c;
+                if (L[sb] > L[sa]) std::swap(sa, sb);
+                const int pcl = pair_class(L[sa], L[sb]);
+                PairClassData& P = b->pc[pcl];
+                P.sa.push_back(sa);
+                P.sb.push_back(sb);
+                P.poff.push_back((int)b->prim.size());
+                if (nmol == 1)
+                    b->pair_slot[(size_t)a * (a + 1) / 2 + c] = (pcl << 24) | (int)(P.sa.size() - 1);
+                const double* A = &b->xyz[3 * sa];
+                const double* B = &b->xyz[3 * sb];
+                const double AB2 = (A[0] - B[0]) * (A[0] - B[0]) + (A[1] - B[1]) * (A[1] - B[1]) +
+                                   (A[2] - B[2]) * (A[2] - B[2]);
+                int kept = 0;
+                for (int i = 0; i < nprim[sa]; ++i)
+                    for (int j = 0; j < nprim[sb]; ++j) {
+                        const double ea = b->expn[b->pstart[sa] + i], eb = b->expn[b->pstart[sb] + j];
+                        const double p = ea + eb;
+                        PrimPair pp;
+                        pp.p = p;
+                        pp.Px = (ea * A[0] + eb * B[0]) / p;
+                        pp.Py = (ea * A[1] + eb * B[1]) / p;
+                        pp.Pz = (ea * A[2] + eb * B[2]) / p;
+                        pp.K = b->coef[b->pstart[sa] + i] * b->coef[b->pstart[sb] + j] *
+                               std::exp(-ea * eb / p * AB2);
+                        pp.ip = 1.0 / p;
+                        pp.Kp = pp.K / p;
+                        pp.eb = eb;
+                        if (34.986836655249725 * std::fabs(pp.Kp) * kp_max / std::sqrt(p) < kPrimTol)
+                            continue;
+                        b->prim.push_back(pp);
+                        ++kept;
+                    }
+                P.np.push_back(kept);
+            }
+        for (auto& P : b->pc) P.mbeg.push_back((int)P.sa.size());
+    }
+    if (b->prim.size() >= (1u << 31)) {
+        set_error(kEUnsupported, "too many primitive pairs");
         delete b;
         return kEUnsupported;
+    }
+    // default layout: molecule blocks packed back to back, ld = n
+    long long lo = 0;
+    for (int m = 0; m < nmol; ++m) {
+        b->mol_off.push_back(lo);
+        b->mol_ld.push_back(b->mol_nbf[m]);
+        lo += (long long)b->mol_nbf[m] * b->mol_nbf[m];
     }
     int rc = b->d_prim.upload(b->prim);
     if (!rc) rc = b->d_atom.upload(b->atom);
     if (!rc) rc = b->d_xyz.upload(b->xyz);
     if (!rc) rc = b->d_foff.upload(b->foff);
     if (!rc) rc = b->d_fnorm.upload(b->fnorm);
+    if (!rc) rc = b->d_sh_mol.upload(b->sh_mol);
+    if (!rc) rc = b->d_sh_loc.upload(b->sh_loc);
+    if (!rc) rc = b->d_mol_ns.upload(mol_ns);
+    if (!rc) rc = b->d_mol_kbase.upload(kbase);
+    if (!rc) rc = b->d_mol_n.upload(b->mol_nbf);
+    if (!rc) rc = b->d_mol_fbase.upload(b->mol_fbase);
+    if (!rc) rc = b->d_mol_nuc.upload(b->mol_atom);
     for (int p = 0; p < kNumPairClass && !rc; ++p) {
         PairClassData& P = b->pc[p];
         rc = P.d_sa.upload(P.sa);
@@ -376,11 +530,77 @@ extern "C" int sf_basis_create(int nshell, const int* L, const int* nprim, const
         if (!rc) rc = P.d_np.upload(P.np);
         if (!rc) rc = P.d_Q.ensure(std::max<size_t>(1, P.sa.size()) * sizeof(double));
     }
+    if (!rc) rc = apply_layout(b);
     if (rc) {
         delete b;
         return rc;
     }
     *out = b;
+    return kOk;
+}
+
+extern "C" int sf_basis_create(int nshell, const int* L, const int* nprim, const double* center,
+                               const double* expns, const double* coefs, int natom,
+                               const int* shell_atom, sf_basis** out)
+{
+    const int ms[2] = {0, nshell};
+    const int ma[2] = {0, natom < 0 ? 0 : natom};
+    return create_basis(1, ms, nshell, L, nprim, center, expns, coefs, ma, natom, shell_atom, out);
+}
+
+extern "C" int sf_basis_create_batch(int nmol, const int* mol_shell_begin, int nshell, const int* L,
+                                     const int* nprim, const double* center, const double* expns,
+                                     const double* coefs, int natom, const int* mol_atom_begin,
+                                     const int* shell_atom, sf_basis** out)
+{
+    if (!mol_atom_begin) {
+        set_error(kEInval, "sf_basis_create_batch: mol_atom_begin is required");
+        return kEInval;
+    }
+    return create_basis(nmol, mol_shell_begin, nshell, L, nprim, center, expns, coefs,
+                        mol_atom_begin, natom, shell_atom, out);
+}
+
+extern "C" int sf_basis_set_layout(sf_basis* b, const long long* block_off, const int* block_ld)
+{
+    if (!b || !block_off || !block_ld) {
+        set_error(kEInval, "sf_basis_set_layout: null argument");
+        return kEInval;
+    }
+    std::vector<std::pair<long long, long long>> span;
+    for (int m = 0; m < b->nmol; ++m) {
+        if (block_off[m] < 0 || block_ld[m] < b->mol_nbf[m]) {
+            set_error(kEInval, "molecule %d: offset %lld, ld %d (needs ld >= %d)", m, block_off[m],
+                      block_ld[m], b->mol_nbf[m]);
+            return kEInval;
+        }
+        span.emplace_back(block_off[m], block_off[m] + (long long)b->mol_nbf[m] * block_ld[m]);
+    }
+    std::sort(span.begin(), span.end());
+    for (size_t k = 1; k < span.size(); ++k)
+        if (span[k].first < span[k - 1].second) {
+            set_error(kEInval, "molecule blocks overlap in the layout");
+            return kEInval;
+        }
+    b->mol_off.assign(block_off, block_off + b->nmol);
+    b->mol_ld.assign(block_ld, block_ld + b->nmol);
+    return apply_layout(b);
+}
+
+extern "C" int sf_basis_layout(const sf_basis* b, int* nmol, int* mol_nbf, long long* block_off,
+                               int* block_ld, long long* extent)
+{
+    if (!b || !nmol) {
+        set_error(kEInval, "sf_basis_layout: null argument");
+        return kEInval;
+    }
+    *nmol = b->nmol;
+    for (int m = 0; m < b->nmol; ++m) {
+        if (mol_nbf) mol_nbf[m] = b->mol_nbf[m];
+        if (block_off) block_off[m] = b->mol_off[m];
+        if (block_ld) block_ld[m] = b->mol_ld[m];
+    }
+    if (extent) *extent = b->extent;
     return kOk;
 }
 
@@ -428,6 +648,10 @@ extern "C" int sf_schwarz(sf_basis* b, double* Q)
         set_error(kEInval, "sf_schwarz: null argument");
         return kEInval;
     }
+    if (b->nmol != 1) {
+        set_error(kEUnsupported, "sf_schwarz: one molecule only (use sf_pair_list for a batch)");
+        return kEUnsupported;
+    }
     SF_TRY(compute_schwarz(b));
     for (size_t k = 0; k < b->pair_slot.size(); ++k) {
         const int s = b->pair_slot[k];
@@ -450,26 +674,35 @@ extern "C" int sf_screen(sf_basis* b, double tau, double tau_mixed)
         return kEInval;
     }
     SF_TRY(compute_schwarz(b));
+    b->drop_graph();
     b->screened = false;
     b->grad_ready = false;
     b->tau = tau;
     b->tau_mixed = tau_mixed;
-    double qmax = 0.0;
+    const int nmol = b->nmol;
+    // largest Schwarz factor of each molecule
+    std::vector<double> qmax(nmol, 0.0);
     for (auto& P : b->pc)
-        for (double q : P.Q) qmax = std::max(qmax, q);
-    // significant pairs, sorted by Q descending (ties: original order)
+        for (int m = 0; m < nmol; ++m)
+            for (int i = P.mbeg[m]; i < P.mbeg[m + 1]; ++i) qmax[m] = std::max(qmax[m], P.Q[i]);
+    // significant pairs of each molecule, sorted by Q descending (ties: original order)
     for (auto& P : b->pc) {
-        std::vector<int> idx;
-        for (int i = 0; i < (int)P.Q.size(); ++i)
-            if (P.Q[i] * qmax >= tau) idx.push_back(i);
-        std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return P.Q[x] > P.Q[y]; });
         P.s_sa.clear(); P.s_sb.clear(); P.s_poff.clear(); P.s_np.clear(); P.s_Q.clear();
-        for (int i : idx) {
-            P.s_sa.push_back(P.sa[i]);
-            P.s_sb.push_back(P.sb[i]);
-            P.s_poff.push_back(P.poff[i]);
-            P.s_np.push_back(P.np[i]);
-            P.s_Q.push_back(P.Q[i]);
+        P.s_mol.clear(); P.s_mbeg.assign(1, 0);
+        for (int m = 0; m < nmol; ++m) {
+            std::vector<int> idx;
+            for (int i = P.mbeg[m]; i < P.mbeg[m + 1]; ++i)
+                if (P.Q[i] * qmax[m] >= tau) idx.push_back(i);
+            std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return P.Q[x] > P.Q[y]; });
+            for (int i : idx) {
+                P.s_sa.push_back(P.sa[i]);
+                P.s_sb.push_back(P.sb[i]);
+                P.s_poff.push_back(P.poff[i]);
+                P.s_np.push_back(P.np[i]);
+                P.s_Q.push_back(P.Q[i]);
+                P.s_mol.push_back(m);
+            }
+            P.s_mbeg.push_back((int)P.s_sa.size());
         }
         SF_TRY(P.ds_sa.upload(P.s_sa));
         SF_TRY(P.ds_sb.upload(P.s_sb));
@@ -500,10 +733,19 @@ extern "C" int sf_screen(sf_basis* b, double tau, double tau_mixed)
             Q.ntile = class_ntile(c);
             Q.n_all.assign(Q.nbra, 0);
             Q.n64.assign(Q.nbra, 0);
+            Q.ket0.assign(Q.nbra, 0);
+            Q.nket.assign(Q.nbra, 0);
+            for (int i = 0; i < Q.nbra; ++i) {
+                const int m = B.s_mol[i];
+                Q.ket0[i] = K.s_mbeg[m];
+                Q.nket[i] = K.s_mbeg[m + 1] - K.s_mbeg[m];
+            }
+            SF_TRY(Q.d_ket0.upload(Q.ket0.empty() ? std::vector<int>(1, 0) : Q.ket0));
+            SF_TRY(Q.d_nket.upload(Q.nket.empty() ? std::vector<int>(1, 0) : Q.nket));
             if (Q.nbra > 0 && !K.s_sa.empty()) {
                 SF_TRY(launch_count_kets(B.ds_Q.as<double>(), Q.nbra, K.ds_Q.as<double>(),
-                                         (int)K.s_sa.size(), Q.same, tau, tau_mixed,
-                                         d_nall.as<int>(), d_n64.as<int>(), 0));
+                                         Q.d_ket0.as<int>(), Q.d_nket.as<int>(), Q.same, tau,
+                                         tau_mixed, d_nall.as<int>(), d_n64.as<int>(), 0));
                 SF_HIP(hipMemcpy(Q.n_all.data(), d_nall.p, Q.nbra * sizeof(int), hipMemcpyDeviceToHost));
                 SF_HIP(hipMemcpy(Q.n64.data(), d_n64.p, Q.nbra * sizeof(int), hipMemcpyDeviceToHost));
             }
@@ -521,7 +763,7 @@ extern "C" int sf_screen(sf_basis* b, double tau, double tau_mixed)
             SF_TRY(Q.d_base_all.upload(Q.base_all));
             SF_TRY(Q.d_base64.upload(Q.base64));
             SF_TRY(Q.d_base32.upload(Q.base32));
-            SF_TRY(Q.d_k032.upload(Q.n64));
+            SF_TRY(Q.d_k032.upload(Q.n64.empty() ? std::vector<int>(1, 0) : Q.n64));
             SF_TRY(Q.d_pos.ensure((size_t)std::max(1LL, Q.nq_all) * 6 * sizeof(unsigned)));
             if (class_split(c))
                 SF_TRY(Q.d_raw.ensure((size_t)std::max(1LL, Q.nq_all) * class_nint(c) * sizeof(double)));
@@ -534,7 +776,8 @@ extern "C" int sf_screen(sf_basis* b, double tau, double tau_mixed)
             for (int i = 0; i < Q.nbra; ++i) {
                 const bool eqab = B.s_sa[i] == B.s_sb[i];
                 const long long pab = eqab ? nA * (nA + 1) / 2 : nA * nB;
-                for (int j = 0; j < Q.n_all[i]; ++j) {
+                for (int jr = 0; jr < Q.n_all[i]; ++jr) {
+                    const int j = Q.ket0[i] + jr;
                     const bool eqcd = K.s_sa[j] == K.s_sb[j];
                     const long long pcd = eqcd ? nC * (nC + 1) / 2 : nC * nD;
                     b->funcq += (Q.same && i == j) ? pab * (pab + 1) / 2 : pab * pcd;
@@ -542,13 +785,20 @@ extern "C" int sf_screen(sf_basis* b, double tau, double tau_mixed)
                 }
             }
         }
-    // digestion plan: (destination, entry) pairs sorted by destination
-    const int ns = b->nshell;
-    const unsigned long long nkeys = 2ULL * ns * ns;
-    if (nkeys >= (1ULL << 32)) {
-        set_error(kEUnsupported, "too many shells for the digestion plan");
-        return kEUnsupported;
+    // digestion plan: (destination, entry) pairs sorted by destination;
+    // destinations are numbered per molecule (KeyMap)
+    const int nmol_ = b->nmol;
+    std::vector<unsigned long long> mkb(nmol_ + 1, 0);
+    for (int m = 0; m < nmol_; ++m) {
+        const unsigned long long ns = b->mol_shell[m + 1] - b->mol_shell[m];
+        mkb[m + 1] = mkb[m] + 2 * ns * ns;
     }
+    const unsigned long long nkeys = mkb[nmol_];
+    KeyMap km;
+    km.sh_mol = b->d_sh_mol.as<int>();
+    km.sh_loc = b->d_sh_loc.as<int>();
+    km.mol_ns = b->d_mol_ns.as<int>();
+    km.mol_kbase = b->d_mol_kbase.as<unsigned>();
     const size_t E = (size_t)std::max(1LL, total_entries);
     DevBuf d_keys, d_vals, d_keys2, d_vals2, d_sort_tmp;
     SF_TRY(d_keys.ensure(E * sizeof(unsigned)));
@@ -563,7 +813,7 @@ extern "C" int sf_screen(sf_basis* b, double tau, double tau_mixed)
         tab[c].qstride = Q.nq_all;
         if (Q.nq_all == 0) continue;
         SF_TRY(launch_plan_entries(c, b->pc[Q.pcb].sorted_dev(), b->pc[Q.pck].sorted_dev(),
-                                   Q.d_base_all.as<long long>(), Q.nq_all, ns,
+                                   Q.d_ket0.as<int>(), Q.d_base_all.as<long long>(), Q.nq_all, km,
                                    d_keys.as<unsigned>() + ebase,
                                    d_vals.as<unsigned long long>() + ebase, 0));
         ebase += 6 * Q.nq_all;
@@ -597,17 +847,25 @@ extern "C" int sf_screen(sf_basis* b, double tau, double tau_mixed)
     // slot layout: destinations in key order, each a run of NE-double tiles
     std::vector<unsigned long long> key_base(nkeys);
     std::vector<int> key_ne(nkeys);
-    std::vector<int> dest[9];
+    std::vector<int> dkey[9], ds1[9], ds2[9], dxy[9];
     unsigned long long acc = 0;
-    const unsigned long long nsq = (unsigned long long)ns * ns;
-    for (unsigned long long k = 0; k < nkeys; ++k) {
-        const unsigned long long rem = k % nsq;
-        const int s1 = (int)(rem / ns), s2 = (int)(rem % ns);
-        key_ne[k] = cart_n(b->L[s1]) * cart_n(b->L[s2]);
-        key_base[k] = acc;
-        if (sege[k] <= segb[k]) continue;
-        acc += (unsigned long long)(sege[k] - segb[k]) * key_ne[k];
-        dest[3 * b->L[s1] + b->L[s2]].push_back((int)k);
+    for (int m = 0; m < nmol_; ++m) {
+        const int s0 = b->mol_shell[m];
+        const unsigned long long ns = b->mol_shell[m + 1] - s0, n2 = ns * ns;
+        for (unsigned long long k = mkb[m]; k < mkb[m + 1]; ++k) {
+            const unsigned long long r = k - mkb[m];
+            const int xy = (int)(r / n2);
+            const int s1 = s0 + (int)((r % n2) / ns), s2 = s0 + (int)(r % ns);
+            key_ne[k] = cart_n(b->L[s1]) * cart_n(b->L[s2]);
+            key_base[k] = acc;
+            if (sege[k] <= segb[k]) continue;
+            acc += (unsigned long long)(sege[k] - segb[k]) * key_ne[k];
+            const int g = 3 * b->L[s1] + b->L[s2];
+            dkey[g].push_back((int)k);
+            ds1[g].push_back(s1);
+            ds2[g].push_back(s2);
+            dxy[g].push_back(xy);
+        }
     }
     if (acc >= (1ULL << 32)) {
         set_error(kEUnsupported, "digestion plan of %llu doubles exceeds 32-bit slots", acc);
@@ -626,13 +884,12 @@ extern "C" int sf_screen(sf_basis* b, double tau, double tau_mixed)
     d_vals2.release();
     SF_TRY(b->d_tiles.ensure(std::max<size_t>(1, acc) * sizeof(double)));
     for (int g = 0; g < 9; ++g) {
-        b->ndest[g] = (int)dest[g].size();
-        SF_TRY(b->d_dest[g].upload(dest[g]));
+        b->ndest[g] = (int)dkey[g].size();
+        SF_TRY(b->d_dkey[g].upload(dkey[g]));
+        SF_TRY(b->d_ds1[g].upload(ds1[g]));
+        SF_TRY(b->d_ds2[g].upload(ds2[g]));
+        SF_TRY(b->d_dxy[g].upload(dxy[g]));
     }
-    const size_t nn = (size_t)b->nbf * b->nbf;
-    SF_TRY(b->d_Ps.ensure(nn * sizeof(double)));
-    SF_TRY(b->d_X.ensure(nn * sizeof(double)));
-    SF_TRY(b->d_Y.ensure(nn * sizeof(double)));
     SF_HIP(hipDeviceSynchronize());
     b->screened = true;
     return kOk;
@@ -659,8 +916,7 @@ extern "C" int sf_quartet_counts(const sf_basis* b, long long* counts, long long
     return kOk;
 }
 
-extern "C" int sf_pair_list(const sf_basis* b, int pc, int* n, int* sa, int* sb, double* Q,
-                            int* nprim)
+extern "C" int sf_pair_list(const sf_basis* b, int pc, int* n, int* sa, int* sb, double* Q, int* nprim)
 {
     if (!b || pc < 0 || pc >= kNumPairClass || !n) {
         set_error(kEInval, "sf_pair_list: bad arguments");
@@ -708,20 +964,42 @@ static ClassParams class_params(sf_basis* b, int c, const double* Ps)
     cp.ket = b->pc[Q.pck].sorted_dev();
     cp.prim = b->d_prim.as<PrimPair>();
     cp.shell_xyz = b->d_xyz.as<double>();
-    cp.shell_foff = b->d_foff.as<int>();
+    cp.map = b->map();
     cp.seg_base = Q.d_base_all.as<long long>();
-    cp.seg_k0 = b->d_zero.as<int>();
+    cp.seg_k0 = nullptr;  // set per segment
+    cp.ket0 = Q.d_ket0.as<int>();
     cp.qall_base = Q.d_base_all.as<long long>();
     cp.nq = Q.nq_all;
     cp.qstride = Q.nq_all;
     cp.same = Q.same;
-    cp.nbf = b->nbf;
     cp.P = Ps;
     cp.out = nullptr;
     cp.raw = Q.d_raw.as<double>();
     cp.pos = Q.d_pos.as<unsigned>();
     cp.tiles = b->d_tiles.as<double>();
     return cp;
+}
+
+// Segment `seg` of quartet class c: 0 = all quartets (precision 0) or the
+// FP64 part (precision 1), 1 = the FP32 part.  Zero-ket offsets come from a
+// per-class zero array (d_k032 holds n64, the FP32 segment's first ket).
+static void set_segment(sf_basis* b, int c, int precision, int seg, ClassParams& cp,
+                        const int* zeros)
+{
+    QuartetClassData& Q = b->qc[c];
+    if (precision == 0) {
+        cp.seg_base = Q.d_base_all.as<long long>();
+        cp.seg_k0 = zeros;
+        cp.nq = Q.nq_all;
+    } else if (seg == 0) {
+        cp.seg_base = Q.d_base64.as<long long>();
+        cp.seg_k0 = zeros;
+        cp.nq = Q.nq64;
+    } else {
+        cp.seg_base = Q.d_base32.as<long long>();
+        cp.seg_k0 = Q.d_k032.as<int>();
+        cp.nq = Q.nq32;
+    }
 }
 
 extern "C" int sf_eri_dense(sf_basis* b, double* eri)
@@ -733,6 +1011,10 @@ extern "C" int sf_eri_dense(sf_basis* b, double* eri)
     if (!b->screened) {
         set_error(kEState, "sf_eri_dense: call sf_screen first");
         return kEState;
+    }
+    if (b->nmol != 1) {
+        set_error(kEUnsupported, "sf_eri_dense: one molecule only");
+        return kEUnsupported;
     }
     const long long n = b->nbf;
     if (n > 256) {
@@ -747,8 +1029,9 @@ extern "C" int sf_eri_dense(sf_basis* b, double* eri)
         DevBuf d_dense;
         SF_TRY(d_dense.ensure((size_t)Q.nq_all * nint * sizeof(double)));
         ClassParams cp = class_params(b, c, nullptr);
+        set_segment(b, c, 0, 0, cp, b->d_zero.as<int>());
         cp.out = d_dense.as<double>();
-        SF_TRY(launch_eri_class(c, 0, kModeRaw, cp, 0));
+        SF_TRY(launch_eri_class(c, 0, kModeRaw, cp, Tune(), 0));
         std::vector<double> raw((size_t)Q.nq_all * nint);
         SF_HIP(hipMemcpy(raw.data(), d_dense.p, raw.size() * sizeof(double), hipMemcpyDeviceToHost));
         const PairClassData& B = b->pc[Q.pcb];
@@ -779,12 +1062,7 @@ extern "C" int sf_eri_dense(sf_basis* b, double* eri)
     return kOk;
 }
 
-// J/K driver; when ms is non-null, HIP events bracket every launch group on
-// the stream and ms receives milliseconds per group: [2c] FP64 and [2c+1]
-// FP32 ERI+digestion kernels of quartet class c (c = 0..20), [42] density
-// scaling, [43] plan reduction + finalisation.
-static int run_jk(sf_basis* b, const double* P, double* J, double* K, int precision,
-                  hipStream_t s, double* ms)
+static int check_jk_args(sf_basis* b, const double* P, double* J, double* K, int precision)
 {
     if (!b || !P || !J || !K || precision < 0 || precision > 1) {
         set_error(kEInval, "sf_jk: bad arguments");
@@ -794,7 +1072,20 @@ static int run_jk(sf_basis* b, const double* P, double* J, double* K, int precis
         set_error(kEState, "sf_jk: call sf_screen first");
         return kEState;
     }
-    const int nbf = b->nbf;
+    return kOk;
+}
+
+// J/K driver (kernel launches only: no allocation, no synchronisation, no
+// memset, so a stream capture of it is a valid graph); when ms is non-null,
+// HIP events bracket every launch group on the stream and ms receives
+// milliseconds per group: [2c] FP64 and [2c+1] FP32 ERI+digestion kernels
+// of quartet class c (c = 0..20), [42] density scaling, [43] plan
+// reduction + finalisation.
+static int run_jk(sf_basis* b, const double* P, double* J, double* K, int precision,
+                  hipStream_t s, double* ms)
+{
+    SF_TRY(check_jk_args(b, P, J, K, precision));
+    const MolLayout ml = b->layout();
     std::vector<hipEvent_t> ev;
     auto mark = [&]() -> int {
         if (!ms) return kOk;
@@ -806,44 +1097,37 @@ static int run_jk(sf_basis* b, const double* P, double* J, double* K, int precis
     };
     std::vector<int> slot;  // ms index of the interval ending at each mark
     SF_TRY(mark());
-    SF_TRY(launch_scale_density(P, b->d_fnorm.as<double>(), nbf, b->d_Ps.as<double>(), s));
+    SF_TRY(launch_scale_density(P, b->d_fnorm.as<double>(), ml, b->d_Ps.as<double>(), s));
     SF_TRY(mark());
     slot.push_back(42);
     for (int c = 0; c < kNumQuartetClass; ++c) {
         QuartetClassData& Q = b->qc[c];
         if (Q.nq_all == 0) continue;
         ClassParams cp = class_params(b, c, b->d_Ps.as<double>());
-        if (precision == 0) {
-            SF_TRY(launch_eri_class(c, 0, kModeJK, cp, s));
+        for (int seg = 0; seg <= precision; ++seg) {
+            set_segment(b, c, precision, seg, cp, b->d_zero.as<int>());
+            SF_TRY(launch_eri_class(c, seg, kModeJK, cp, b->tune[c][seg], s));
             SF_TRY(mark());
-            slot.push_back(2 * c);
-        } else {
-            cp.seg_base = Q.d_base64.as<long long>();
-            cp.seg_k0 = b->d_zero.as<int>();
-            cp.nq = Q.nq64;
-            SF_TRY(launch_eri_class(c, 0, kModeJK, cp, s));
-            SF_TRY(mark());
-            slot.push_back(2 * c);
-            cp.seg_base = Q.d_base32.as<long long>();
-            cp.seg_k0 = Q.d_k032.as<int>();
-            cp.nq = Q.nq32;
-            SF_TRY(launch_eri_class(c, 1, kModeJK, cp, s));
-            SF_TRY(mark());
-            slot.push_back(2 * c + 1);
+            slot.push_back(2 * c + seg);
         }
     }
-    const long long nn = (long long)nbf * nbf;
-    SF_TRY(launch_fill(b->d_X.as<double>(), nn, 0.0, s));
-    SF_TRY(launch_fill(b->d_Y.as<double>(), nn, 0.0, s));
+    SF_TRY(launch_fill(b->d_X.as<double>(), b->extent, 0.0, s));
+    SF_TRY(launch_fill(b->d_Y.as<double>(), b->extent, 0.0, s));
     const int nf[3] = {1, 3, 6};
-    for (int g = 0; g < 9; ++g)
-        SF_TRY(launch_reduce(nf[g / 3], nf[g % 3], b->d_dest[g].as<int>(), b->ndest[g],
-                             b->d_segb.as<long long>(), b->d_sege.as<long long>(),
-                             b->d_key_base.as<unsigned long long>(), b->d_tiles.as<double>(),
-                             b->d_foff.as<int>(), b->nshell, nbf, b->d_X.as<double>(),
+    for (int g = 0; g < 9; ++g) {
+        DestList dl;
+        dl.key = b->d_dkey[g].as<int>();
+        dl.s1 = b->d_ds1[g].as<int>();
+        dl.s2 = b->d_ds2[g].as<int>();
+        dl.xy = b->d_dxy[g].as<int>();
+        dl.n = b->ndest[g];
+        SF_TRY(launch_reduce(nf[g / 3], nf[g % 3], dl, b->d_segb.as<long long>(),
+                             b->d_sege.as<long long>(), b->d_key_base.as<unsigned long long>(),
+                             b->d_tiles.as<double>(), b->map(), b->d_X.as<double>(),
                              b->d_Y.as<double>(), s));
-    SF_TRY(launch_finalize(b->d_X.as<double>(), b->d_Y.as<double>(), b->d_fnorm.as<double>(), nbf,
-                           J, K, s));
+    }
+    SF_TRY(launch_finalize(b->d_X.as<double>(), b->d_Y.as<double>(), b->d_fnorm.as<double>(), ml, J,
+                           K, s));
     SF_TRY(mark());
     slot.push_back(43);
     if (ms) {
@@ -875,6 +1159,151 @@ extern "C" int sf_jk_profile(sf_basis* b, const double* P, double* J, double* K,
     return run_jk(b, P, J, K, precision, (hipStream_t)stream, ms);
 }
 
+// ------------------------------------------------------------ graphs ----
+// The whole J/K build (density scaling, every class kernel, plan
+// reduction, finalisation) captured once into a hipGraph and replayed.
+// The capture runs on a private stream (capture needs a non-default
+// stream); the replay goes on the caller's stream.  A new capture is made
+// when P/J/K/precision change and after sf_screen, sf_autotune,
+// sf_set_tuning or sf_basis_set_layout.
+extern "C" int sf_jk_graph(sf_basis* b, const double* P, double* J, double* K, int precision,
+                           void* stream)
+{
+    SF_TRY(check_jk_args(b, P, J, K, precision));
+    JKGraph& g = b->graph;
+    if (!g.exec || g.P != P || g.J != J || g.K != K || g.precision != precision) {
+        b->drop_graph();
+        if (!b->cap_stream) SF_HIP(hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking));
+        SF_HIP(hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal));
+        const int rc = run_jk(b, P, J, K, precision, b->cap_stream, nullptr);
+        hipGraph_t graph = nullptr;
+        const hipError_t e = hipStreamEndCapture(b->cap_stream, &graph);
+        if (rc != kOk) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return rc;
+        }
+        SF_HIP(e);
+        hipGraphExec_t exec = nullptr;
+        const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        SF_HIP(ei);
+        g.exec = exec;
+        g.P = P;
+        g.J = J;
+        g.K = K;
+        g.precision = precision;
+    }
+    SF_HIP(hipGraphLaunch(g.exec, (hipStream_t)stream));
+    return kOk;
+}
+
+// ---------------------------------------------------------- autotuner ----
+// Launch shapes tried per quartet class and precision segment: 64/128/256
+// threads per block, each with one thread (or wavefront) per work item
+// (grid 0) or a persistent grid of 2 or 8 blocks per compute unit.  The
+// class kernel runs once untimed (caches, code load), then each candidate
+// once between HIP events on the stream; the fastest wins.
+extern "C" int sf_autotune(sf_basis* b, const double* P, int precision, void* stream, int* shapes,
+                           double* ms)
+{
+    if (!b || !P || precision < 0 || precision > 1) {
+        set_error(kEInval, "sf_autotune: bad arguments");
+        return kEInval;
+    }
+    if (!b->screened) {
+        set_error(kEState, "sf_autotune: call sf_screen first");
+        return kEState;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    int dev = 0, ncu = 0;
+    SF_HIP(hipGetDevice(&dev));
+    SF_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    if (ncu <= 0) ncu = 1;
+    b->drop_graph();
+    SF_TRY(launch_scale_density(P, b->d_fnorm.as<double>(), b->layout(), b->d_Ps.as<double>(), s));
+    hipEvent_t e0, e1;
+    SF_HIP(hipEventCreate(&e0));
+    SF_HIP(hipEventCreate(&e1));
+    int rc = kOk;
+    for (int c = 0; c < kNumQuartetClass && rc == kOk; ++c) {
+        QuartetClassData& Q = b->qc[c];
+        ClassParams cp = class_params(b, c, b->d_Ps.as<double>());
+        for (int seg = 0; seg <= precision && rc == kOk; ++seg) {
+            set_segment(b, c, precision, seg, cp, b->d_zero.as<int>());
+            if (ms) ms[2 * c + seg] = 0.0;
+            if (Q.nq_all == 0 || cp.nq == 0) continue;
+            // work items of one launch: quartets, or waves x component slices
+            const long long items = class_split(c) ? ((cp.nq + 63) / 64) * 64 * class_nslice(c) : cp.nq;
+            Tune best = b->tune[c][seg];
+            float best_ms = 1e30f;
+            const int blocks[3] = {64, 128, 256};
+            const int mults[3] = {0, 2, 8};
+            rc = launch_eri_class(c, seg, kModeJK, cp, Tune(), s);  // untimed warm-up
+            for (int bi = 0; bi < 3 && rc == kOk; ++bi)
+                for (int mi = 0; mi < 3 && rc == kOk; ++mi) {
+                    Tune t;
+                    t.block = blocks[bi];
+                    t.grid = mults[mi] * ncu;
+                    const long long need = (items + t.block - 1) / t.block;
+                    if (t.grid > 0 && t.grid >= need) continue;  // same as grid 0
+                    if (hipEventRecord(e0, s) != hipSuccess) {
+                        set_error(kEHip, "sf_autotune: event record failed");
+                        rc = kEHip;
+                        break;
+                    }
+                    rc = launch_eri_class(c, seg, kModeJK, cp, t, s);
+                    if (rc) break;
+                    float el = 0.f;
+                    if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                        hipEventElapsedTime(&el, e0, e1) != hipSuccess) {
+                        set_error(kEHip, "sf_autotune: event timing failed");
+                        rc = kEHip;
+                        break;
+                    }
+                    if (el < best_ms) {
+                        best_ms = el;
+                        best = t;
+                    }
+                }
+            if (rc) break;
+            b->tune[c][seg] = best;
+            if (ms) ms[2 * c + seg] = best_ms;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    if (shapes)
+        for (int c = 0; c < kNumQuartetClass; ++c)
+            for (int seg = 0; seg < 2; ++seg) {
+                shapes[4 * c + 2 * seg] = b->tune[c][seg].block;
+                shapes[4 * c + 2 * seg + 1] = b->tune[c][seg].grid;
+            }
+    return kOk;
+}
+
+extern "C" int sf_set_tuning(sf_basis* b, const int* shapes)
+{
+    if (!b || !shapes) {
+        set_error(kEInval, "sf_set_tuning: null argument");
+        return kEInval;
+    }
+    for (int k = 0; k < 2 * kNumQuartetClass; ++k) {
+        const int block = shapes[2 * k], grid = shapes[2 * k + 1];
+        if (block < 64 || block > 256 || block % 64 != 0 || grid < 0) {
+            set_error(kEInval, "sf_set_tuning: entry %d: block %d grid %d", k, block, grid);
+            return kEInval;
+        }
+    }
+    for (int c = 0; c < kNumQuartetClass; ++c)
+        for (int seg = 0; seg < 2; ++seg) {
+            b->tune[c][seg].block = shapes[4 * c + 2 * seg];
+            b->tune[c][seg].grid = shapes[4 * c + 2 * seg + 1];
+        }
+    b->drop_graph();
+    return kOk;
+}
+
 extern "C" int sf_basis_set_nuclei(sf_basis* b, int natom, const double* Z, const double* xyz)
 {
     if (!b || natom < 0 || (natom > 0 && (!Z || !xyz))) {
@@ -893,13 +1322,19 @@ extern "C" int sf_one_electron(sf_basis* b, double* S, double* T, double* V, voi
         set_error(kEInval, "sf_one_electron: null argument");
         return kEInval;
     }
+    if (b->nnuc < b->mol_atom[b->nmol]) {
+        set_error(kEState, "sf_one_electron: %d nuclei set, the molecules need %d "
+                  "(sf_basis_set_nuclei)", b->nnuc, b->mol_atom[b->nmol]);
+        return kEState;
+    }
     for (int p = 0; p < kNumPairClass; ++p) {
         PairClassData& P = b->pc[p];
         SF_TRY(launch_one_electron(p, P.d_sa.as<int>(), P.d_sb.as<int>(), P.d_poff.as<int>(),
                                    P.d_np.as<int>(), (int)P.sa.size(), b->d_prim.as<PrimPair>(),
                                    b->d_xyz.as<double>(), b->d_foff.as<int>(),
-                                   b->d_fnorm.as<double>(), b->nnuc, b->d_Z.as<double>(),
-                                   b->d_nuc.as<double>(), b->nbf, S, T, V, (hipStream_t)stream));
+                                   b->d_fnorm.as<double>(), b->map(), b->d_sh_mol.as<int>(),
+                                   b->d_mol_nuc.as<int>(), b->d_Z.as<double>(),
+                                   b->d_nuc.as<double>(), S, T, V, (hipStream_t)stream));
     }
     return kOk;
 }
@@ -946,8 +1381,7 @@ static int prepare_grad(sf_basis* b)
     long long npair = 0;
     for (auto& P : b->pc) npair += (long long)P.sa.size();
     SF_TRY(b->d_grows.ensure(std::max<long long>(1, npair) * 3 * std::max(1, natom) * sizeof(double)));
-    const size_t nn = (size_t)b->nbf * b->nbf;
-    SF_TRY(b->d_Ws.ensure(nn * sizeof(double)));
+    SF_TRY(b->d_Ws.ensure((size_t)std::max(1LL, b->extent) * sizeof(double)));
     SF_TRY(b->d_g2.ensure(3 * std::max(1, natom) * sizeof(double)));
     SF_TRY(b->d_gtot.ensure(3 * std::max(1, natom) * sizeof(double)));
     b->grad_ready = true;
@@ -964,19 +1398,25 @@ extern "C" int sf_grad(sf_basis* b, const double* P, const double* W, double* gr
         set_error(kEState, "sf_grad: call sf_screen first");
         return kEState;
     }
+    if (b->nmol != 1) {
+        set_error(kEUnsupported, "sf_grad: one molecule only");
+        return kEUnsupported;
+    }
     if (b->nnuc != b->natom || b->natom <= 0) {
         set_error(kEState, "sf_grad: nuclei (sf_basis_set_nuclei) must match the %d atoms", b->natom);
         return kEState;
     }
     hipStream_t s = (hipStream_t)stream;
     SF_TRY(prepare_grad(b));
-    const int nbf = b->nbf, natom = b->natom;
-    SF_TRY(launch_scale_density(P, b->d_fnorm.as<double>(), nbf, b->d_Ps.as<double>(), s));
-    SF_TRY(launch_scale_density(W, b->d_fnorm.as<double>(), nbf, b->d_Ws.as<double>(), s));
+    const int natom = b->natom;
+    const MolLayout ml = b->layout();
+    SF_TRY(launch_scale_density(P, b->d_fnorm.as<double>(), ml, b->d_Ps.as<double>(), s));
+    SF_TRY(launch_scale_density(W, b->d_fnorm.as<double>(), ml, b->d_Ws.as<double>(), s));
     for (int c = 0; c < kNumQuartetClass; ++c) {
         QuartetClassData& Q = b->qc[c];
         if (Q.nq_all == 0) continue;
         ClassParams cp = class_params(b, c, b->d_Ps.as<double>());
+        set_segment(b, c, 0, 0, cp, b->d_zero.as<int>());
         SF_TRY(launch_grad_class(c, cp, b->d_gout.as<double>(), s));
         SF_TRY(launch_grad_reduce_bra(b->d_gout.as<double>(), class_nslice(c), Q.nq_all,
                                       Q.d_base_all.as<long long>(), Q.nbra,
@@ -994,8 +1434,8 @@ extern "C" int sf_grad(sf_basis* b, const double* P, const double* W, double* gr
         SF_TRY(launch_grad_one_electron(p, Pc.d_sa.as<int>(), Pc.d_sb.as<int>(), Pc.d_poff.as<int>(),
                                         Pc.d_np.as<int>(), (int)Pc.sa.size(),
                                         b->d_prim.as<PrimPair>(), b->d_xyz.as<double>(),
-                                        b->d_foff.as<int>(), b->d_atom.as<int>(), natom,
-                                        b->d_Z.as<double>(), b->d_nuc.as<double>(), nbf,
+                                        b->map(), b->d_atom.as<int>(), natom,
+                                        b->d_Z.as<double>(), b->d_nuc.as<double>(),
                                         b->d_Ps.as<double>(), b->d_Ws.as<double>(),
                                         b->d_grows.as<double>() + row * 3 * natom, s));
         row += (long long)Pc.sa.size();

@@ -49,19 +49,38 @@ struct PairListDev {
     int n;
 };
 
+// Where each shell's functions live in the N x N matrices (P, J, K, S, T, V)
+// of its molecule: component i of shell a and component j of shell b (same
+// molecule) are at row[a] + i * ld[a] + col[b] + j.  With one molecule,
+// row = foff * nbf, col = foff, ld = nbf; a batch of molecules puts each
+// molecule's block wherever the caller's layout says (see sf_basis_create_batch).
+struct ShellMap {
+    const long long* row;
+    const int* col;
+    const int* ld;
+};
+
+// Launch shape of a kernel: `block` threads per block; `grid` = 0 launches
+// one thread (or wavefront) per work item, grid > 0 launches at most that
+// many blocks, which then stride over the work (persistent blocks).
+struct Tune {
+    int block = 256;
+    int grid = 0;
+};
+
 // One launch of a quartet-class kernel over a precision segment.
 struct ClassParams {
     PairListDev bra, ket;
     const PrimPair* prim;
     const double* shell_xyz;   // 3 per shell
-    const int* shell_foff;     // first function of each shell
+    ShellMap map;              // matrix positions of each shell's functions
     const long long* seg_base; // nbra+1: flat segment index of each bra's first quartet
-    const int* seg_k0;         // nbra: first ket position of the segment for each bra
+    const int* seg_k0;         // nbra: first ket of the segment, relative to ket0
+    const int* ket0;           // nbra: first ket of the bra's molecule in the ket list
     const long long* qall_base;// nbra+1: class-wide quartet index of ket 0 of each bra
     long long nq;              // quartets in this segment
     long long qstride;         // quartets in the class (plane stride of out)
     int same;                  // bra and ket pair classes coincide
-    int nbf;
     const double* P;           // density in the shell-normalised basis
     double* out;               // kModeRaw: SoA planes out[k * qstride + q]
     double* raw;               // split classes: raw integral planes (NINT * qstride)
@@ -77,8 +96,42 @@ struct ClassTab {
     long long qstride;
 };
 
+// Per-molecule layout of the N x N matrices: molecule m has n[m] functions
+// (global function indices fbase[m] ...), its block starts at off[m] with
+// leading dimension ld[m] >= n[m]; nmax = max n[m].
+struct MolLayout {
+    const long long* off;
+    const int* ld;
+    const int* n;
+    const int* fbase;
+    int nmol;
+    int nmax;
+};
+
+// Digestion destinations of one tile shape: key (plan segment), the two
+// shells of the destination block and which matrix it belongs to (0 = X
+// for J, 1 = Y for K).
+struct DestList {
+    const int* key;
+    const int* s1;
+    const int* s2;
+    const int* xy;
+    int n;
+};
+
+// Shell -> molecule bookkeeping for plan keys: molecule of each shell, its
+// index inside the molecule, shells per molecule, first plan key of each
+// molecule (keys of molecule m: kbase[m] + xy * ns^2 + s1 * ns + s2).
+struct KeyMap {
+    const int* sh_mol;
+    const int* sh_loc;
+    const int* mol_ns;
+    const unsigned* mol_kbase;
+};
+
 // Launchers (k_eri.hip).
-int launch_eri_class(int qcls, int fp32, int mode, const ClassParams& cp, hipStream_t s);
+int launch_eri_class(int qcls, int fp32, int mode, const ClassParams& cp, const Tune& tn,
+                     hipStream_t s);
 int launch_schwarz(int pc, const int* sa, const int* sb, const int* poff, const int* np, int n,
                    const PrimPair* prim, const double* shell_xyz, double* Q, hipStream_t s);
 int class_ntile(int qcls);
@@ -94,10 +147,9 @@ int launch_grad_reduce_ket(const double* gout, int nslice, long long qstride,
 int launch_grad_atoms(const double* part, const long long* csr_off, const long long* csr_idx,
                       int natom, double* grad, hipStream_t s);
 int launch_grad_one_electron(int pc, const int* sa, const int* sb, const int* poff, const int* np,
-                             int n, const PrimPair* prim, const double* xyz, const int* foff,
+                             int n, const PrimPair* prim, const double* xyz, const ShellMap& map,
                              const int* shell_atom, int natom, const double* Z, const double* Cxyz,
-                             int nbf, const double* Ps, const double* Ws, double* rows,
-                             hipStream_t s);
+                             const double* Ps, const double* Ws, double* rows, hipStream_t s);
 int launch_grad_rows(const double* rows, long long nrows, int ncol, const double* add, double* out,
                      hipStream_t s);
 void class_tile_offsets(int qcls, int off[6]);
@@ -105,28 +157,30 @@ void class_tile_offsets(int qcls, int off[6]);
 // Launchers (k_util.hip).
 int launch_boys(long long n, int mmax, const double* T, double* F, hipStream_t s);
 int launch_rys(long long n, int nroots, const double* X, double* u, double* w, hipStream_t s);
-int launch_count_kets(const double* Qbra, int nbra, const double* Qket, int nket, int same,
-                      double tau, double tau_mixed, int* n_all, int* n_fp64, hipStream_t s);
-int launch_plan_entries(int qcls, const PairListDev& bra, const PairListDev& ket,
-                        const long long* qall_base, long long nq, int nshell,
+int launch_count_kets(const double* Qbra, int nbra, const double* Qket, const int* ket0,
+                      const int* nket, int same, double tau, double tau_mixed, int* n_all,
+                      int* n_fp64, hipStream_t s);
+int launch_plan_entries(int qcls, const PairListDev& bra, const PairListDev& ket, const int* ket0,
+                        const long long* qall_base, long long nq, const KeyMap& km,
                         unsigned int* keys, unsigned long long* vals, hipStream_t s);
 int launch_segments(const unsigned int* keys, long long n, long long* seg_begin,
                     long long* seg_end, hipStream_t s);
 int launch_plan_positions(const unsigned int* keys, const unsigned long long* vals, long long n,
                           const long long* seg_begin, const unsigned long long* key_base,
                           const int* key_ne, const ClassTab* tab, hipStream_t s);
-int launch_reduce(int n1, int n2, const int* dest_keys, int ndest, const long long* seg_begin,
+int launch_reduce(int n1, int n2, const DestList& dl, const long long* seg_begin,
                   const long long* seg_end, const unsigned long long* key_base,
-                  const double* tiles, const int* shell_foff, int nshell, int nbf, double* X,
-                  double* Y, hipStream_t s);
-int launch_scale_density(const double* P, const double* fnorm, int nbf, double* Ps, hipStream_t s);
-int launch_finalize(const double* X, const double* Y, const double* fnorm, int nbf, double* J,
-                    double* K, hipStream_t s);
+                  const double* tiles, const ShellMap& map, double* X, double* Y, hipStream_t s);
+int launch_scale_density(const double* P, const double* fnorm, const MolLayout& ml, double* Ps,
+                         hipStream_t s);
+int launch_finalize(const double* X, const double* Y, const double* fnorm, const MolLayout& ml,
+                    double* J, double* K, hipStream_t s);
 int launch_fill(double* p, long long n, double v, hipStream_t s);
 int launch_one_electron(int pc, const int* sa, const int* sb, const int* poff, const int* np, int n,
                         const PrimPair* prim, const double* xyz, const int* foff,
-                        const double* fnorm, int natom, const double* Z, const double* Cxyz, int nbf,
-                        double* S, double* T, double* V, hipStream_t s);
+                        const double* fnorm, const ShellMap& map, const int* sh_mol,
+                        const int* mol_nuc, const double* Z, const double* Cxyz, double* S,
+                        double* T, double* V, hipStream_t s);
 int launch_eri_components(int nf, const int* npgf, const int* poff, const double* coefs,
                           const double* expns, const double* norms, const int* shell,
                           const double* center, long long nq, const int* quartets, double* out,

@@ -80,6 +80,15 @@ def lib():
     L.sf_one_electron.argtypes = [vp, vp, vp, vp, vp]
     L.sf_grad.argtypes = [vp, vp, vp, _f64p, vp]
     L.sf_jk_profile.argtypes = [vp, vp, vp, vp, ctypes.c_int, vp, _f64p]
+    L.sf_basis_create_batch.argtypes = [ctypes.c_int, _i32p, ctypes.c_int, _i32p, _i32p, _f64p,
+                                        _f64p, _f64p, ctypes.c_int, _i32p, _i32p,
+                                        ctypes.POINTER(vp)]
+    L.sf_basis_set_layout.argtypes = [vp, _i64p, _i32p]
+    L.sf_basis_layout.argtypes = [vp, ctypes.POINTER(ctypes.c_int), vp, vp, vp,
+                                  ctypes.POINTER(ctypes.c_longlong)]
+    L.sf_jk_graph.argtypes = [vp, vp, vp, vp, ctypes.c_int, vp]
+    L.sf_autotune.argtypes = [vp, vp, ctypes.c_int, vp, _i32p, _f64p]
+    L.sf_set_tuning.argtypes = [vp, _i32p]
     _LIB = L
     return L
 
@@ -95,7 +104,8 @@ def exported_symbols() -> list[str]:
             "sf_eri_components", "sf_basis_create", "sf_basis_destroy", "sf_basis_nbf",
             "sf_schwarz", "sf_screen", "sf_quartet_counts", "sf_pair_list", "sf_quartet_list",
             "sf_eri_dense", "sf_jk", "sf_jk_profile", "sf_basis_set_nuclei", "sf_one_electron",
-            "sf_grad"]
+            "sf_grad", "sf_basis_create_batch", "sf_basis_set_layout", "sf_basis_layout",
+            "sf_jk_graph", "sf_autotune", "sf_set_tuning"]
 
 
 def _torch():
@@ -173,29 +183,54 @@ def eri_components(mol, quartets) -> np.ndarray:
 
 
 # ------------------------------------------------------------ the basis --
+def _shell_arrays(mols):
+    """Concatenated shell arrays of a list of molecules (AO order kept)."""
+    L, nprim, center, expns, coefs, atom = [], [], [], [], [], []
+    mol_shell, mol_atom = [0], [0]
+    for mol in mols:
+        shells = mol.shells()
+        for s in shells:
+            L.append(s.L)
+            nprim.append(s.nprim)
+            center.append(np.asarray(s.center, dtype=np.float64))
+            expns.append(np.asarray(s.expns, dtype=np.float64))
+            coefs.append(np.asarray(s.coefs, dtype=np.float64))
+            atom.append(mol_atom[-1] + s.atom)
+        mol_shell.append(mol_shell[-1] + len(shells))
+        mol_atom.append(mol_atom[-1] + mol.natom)
+    cat = lambda xs: np.ascontiguousarray(np.concatenate(xs), dtype=np.float64)  # noqa: E731
+    return (np.array(L, dtype=np.int32), np.array(nprim, dtype=np.int32), cat(center), cat(expns),
+            cat(coefs), np.array(atom, dtype=np.int32), np.array(mol_shell, dtype=np.int32),
+            np.array(mol_atom, dtype=np.int32))
+
+
 class Basis:
-    """A shell basis on the GPU: Schwarz screening, quartet lists, J/K."""
+    """A shell basis on the GPU: Schwarz screening, quartet lists, J/K.
+
+    Matrices (P, J, K, S, T, V) are float64 CUDA tensors of shape
+    ``self.shape``: (nbf, nbf) here, (nmol, nmax, nmax) for a BatchBasis.
+    """
 
     def __init__(self, mol):
-        shells = mol.shells()
         self.mol = mol
-        self.nshell = len(shells)
-        L = np.array([s.L for s in shells], dtype=np.int32)
-        nprim = np.array([s.nprim for s in shells], dtype=np.int32)
-        center = np.ascontiguousarray(np.concatenate([s.center for s in shells]), dtype=np.float64)
-        expns = np.ascontiguousarray(np.concatenate([s.expns for s in shells]), dtype=np.float64)
-        coefs = np.ascontiguousarray(np.concatenate([s.coefs for s in shells]), dtype=np.float64)
-        atom = np.array([s.atom for s in shells], dtype=np.int32)
+        self.mols = [mol]
+        L, nprim, center, expns, coefs, atom, _, _ = _shell_arrays([mol])
+        self.nshell = len(L)
         h = ctypes.c_void_p()
         check(lib().sf_basis_create(self.nshell, L, nprim, center, expns, coefs, mol.natom, atom,
                                     ctypes.byref(h)))
         self._h = h
+        self._finish([mol])
+
+    def _finish(self, mols):
         n = ctypes.c_int()
         check(lib().sf_basis_nbf(self._h, ctypes.byref(n)))
         self.nbf = n.value
-        check(lib().sf_basis_set_nuclei(self._h, mol.natom,
-                                        np.ascontiguousarray(mol.numbers, dtype=np.float64),
-                                        np.ascontiguousarray(mol.xyz__, dtype=np.float64).reshape(-1)))
+        Z = np.ascontiguousarray(np.concatenate([m.numbers for m in mols]), dtype=np.float64)
+        xyz = np.ascontiguousarray(np.concatenate([np.asarray(m.xyz__, dtype=np.float64).reshape(-1, 3)
+                                                   for m in mols]).reshape(-1))
+        check(lib().sf_basis_set_nuclei(self._h, len(Z), Z, xyz))
+        self.shape = (self.nbf, self.nbf)
         self.tau = None
         self.tau_mixed = None
 
@@ -210,12 +245,22 @@ class Basis:
         except Exception:
             pass
 
-    def one_electron(self):
-        """S, T, V (float64 CUDA tensors, N x N)."""
+    def _zeros(self):
         torch = _torch()
-        S = torch.empty((self.nbf, self.nbf), dtype=torch.float64, device="cuda")
-        T = torch.empty_like(S)
-        V = torch.empty_like(S)
+        return torch.zeros(self.shape, dtype=torch.float64, device="cuda")
+
+    def _check_matrix(self, P, name="P"):
+        torch = _torch()
+        if P.dtype != torch.float64 or tuple(P.shape) != tuple(self.shape) or not P.is_cuda:
+            raise EngineError(1, f"{name} must be a float64 CUDA tensor of shape {self.shape}")
+        return P.contiguous()
+
+    def one_electron(self):
+        """S, T, V (float64 CUDA tensors of shape ``self.shape``)."""
+        torch = _torch()
+        S = self._zeros()
+        T = self._zeros()
+        V = self._zeros()
         check(lib().sf_one_electron(self._h, ctypes.c_void_p(S.data_ptr()),
                                     ctypes.c_void_p(T.data_ptr()), ctypes.c_void_p(V.data_ptr()),
                                     _stream_ptr(torch)))
@@ -284,29 +329,92 @@ class Basis:
         return out.reshape(n, n, n, n)
 
     def jk(self, P, precision: int = 0, out=None):
-        """J, K for the density P (float64 CUDA tensor, N x N)."""
+        """J, K for the density P (float64 CUDA tensor of shape ``self.shape``)."""
         torch = _torch()
-        P = P.contiguous()
-        if P.dtype != torch.float64 or P.shape != (self.nbf, self.nbf) or not P.is_cuda:
-            raise EngineError(1, "P must be a float64 CUDA tensor of shape (nbf, nbf)")
-        if out is None:
-            J = torch.empty_like(P)
-            K = torch.empty_like(P)
-        else:
-            J, K = out
+        P = self._check_matrix(P)
+        J, K = (self._zeros(), self._zeros()) if out is None else out
         check(lib().sf_jk(self._h, ctypes.c_void_p(P.data_ptr()), ctypes.c_void_p(J.data_ptr()),
                           ctypes.c_void_p(K.data_ptr()), int(precision), _stream_ptr(torch)))
         return J, K
+
+    def jk_graph(self, P, out, precision: int = 0):
+        """jk() replayed from a captured hipGraph (sf_jk_graph).  P and the
+        output pair ``out = (J, K)`` must stay the same tensors between calls
+        for the capture to be reused."""
+        torch = _torch()
+        if not P.is_contiguous():
+            raise EngineError(1, "P must be contiguous (the graph keeps its address)")
+        self._check_matrix(P)
+        J, K = out
+        check(lib().sf_jk_graph(self._h, ctypes.c_void_p(P.data_ptr()),
+                                ctypes.c_void_p(J.data_ptr()), ctypes.c_void_p(K.data_ptr()),
+                                int(precision), _stream_ptr(torch)))
+        return J, K
+
+    def autotune(self, P, precision: int = 0):
+        """Pick launch shapes per quartet class (sf_autotune).  Returns
+        (shapes[21, 2, 2] as (block, grid), ms[21, 2])."""
+        torch = _torch()
+        P = self._check_matrix(P)
+        shapes = np.zeros(4 * NUM_QUARTET_CLASSES, dtype=np.int32)
+        ms = np.zeros(2 * NUM_QUARTET_CLASSES)
+        check(lib().sf_autotune(self._h, ctypes.c_void_p(P.data_ptr()), int(precision),
+                                _stream_ptr(torch), shapes, ms))
+        torch.cuda.synchronize()
+        return shapes.reshape(NUM_QUARTET_CLASSES, 2, 2), ms.reshape(NUM_QUARTET_CLASSES, 2)
+
+    def set_tuning(self, shapes):
+        check(lib().sf_set_tuning(self._h, np.ascontiguousarray(shapes, dtype=np.int32).reshape(-1)))
 
     def jk_profile(self, P, precision: int = 0, out=None):
         """jk() plus per-launch-group milliseconds (HIP events on the stream):
         returns J, K, ms[44] (see sf_jk_profile in include/sfeng.h)."""
         torch = _torch()
-        P = P.contiguous()
-        J = torch.empty_like(P) if out is None else out[0]
-        K = torch.empty_like(P) if out is None else out[1]
+        P = self._check_matrix(P)
+        J, K = (self._zeros(), self._zeros()) if out is None else out
         ms = np.zeros(44)
         check(lib().sf_jk_profile(self._h, ctypes.c_void_p(P.data_ptr()),
                                   ctypes.c_void_p(J.data_ptr()), ctypes.c_void_p(K.data_ptr()),
                                   int(precision), _stream_ptr(torch), ms))
         return J, K, ms
+
+
+class BatchBasis(Basis):
+    """Many independent molecules in one handle (sf_basis_create_batch).
+
+    Matrices use a padded layout: tensors of shape (nmol, nmax, nmax),
+    molecule m's n_m x n_m block in the top-left corner of slice m; the
+    padding is never read or written by the library.
+    """
+
+    def __init__(self, mols):
+        self.mols = list(mols)
+        self.mol = None
+        L, nprim, center, expns, coefs, atom, mol_shell, mol_atom = _shell_arrays(self.mols)
+        self.nshell = len(L)
+        h = ctypes.c_void_p()
+        check(lib().sf_basis_create_batch(len(self.mols), mol_shell, self.nshell, L, nprim, center,
+                                          expns, coefs, int(mol_atom[-1]), mol_atom, atom,
+                                          ctypes.byref(h)))
+        self._h = h
+        self._finish(self.mols)
+        nmol = ctypes.c_int()
+        nbf_mol = np.zeros(len(self.mols), dtype=np.int32)
+        check(lib().sf_basis_layout(self._h, ctypes.byref(nmol), nbf_mol.ctypes.data_as(ctypes.c_void_p),
+                                    None, None, ctypes.byref(ctypes.c_longlong())))
+        self.nmol = nmol.value
+        self.nbf_mol = nbf_mol
+        self.nmax = int(nbf_mol.max())
+        off = np.arange(self.nmol, dtype=np.int64) * self.nmax * self.nmax
+        ld = np.full(self.nmol, self.nmax, dtype=np.int32)
+        check(lib().sf_basis_set_layout(self._h, off, ld))
+        self.shape = (self.nmol, self.nmax, self.nmax)
+
+    def gradient(self, P, W):
+        raise EngineError(5, "gradients are computed per molecule (Basis)")
+
+    def eri_dense(self):
+        raise EngineError(5, "the dense ERI tensor is computed per molecule (Basis)")
+
+    def schwarz(self):
+        raise EngineError(5, "Schwarz factors of a batch: use pair_list()")

@@ -144,3 +144,126 @@ def scf_diis(mol, tol=1e-8, maxiter=100, *, tau=1e-12, precision=0, tau_mixed=1e
              verbose=False) -> SCFResult:
     """SCF with Pulay DIIS (hf.py:236-353)."""
     return _run(mol, "diis", tol, maxiter, tau, precision, tau_mixed, basis, verbose)
+
+
+@dataclass
+class BatchSCFResult:
+    energies: list           # total energy per molecule, Eh
+    converged: list          # per molecule
+    iterations: list         # iteration at which each molecule converged (or stopped)
+    wall_s: float            # whole batch, setup included
+    scf_s: float             # SCF iterations only
+    jk_s: float              # J/K builds (graph replays), summed
+    n_iter: int              # iterations run for the batch
+    tuning: object = None    # autotuned launch shapes (or None)
+
+
+def scf_batch(mols, tol=1e-8, maxiter=100, *, tau=1e-12, precision=0, tau_mixed=1e-5,
+              autotune=True, graph=True, basis=None) -> BatchSCFResult:
+    """Pulay-DIIS SCF (hf.py:236-353) of many independent molecules at once.
+
+    Each molecule follows the reference algorithm on its own -- core guess,
+    S^-1/2, DIIS after two plain iterations with at most 7 vectors and the
+    undamped fall-back when its Pulay system is singular, stop when its
+    |dE| <= tol -- but all molecules advance together: their matrices are
+    padded to (nmol, nmax, nmax), eigh/bmm/solve run batched, J and K for
+    every molecule come from one persistent launch per quartet class of a
+    single BatchBasis, replayed from a captured hipGraph every iteration.
+    A converged molecule is frozen (its density, Fock matrix and energy stop
+    changing) while the others continue.
+    """
+    import torch
+
+    t0 = time.perf_counter()
+    b = basis if basis is not None else engine.BatchBasis(mols).screen(tau, tau_mixed)
+    B, n = b.nmol, b.nmax
+    dev = torch.device("cuda")
+    nbf = torch.tensor(b.nbf_mol, device=dev)
+    idx = torch.arange(n, device=dev)
+    real = idx[None, :] < nbf[:, None]                          # (B, n) real functions
+    mask2 = (real[:, :, None] & real[:, None, :]).double()      # (B, n, n)
+    pad = (~real).double()
+    nocc = torch.tensor([int(sum(int(z) for z in m.numbers)) // 2 for m in mols], device=dev)
+    occ = (idx[None, :] < nocc[:, None]).double()               # (B, n)
+    e_nuc = torch.tensor([m.e_nuc for m in mols], dtype=torch.float64, device=dev)
+
+    S, T, V = b.one_electron()
+    H = T + V
+    # S^-1/2 per molecule: identity on the padding keeps S invertible, the
+    # mask removes it again
+    lam, U = torch.linalg.eigh(S + torch.diag_embed(pad))
+    X = (U * (1.0 / torch.sqrt(lam))[:, None, :]) @ U.transpose(1, 2)
+    X = (X * mask2).contiguous()
+    shift = torch.diag_embed(pad * 1e6)                          # padded orbitals last
+    P_buf = torch.zeros((B, n, n), dtype=torch.float64, device=dev)
+    J_buf = torch.zeros_like(P_buf)
+    K_buf = torch.zeros_like(P_buf)
+    tuning = None
+    if autotune:
+        tuning = b.autotune(P_buf, precision)
+    F = H.clone()
+    P = torch.zeros_like(P_buf)
+    E = torch.zeros(B, dtype=torch.float64, device=dev)
+    E_old = torch.full((B,), float("inf"), dtype=torch.float64, device=dev)
+    done = torch.zeros(B, dtype=torch.bool, device=dev)
+    done_iter = torch.full((B,), -1, dtype=torch.long, device=dev)
+    Fs, errs = [], []
+    jk_s = 0.0
+    counter = 0
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    while True:
+        eps, Cp = torch.linalg.eigh(X @ F @ X + shift)
+        C = X @ Cp
+        P_new = 2.0 * (C * occ[:, None, :]) @ C.transpose(1, 2)
+        keep = done[:, None, None]
+        P = torch.where(keep, P, P_new)
+        P_buf.copy_(P)
+        tj = time.perf_counter()
+        if graph:
+            b.jk_graph(P_buf, (J_buf, K_buf), precision)
+        else:
+            b.jk(P_buf, precision, out=(J_buf, K_buf))
+        torch.cuda.synchronize()
+        jk_s += time.perf_counter() - tj
+        F_new = H + J_buf - 0.5 * K_buf
+        E_new = 0.5 * torch.sum(P * (H + F_new), dim=(1, 2))
+        F = torch.where(keep, F, F_new)
+        E = torch.where(done, E, E_new)
+        newly = (~done) & (torch.abs(E - E_old) <= tol)
+        done_iter = torch.where(newly, torch.full_like(done_iter, counter), done_iter)
+        done = done | newly
+        if bool(done.all()):
+            break
+        counter += 1
+        E_old = E.clone()
+        if counter > maxiter:
+            break
+        if counter <= 2:
+            continue
+        if len(Fs) > 6:
+            Fs.pop(0)
+            errs.pop(0)
+        errs.append(X @ (F @ P @ S - S @ P @ F) @ X)
+        Fs.append(F)
+        k = len(Fs)
+        Bm = torch.zeros((B, k + 1, k + 1), dtype=torch.float64, device=dev)
+        Bm[:, -1, :] = -1.0
+        Bm[:, :, -1] = -1.0
+        Bm[:, -1, -1] = 0.0
+        for i in range(k):
+            for j in range(k):
+                Bm[:, i, j] += torch.sum(errs[i] * errs[j], dim=(1, 2))
+        rhs = torch.zeros((B, k + 1), dtype=torch.float64, device=dev)
+        rhs[:, -1] = -1.0
+        coef, info = torch.linalg.solve_ex(Bm, rhs)
+        ok = (info == 0) & torch.isfinite(coef).all(dim=1) & (~done)
+        F_diis = sum(Fs[i] * coef[:, i, None, None] for i in range(k))
+        F = torch.where(ok[:, None, None], F_diis, F)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    energies = (E + e_nuc).cpu().tolist()
+    its = torch.where(done, done_iter, torch.full_like(done_iter, counter)).cpu().tolist()
+    return BatchSCFResult(energies=energies, converged=done.cpu().tolist(), iterations=its,
+                          wall_s=t2 - t0, scf_s=t2 - t1, jk_s=jk_s, n_iter=counter,
+                          tuning=tuning)

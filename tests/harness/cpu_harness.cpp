@@ -61,23 +61,35 @@ static void run_split(const double* bra, int nb, const double* ket, int nk, cons
     }
 }
 
+static QuartetMap dense_map(int nbf, const int* f0)
+{
+    QuartetMap qm;
+    for (int k = 0; k < 4; ++k) {
+        qm.row[k] = (long long)f0[k] * nbf;
+        qm.col[k] = f0[k];
+    }
+    qm.ld = nbf;
+    return qm;
+}
+
 template <int LA, int LB, int LC, int LD>
 static void run_tiles(const double* v, double fac, const double* P, int nbf, const int* f0,
                       double* t)
 {
     using S = QuartetShape<LA, LB, LC, LD>;
-    digest_tile<LA, LB, LC, LD, 0>(v, 1, fac, P, nbf, f0[0], f0[1], f0[2], f0[3], t + S::T_AB);
-    digest_tile<LA, LB, LC, LD, 1>(v, 1, fac, P, nbf, f0[0], f0[1], f0[2], f0[3], t + S::T_CD);
-    digest_tile<LA, LB, LC, LD, 2>(v, 1, fac, P, nbf, f0[0], f0[1], f0[2], f0[3], t + S::T_AC);
-    digest_tile<LA, LB, LC, LD, 3>(v, 1, fac, P, nbf, f0[0], f0[1], f0[2], f0[3], t + S::T_AD);
-    digest_tile<LA, LB, LC, LD, 4>(v, 1, fac, P, nbf, f0[0], f0[1], f0[2], f0[3], t + S::T_BC);
-    digest_tile<LA, LB, LC, LD, 5>(v, 1, fac, P, nbf, f0[0], f0[1], f0[2], f0[3], t + S::T_BD);
+    const QuartetMap qm = dense_map(nbf, f0);
+    digest_tile<LA, LB, LC, LD, 0>(v, 1, fac, P, qm, t + S::T_AB);
+    digest_tile<LA, LB, LC, LD, 1>(v, 1, fac, P, qm, t + S::T_CD);
+    digest_tile<LA, LB, LC, LD, 2>(v, 1, fac, P, qm, t + S::T_AC);
+    digest_tile<LA, LB, LC, LD, 3>(v, 1, fac, P, qm, t + S::T_AD);
+    digest_tile<LA, LB, LC, LD, 4>(v, 1, fac, P, qm, t + S::T_BC);
+    digest_tile<LA, LB, LC, LD, 5>(v, 1, fac, P, qm, t + S::T_BD);
 }
 
 template <int LA, int LB, int LC, int LD>
 static void run_digest(const double* v, const double* P, int nbf, const int* f0, double* t)
 {
-    digest_quartet<LA, LB, LC, LD, double>(v, P, nbf, f0[0], f0[1], f0[2], f0[3], t);
+    digest_quartet<LA, LB, LC, LD, double>(v, P, dense_map(nbf, f0), t);
 }
 
 #define SF_CLASSES(X)                                                                        \

@@ -35,7 +35,7 @@ def test_exports_every_declared_symbol(lib):
 
 
 def test_version_and_error_string(lib):
-    assert lib.sf_version() == 100
+    assert lib.sf_version() == 200
     assert isinstance(lib.sf_last_error(), bytes)
 
 
@@ -62,3 +62,31 @@ def test_argument_errors_are_status_codes(lib):
     assert rc == 5 and not h.value
     with pytest.raises(engine.EngineError):
         engine.check(rc)
+
+
+def test_batch_and_graph_argument_errors(lib):
+    """Bad molecule ranges, tuning shapes and null handles are rejected
+    before any device work."""
+    h = ctypes.c_void_p()
+    L = np.zeros(2, dtype=np.int32)
+    npr = np.ones(2, dtype=np.int32)
+    # molecule shell ranges must cover [0, nshell] with no empty molecule
+    for ms in ([0, 2, 2], [1, 2], [0, 1]):
+        ms = np.array(ms, dtype=np.int32)
+        rc = lib.sf_basis_create_batch(len(ms) - 1, ms, 2, L, npr, np.zeros(6), np.ones(2),
+                                       np.ones(2), 2, np.array([0, 1, 2][:len(ms)], dtype=np.int32),
+                                       np.zeros(2, dtype=np.int32), ctypes.byref(h))
+        assert rc == 1 and not h.value, ms
+    # nuclei ranges beyond natom
+    rc = lib.sf_basis_create_batch(2, np.array([0, 1, 2], dtype=np.int32), 2, L, npr, np.zeros(6),
+                                   np.ones(2), np.ones(2), 1, np.array([0, 1, 2], dtype=np.int32),
+                                   np.zeros(2, dtype=np.int32), ctypes.byref(h))
+    assert rc == 1 and not h.value
+    assert lib.sf_basis_create_batch(0, np.zeros(1, dtype=np.int32), 2, L, npr, np.zeros(6),
+                                     np.ones(2), np.ones(2), 0, np.zeros(1, dtype=np.int32),
+                                     np.zeros(2, dtype=np.int32), ctypes.byref(h)) == 1
+    shapes = np.full(84, 256, dtype=np.int32)
+    assert lib.sf_set_tuning(None, shapes) == 1
+    assert lib.sf_jk_graph(None, None, None, None, 0, None) == 1
+    assert lib.sf_autotune(None, None, 0, None, shapes, np.zeros(42)) == 1
+    assert lib.sf_basis_set_layout(None, np.zeros(1, dtype=np.int64), np.zeros(1, dtype=np.int32)) == 1
