@@ -50,6 +50,9 @@ def parse():
                     help="target host time of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-scf", action="store_true", help="skip the FP64/mixed SCF energy check")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the 256-molecule batched SCF measurement")
+    ap.add_argument("--batch-size", type=int, default=256)
     ap.add_argument("--workload", type=int, default=1, help="workload index (1 = 20 waters)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (for HBM counter scaling)")
@@ -156,7 +159,10 @@ def main():
     J = torch.empty_like(P)
     K = torch.empty_like(P)
 
+    shapes = {p: basis.autotune(P, p)[0] for p in (0, 1)}
+
     def timed(precision):
+        basis.set_tuning(shapes[precision])
         for _ in range(args.warmup):
             basis.jk(P, precision, out=(J, K))
         torch.cuda.synchronize()
@@ -172,6 +178,7 @@ def main():
     Emix = float(0.5 * (P * (J - 0.5 * K)).sum())
 
     # per-kernel times from HIP events on the launch stream
+    basis.set_tuning(shapes[0])
     _, _, ms = basis.jk_profile(P, 0, out=(J, K))
     kern = [(ms[2 * w["class"]], w) for w in work if w["shellq"] > 0]
     dom_ms, dom = max(kern, key=lambda x: x[0])
@@ -229,9 +236,123 @@ def main():
                        "converged": bool(r64.converged and rmx.converged),
                        "iterations_fp64": r64.iterations, "iterations_mixed": rmx.iterations,
                        "wall_s_fp64": r64.wall_s, "wall_s_mixed": rmx.wall_s}
+    if not args.no_scf:
+        line["alkane_mixed"] = alkane_workload(args.steps)
+        line["gradient"] = gradient_workload()
+    if not args.no_batch:
+        del basis
+        line["batch"] = batch_workload(args.batch_size)
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(basis, mol, work, args.cpu_seconds)
+        line["cpu_baseline"] = cpu_baseline(engine.Basis(mol).screen(tau, tau_mixed), mol, work,
+                                            args.cpu_seconds)
     print(json.dumps(line))
+
+
+def alkane_workload(steps: int) -> dict:
+    """Config 3: zigzag C16H34 (C 4s3p1d, H 3s1p, N = 508), quartets with
+    Schwarz bound < 1e-5 in FP32: J/K builds/s FP64 vs mixed and the SCF
+    energy error of mixed against FP64."""
+    import torch
+    from segfault_py_amd import engine, hf, workloads as W
+
+    mol = W.config_molecule(2)
+    b = engine.Basis(mol).screen(1e-12, 1e-5)
+    nocc = int(mol.numbers.sum()) // 2
+    P = torch.tensor(synthetic_density(mol.nbf, nocc), dtype=torch.float64, device="cuda")
+    J, K = torch.empty_like(P), torch.empty_like(P)
+    out = {"workload": f"C16H34, 50 atoms, N={mol.nbf}, C 4s3p1d / H 3s1p, tau_mixed 1e-5"}
+    for name, prec in (("fp64", 0), ("mixed", 1)):
+        b.set_tuning(b.autotune(P, prec)[0])
+        b.jk(P, prec, out=(J, K))
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(steps):
+            b.jk(P, prec, out=(J, K))
+        torch.cuda.synchronize()
+        out[f"{name}_builds_per_s"] = steps / (time.perf_counter() - t)
+        r = hf.scf_diis(mol, basis=b, precision=prec, tol=1e-9)
+        out[f"energy_{name}"] = r.energy
+        out[f"scf_wall_s_{name}"] = r.wall_s
+    out["mixed_speedup"] = out["mixed_builds_per_s"] / out["fp64_builds_per_s"]
+    out["dE_mixed_vs_fp64"] = abs(out["energy_mixed"] - out["energy_fp64"])
+    c = b.counts()[0]
+    out["fp32_fraction_of_shell_quartets"] = float(c[:, 2].sum() / max(c[:, 0].sum(), 1))
+    return out
+
+
+def gradient_workload() -> dict:
+    """Config 4: ten waters (30 atoms, s/p/d basis): SCF + analytic RHF
+    gradient (derivative-ERI kernels) in FP64 and with the mixed-precision
+    SCF density; max gradient difference mixed vs FP64 (the finite-difference
+    checks are in tests/test_gpu_grad.py)."""
+    import torch
+    from segfault_py_amd import engine, grad as G, hf, workloads as W
+
+    mol = W.config_molecule(3)
+    b = engine.Basis(mol).screen(1e-12, 1e-5)
+    out = {"workload": f"(H2O)10, 30 atoms, N={mol.nbf}, s/p + d(O)"}
+    g = {}
+    for name, prec in (("fp64", 0), ("mixed", 1)):
+        r = hf.scf_diis(mol, basis=b, precision=prec, tol=1e-10)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        g[name] = G.rhf_gradient(mol, r, b)
+        torch.cuda.synchronize()
+        out[f"gradient_s_{name}"] = time.perf_counter() - t
+        out[f"energy_{name}"] = r.energy
+    out["max_dgrad_mixed_vs_fp64"] = float(np.max(np.abs(g["mixed"] - g["fp64"])))
+    out["max_abs_gradient"] = float(np.max(np.abs(g["fp64"])))
+    return out
+
+
+def batch_workload(nmol: int) -> dict:
+    """Config 5: nmol uneven molecules (water clusters of 1-8 waters and
+    CkH2k+2, k = 1..7, s/p basis) solved to 1e-8 Eh by the batched DIIS SCF:
+    one BatchBasis, autotuned persistent class kernels, J/K replayed from a
+    hipGraph every iteration.  Throughput = molecules / wall time of the
+    whole batch (setup included); latency = one molecule solved alone."""
+    import torch
+    from segfault_py_amd import engine, hf, workloads as W
+
+    mols = W.batch(nmol, seed=5)
+    out = {"workload": f"{nmol} molecules, 3-24 atoms (water clusters and alkanes), "
+                       "even-tempered s/p basis, Schwarz 1e-12, SCF tol 1e-8",
+           "atoms": int(sum(m.natom for m in mols)), "nbf_max": int(max(m.nbf for m in mols))}
+    for name, prec in (("fp64", 0), ("mixed", 1)):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        bb = engine.BatchBasis(mols).screen(1e-12, 1e-5)
+        res = hf.scf_batch(mols, precision=prec, basis=bb)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        c, fq, pq = bb.counts()
+        out[name] = {"molecules_per_s": nmol / wall, "wall_s": wall, "scf_s": res.scf_s,
+                     "jk_s": res.jk_s, "iterations": res.n_iter,
+                     "jk_ms_per_iteration": 1e3 * res.jk_s / max(res.n_iter + 1, 1),
+                     "converged": int(sum(res.converged)),
+                     "shell_quartets_per_build": int(c[:, 0].sum()),
+                     "function_quartets_per_build": int(fq),
+                     "tuned_shapes": res.tuning[0][:, :, :].reshape(-1, 4).tolist()
+                     if res.tuning is not None else None}
+        out[name]["_energies"] = res.energies
+        del bb
+    e64, emx = out["fp64"].pop("_energies"), out["mixed"].pop("_energies")
+    out["max_dE_mixed_vs_fp64"] = max(abs(a - b) for a, b in zip(e64, emx))
+    # latency: a few molecules solved alone (own handle, FP64)
+    sample = sorted({0, nmol // 3, (2 * nmol) // 3, nmol - 1})
+    lat, dE = [], []
+    for k in sample:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = hf.scf_diis(mols[k], tol=1e-8)
+        torch.cuda.synchronize()
+        lat.append(time.perf_counter() - t0)
+        dE.append(abs(r.energy - e64[k]))
+    out["single_molecule_latency_s"] = {"median": float(np.median(lat)), "max": float(max(lat)),
+                                        "molecules": sample}
+    out["max_dE_batch_vs_single"] = float(max(dE))
+    out["sequential_molecules_per_s"] = 1.0 / float(np.median(lat))
+    return out
 
 
 if __name__ == "__main__":

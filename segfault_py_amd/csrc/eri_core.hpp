@@ -96,23 +96,19 @@ struct QuartetShape {
     }
 };
 
-// Rys nodes for N, with the closed-form F0/F1 rule when N == 1.
-template <int N>
-SF_HD SF_INLINE void rys_nodes(double X, double* u, double* w)
-{
-    if constexpr (N == 1) rys_roots1_closed(X, u, w);
-    else rys_roots<N>(X, u, w);
-}
-
 // Accumulate (ab|cd) for all Cartesian components into acc[NINT] (not
 // zeroed here).  Bra primitive pairs bra[0..nbra), ket pairs ket[0..nket).
 // R is the type of the recurrences and of acc; X, nodes and weights are
-// always evaluated in double.
+// always evaluated in double, from the node table nt (global or a copy in
+// LDS).  The ket primitive loop is the outer one: in the kernels each lane
+// has its own ket pair (loaded once per quartet) while the bra pair is
+// shared by the wavefront's lanes (one broadcast load per bra primitive).
 template <int LA, int LB, int LC, int LD, typename R>
 SF_HD SF_INLINE void eri_shell_quartet(const PrimPair* __restrict__ bra, int nbra,
                                        const PrimPair* __restrict__ ket, int nket,
                                        const double* A, const double* B, const double* C,
-                                       const double* D, R* __restrict__ acc)
+                                       const double* D, R* __restrict__ acc,
+                                       const NodeTable& nt = global_node_table<QuartetShape<LA, LB, LC, LD>::NR>())
 {
     using S = QuartetShape<LA, LB, LC, LD>;
     constexpr int NR = S::NR, NAB = S::NAB, NCD = S::NCD;
@@ -127,13 +123,14 @@ SF_HD SF_INLINE void eri_shell_quartet(const PrimPair* __restrict__ bra, int nbr
 #pragma unroll
         for (int e = 1; e <= LD; ++e) CDp[k][e] = CDp[k][e - 1] * cd;
     }
-    for (int ib = 0; ib < nbra; ++ib) {
-        const PrimPair pb = bra[ib];
-        const double p = pb.p;
-        const double PA[3] = {pb.Px - A[0], pb.Py - A[1], pb.Pz - A[2]};
-        for (int ik = 0; ik < nket; ++ik) {
-            const PrimPair pk = ket[ik];
-            const double q = pk.p;
+    for (int ik = 0; ik < nket; ++ik) {
+        const PrimPair pk = ket[ik];
+        const double q = pk.p;
+        const double QC[3] = {pk.Px - C[0], pk.Py - C[1], pk.Pz - C[2]};
+        for (int ib = 0; ib < nbra; ++ib) {
+            const PrimPair pb = bra[ib];
+            const double p = pb.p;
+            const double PA[3] = {pb.Px - A[0], pb.Py - A[1], pb.Pz - A[2]};
             const double pq = p + q;
             const double rs = fast_rsqrt(pq);
             const double ipq = rs * rs;
@@ -143,8 +140,7 @@ SF_HD SF_INLINE void eri_shell_quartet(const PrimPair* __restrict__ bra, int nbr
             // 2 pi^{5/2} / (p q sqrt(p+q)) K_ab K_cd
             const double pref = 34.986836655249725 * pb.Kp * pk.Kp * rs;
             double u[NR], w[NR];
-            rys_nodes<NR>(X, u, w);
-            const double QC[3] = {pk.Px - C[0], pk.Py - C[1], pk.Pz - C[2]};
+            rys_nodes_tab<NR>(X, nt, u, w);
 #pragma unroll
             for (int r = 0; r < NR; ++r) {
                 const double t2 = u[r];
@@ -292,7 +288,8 @@ template <int LA, int LB, int LC, int LD, int FA, int FB, typename R>
 SF_HD SF_INLINE void eri_bra_component(const PrimPair* __restrict__ bra, int nbra,
                                        const PrimPair* __restrict__ ket, int nket,
                                        const double* A, const double* B, const double* C,
-                                       const double* D, R* __restrict__ acc)
+                                       const double* D, R* __restrict__ acc,
+                                       const NodeTable& nt = global_node_table<QuartetShape<LA, LB, LC, LD>::NR>())
 {
     using S = QuartetShape<LA, LB, LC, LD>;
     constexpr int NR = S::NR;
@@ -308,13 +305,14 @@ SF_HD SF_INLINE void eri_bra_component(const PrimPair* __restrict__ bra, int nbr
 #pragma unroll
         for (int e = 1; e <= LD; ++e) CDp[k][e] = CDp[k][e - 1] * cd;
     }
-    for (int ib = 0; ib < nbra; ++ib) {
-        const PrimPair pb = bra[ib];
-        const double p = pb.p;
-        const double PA[3] = {pb.Px - A[0], pb.Py - A[1], pb.Pz - A[2]};
-        for (int ik = 0; ik < nket; ++ik) {
-            const PrimPair pk = ket[ik];
-            const double q = pk.p;
+    for (int ik = 0; ik < nket; ++ik) {
+        const PrimPair pk = ket[ik];
+        const double q = pk.p;
+        const double QC[3] = {pk.Px - C[0], pk.Py - C[1], pk.Pz - C[2]};
+        for (int ib = 0; ib < nbra; ++ib) {
+            const PrimPair pb = bra[ib];
+            const double p = pb.p;
+            const double PA[3] = {pb.Px - A[0], pb.Py - A[1], pb.Pz - A[2]};
             const double pq = p + q;
             const double rs = fast_rsqrt(pq);
             const double ipq = rs * rs;
@@ -323,8 +321,7 @@ SF_HD SF_INLINE void eri_bra_component(const PrimPair* __restrict__ bra, int nbr
             const double X = rho * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]);
             const double pref = 34.986836655249725 * pb.Kp * pk.Kp * rs;
             double u[NR], w[NR];
-            rys_nodes<NR>(X, u, w);
-            const double QC[3] = {pk.Px - C[0], pk.Py - C[1], pk.Pz - C[2]};
+            rys_nodes_tab<NR>(X, nt, u, w);
 #pragma unroll
             for (int r = 0; r < NR; ++r) {
                 const double t2 = u[r];
@@ -471,15 +468,16 @@ SF_HD SF_INLINE void grad_bra_component(const PrimPair* __restrict__ bra, int nb
 #pragma unroll
         for (int e = 1; e <= LD; ++e) CDp[k][e] = CDp[k][e - 1] * cd;
     }
-    for (int ib = 0; ib < nbra; ++ib) {
-        const PrimPair pb = bra[ib];
-        const double p = pb.p;
-        const R ta = (R)(2.0 * (p - pb.eb)), tb = (R)(2.0 * pb.eb);
-        const double PA[3] = {pb.Px - A[0], pb.Py - A[1], pb.Pz - A[2]};
-        for (int ik = 0; ik < nket; ++ik) {
-            const PrimPair pk = ket[ik];
-            const double q = pk.p;
-            const R tc = (R)(2.0 * (q - pk.eb));
+    for (int ik = 0; ik < nket; ++ik) {
+        const PrimPair pk = ket[ik];
+        const double q = pk.p;
+        const R tc = (R)(2.0 * (q - pk.eb));
+        const double QC[3] = {pk.Px - C[0], pk.Py - C[1], pk.Pz - C[2]};
+        for (int ib = 0; ib < nbra; ++ib) {
+            const PrimPair pb = bra[ib];
+            const double p = pb.p;
+            const R ta = (R)(2.0 * (p - pb.eb)), tb = (R)(2.0 * pb.eb);
+            const double PA[3] = {pb.Px - A[0], pb.Py - A[1], pb.Pz - A[2]};
             const double pq = p + q;
             const double rs = fast_rsqrt(pq);
             const double ipq = rs * rs;
@@ -488,8 +486,7 @@ SF_HD SF_INLINE void grad_bra_component(const PrimPair* __restrict__ bra, int nb
             const double X = rho * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]);
             const double pref = 34.986836655249725 * pb.Kp * pk.Kp * rs;
             double u[NR], w[NR];
-            rys_nodes<NR>(X, u, w);
-            const double QC[3] = {pk.Px - C[0], pk.Py - C[1], pk.Pz - C[2]};
+            rys_nodes_tab<NR>(X, global_node_table<NR>(), u, w);
 #pragma unroll
             for (int r = 0; r < NR; ++r) {
                 const double t2 = u[r];

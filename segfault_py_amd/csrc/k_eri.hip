@@ -64,14 +64,38 @@ struct QuartetIdx {
     long long qa;
 };
 
-__device__ __forceinline__ QuartetIdx locate(const ClassParams& cp, long long q)
+// The bra of quartet q: one binary search per wavefront for its first
+// lane's quartet qw (wave-uniform, so scalar loads), then each lane walks
+// forward over the few bras the wavefront spans.  Requires qw <= q < nq.
+__device__ __forceinline__ QuartetIdx locate(const ClassParams& cp, long long q, long long qw)
 {
+    const unsigned lo32 = __builtin_amdgcn_readfirstlane((unsigned)(qw & 0xffffffffLL));
+    const unsigned hi32 = __builtin_amdgcn_readfirstlane((unsigned)(qw >> 32));
+    const long long qu = ((long long)hi32 << 32) | lo32;
+    int i = __builtin_amdgcn_readfirstlane(find_bra(cp.seg_base, cp.bra.n, qu));
+    while (cp.seg_base[i + 1] <= q) ++i;
     QuartetIdx r;
-    r.i = find_bra(cp.seg_base, cp.bra.n, q);
-    const int jr = cp.seg_k0[r.i] + (int)(q - cp.seg_base[r.i]);
-    r.j = cp.ket0[r.i] + jr;
-    r.qa = cp.qall_base[r.i] + jr;
+    r.i = i;
+    const int jr = cp.seg_k0[i] + (int)(q - cp.seg_base[i]);
+    r.j = cp.ket0[i] + jr;
+    r.qa = cp.qall_base[i] + jr;
     return r;
+}
+
+// Copies the node table of NR nodes into LDS (block-wide; every thread of
+// the block must call it) and returns its descriptor; global table when
+// the class has no LDS copy.
+template <int NR>
+__device__ __forceinline__ NodeTable stage_node_table(double* s_tab)
+{
+    constexpr int n = node_table_lds(NR);
+    if constexpr (n > 0) {
+        for (int k = threadIdx.x; k < n; k += blockDim.x) s_tab[k] = node_table_elem<NR>(k);
+        __syncthreads();
+        return NodeTable{s_tab, NR == 1 ? 10 : 0};
+    } else {
+        return global_node_table<NR>();
+    }
 }
 
 __device__ __forceinline__ QuartetMap quartet_map(const ShellMap& m, int a, int b, int c, int d)
@@ -95,9 +119,13 @@ template <int LA, int LB, int LC, int LD, typename R, int MODE>
 __global__ __launch_bounds__(256) void k_eri_class(const ClassParams cp)
 {
     using S = QuartetShape<LA, LB, LC, LD>;
+    __shared__ double s_tab[node_table_lds(S::NR) > 0 ? node_table_lds(S::NR) : 1];
+    const NodeTable nt = stage_node_table<S::NR>(s_tab);
     const long long stride = (long long)gridDim.x * blockDim.x;
-    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < cp.nq; q += stride) {
-    const QuartetIdx ix = locate(cp, q);
+    const long long lane0 = (long long)blockIdx.x * blockDim.x + (threadIdx.x & ~63u);
+    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x, qw = lane0; q < cp.nq;
+         q += stride, qw += stride) {
+    const QuartetIdx ix = locate(cp, q, qw);
     const int i = ix.i, j = ix.j;
     const long long qa = ix.qa;
     const int a = cp.bra.sa[i], b = cp.bra.sb[i];
@@ -110,7 +138,8 @@ __global__ __launch_bounds__(256) void k_eri_class(const ClassParams cp)
 #pragma unroll
     for (int k = 0; k < S::NINT; ++k) acc[k] = (R)0;
     eri_shell_quartet<LA, LB, LC, LD, R>(cp.prim + cp.bra.poff[i], cp.bra.np[i],
-                                         cp.prim + cp.ket.poff[j], cp.ket.np[j], A, B, C, D, acc);
+                                         cp.prim + cp.ket.poff[j], cp.ket.np[j], A, B, C, D, acc,
+                                         nt);
     if constexpr (MODE == kModeRaw) {
 #pragma unroll
         for (int k = 0; k < S::NINT; ++k) cp.out[k * cp.qstride + qa] = (double)acc[k];
@@ -170,6 +199,8 @@ __global__ __launch_bounds__(256) void k_eri_split(const ClassParams cp, double*
 {
     using S = QuartetShape<LA, LB, LC, LD>;
     constexpr int NS = S::NA * S::NB, NK = S::NC * S::ND;
+    __shared__ double s_tab[node_table_lds(S::NR) > 0 ? node_table_lds(S::NR) : 1];
+    const NodeTable nt = stage_node_table<S::NR>(s_tab);
     const long long nwork = ((cp.nq + 63) >> 6) * NS;
     const long long wstride = ((long long)gridDim.x * blockDim.x) >> 6;
     for (long long gw = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6; gw < nwork;
@@ -177,7 +208,7 @@ __global__ __launch_bounds__(256) void k_eri_split(const ClassParams cp, double*
     const int s = __builtin_amdgcn_readfirstlane((int)(gw % NS));
     const long long q = (gw / NS) * 64 + (threadIdx.x & 63);
     if (q >= cp.nq) continue;
-    const QuartetIdx ix = locate(cp, q);
+    const QuartetIdx ix = locate(cp, q, (gw / NS) * 64);
     const int i = ix.i, j = ix.j;
     const long long qa = ix.qa;
     const int a = cp.bra.sa[i], b = cp.bra.sb[i];
@@ -193,7 +224,7 @@ __global__ __launch_bounds__(256) void k_eri_split(const ClassParams cp, double*
         constexpr int FA = decltype(sc)::value / S::NB, FB = decltype(sc)::value % S::NB;
         eri_bra_component<LA, LB, LC, LD, FA, FB, R>(cp.prim + cp.bra.poff[i], cp.bra.np[i],
                                                      cp.prim + cp.ket.poff[j], cp.ket.np[j], A, B,
-                                                     C, D, acc);
+                                                     C, D, acc, nt);
     });
 #pragma unroll
     for (int k = 0; k < NK; ++k) raw[(long long)(s * NK + k) * cp.qstride + qa] = (double)acc[k];
@@ -213,7 +244,7 @@ __global__ __launch_bounds__(256) void k_digest_split(const ClassParams cp,
     const int t = __builtin_amdgcn_readfirstlane((int)(gw % 6));
     const long long q = (gw / 6) * 64 + (threadIdx.x & 63);
     if (q >= cp.nq) continue;
-    const QuartetIdx ix = locate(cp, q);
+    const QuartetIdx ix = locate(cp, q, (gw / 6) * 64);
     const int i = ix.i, j = ix.j;
     const long long qa = ix.qa;
     const int a = cp.bra.sa[i], b = cp.bra.sb[i];
@@ -247,7 +278,7 @@ __global__ __launch_bounds__(256) void k_grad_split(const ClassParams cp, double
     const int s = __builtin_amdgcn_readfirstlane((int)(gw % NS));
     const long long q = (gw / NS) * 64 + (threadIdx.x & 63);
     if (q >= cp.nq) return;
-    const QuartetIdx ix = locate(cp, q);
+    const QuartetIdx ix = locate(cp, q, (gw / NS) * 64);
     const int i = ix.i, j = ix.j;
     const long long qa = ix.qa;
     const int a = cp.bra.sa[i], b = cp.bra.sb[i];

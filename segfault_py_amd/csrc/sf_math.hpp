@@ -124,18 +124,26 @@ SF_HD SF_INLINE void boys_grid(int mmax, double X, double* F)
 // F_0 and F_1 without exp or erf: grid Taylor series below X = 40, the
 // asymptotic forms above (e^-X < 4.3e-18 there, below double rounding of
 // F_0 and F_1).  When only F_0 is used the compiler drops the F_1 terms.
-SF_HD SF_INLINE void boys01(double X, double& F0, double& F1)
+// g: grid rows of F_0..F_9 at X_k = 0.1 k, `stride` doubles apart (the
+// global table sf_boys_grid has stride SF_BOYS_MTAB; kernels may stage the
+// first 10 columns in LDS with stride 10).
+SF_HD SF_INLINE void boys01_tab(double X, const double* g, int stride, double& F0, double& F1)
 {
     if (X < 39.95) {
         const int k = (int)(X * 10.0 + 0.5);
         const double d = k * SF_BOYS_DX - X;
-        const double* g = sf_boys_grid + k * SF_BOYS_MTAB;
-        F0 = taylor9(g, d);
-        F1 = taylor9(g + 1, d);
+        const double* gk = g + k * stride;
+        F0 = taylor9(gk, d);
+        F1 = taylor9(gk + 1, d);
     } else {
         F0 = 0.5 * kSqrtPi * fast_rsqrt(X);
         F1 = 0.5 * F0 * fast_rcp(X);
     }
+}
+
+SF_HD SF_INLINE void boys01(double X, double& F0, double& F1)
+{
+    boys01_tab(X, sf_boys_grid, SF_BOYS_MTAB, F0, F1);
 }
 
 // ----------------------------------------------------------------- Rys ----
@@ -171,16 +179,17 @@ SF_HD SF_INLINE double herm_w(int r)
 
 constexpr int rys_xmax(int n) { return n == 1 ? 40 : n == 2 ? 48 : n == 3 ? 54 : n == 4 ? 60 : 66; }
 
-// Nodes u_r = t_r^2 (ascending) and weights w_r of the N-point Rys rule.
+// Nodes u_r = t_r^2 (ascending) and weights w_r of the N-point Rys rule,
+// from the Chebyshev table c of N nodes (rys_table<N>() or a copy of it).
 template <int N>
-SF_HD SF_INLINE void rys_roots(double X, double* u, double* w)
+SF_HD SF_INLINE void rys_roots_tab(double X, const double* __restrict__ tab, double* u, double* w)
 {
     static_assert(N >= 1 && N <= 5, "1..5 Rys nodes");
     if (X < (double)rys_xmax(N)) {
         const int it = (int)X;
         const double s = 2.0 * (X - it) - 1.0;
         const double s2 = 2.0 * s;
-        const double* c = rys_table<N>() + it * (2 * N) * (SF_RYS_DEG + 1);
+        const double* c = tab + it * (2 * N) * (SF_RYS_DEG + 1);
 #pragma unroll
         for (int v = 0; v < 2 * N; ++v) {
             const double* cv = c + v * (SF_RYS_DEG + 1);
@@ -207,6 +216,12 @@ SF_HD SF_INLINE void rys_roots(double X, double* u, double* w)
     }
 }
 
+template <int N>
+SF_HD SF_INLINE void rys_roots(double X, double* u, double* w)
+{
+    rys_roots_tab<N>(X, rys_table<N>(), u, w);
+}
+
 // One-node rule from F_0, F_1 (used where only one node is needed).
 SF_HD SF_INLINE void rys_roots1_closed(double X, double* u, double* w)
 {
@@ -214,6 +229,49 @@ SF_HD SF_INLINE void rys_roots1_closed(double X, double* u, double* w)
     boys01(X, F0, F1);
     u[0] = F1 * fast_rcp(F0);
     w[0] = F0;
+}
+
+// The node table a kernel reads: for one node the Boys grid rows (F_0..F_9,
+// `stride` doubles apart), for N >= 2 nodes the Chebyshev table of N nodes.
+struct NodeTable {
+    const double* p;
+    int stride;
+};
+
+template <int N>
+SF_HD SF_INLINE NodeTable global_node_table()
+{
+    if constexpr (N == 1) return NodeTable{sf_boys_grid, SF_BOYS_MTAB};
+    else return NodeTable{rys_table<N>(), 0};
+}
+
+// Doubles a kernel stages in LDS for N nodes (0: read the global table):
+// the 400 grid rows below X = 39.95 (10 columns) for N = 1, the Chebyshev
+// table for N = 2, 3.
+constexpr int node_table_lds(int n)
+{
+    return n == 1 ? 400 * 10 : n == 2 ? 48 * 4 * (SF_RYS_DEG + 1) : n == 3 ? 54 * 6 * (SF_RYS_DEG + 1) : 0;
+}
+
+// Element k of the LDS copy for N nodes.
+template <int N>
+SF_HD SF_INLINE double node_table_elem(int k)
+{
+    if constexpr (N == 1) return sf_boys_grid[(k / 10) * SF_BOYS_MTAB + k % 10];
+    else return rys_table<N>()[k];
+}
+
+template <int N>
+SF_HD SF_INLINE void rys_nodes_tab(double X, const NodeTable& t, double* u, double* w)
+{
+    if constexpr (N == 1) {
+        double F0, F1;
+        boys01_tab(X, t.p, t.stride, F0, F1);
+        u[0] = F1 * fast_rcp(F0);
+        w[0] = F0;
+    } else {
+        rys_roots_tab<N>(X, t.p, u, w);
+    }
 }
 
 }  // namespace sf
