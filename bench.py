@@ -176,6 +176,10 @@ def main():
     E64 = float(0.5 * (P * (J - 0.5 * K)).sum())
     tmix = timed(1)
     Emix = float(0.5 * (P * (J - 0.5 * K)).sum())
+    basis.set_prim_tau(0.0)              # FP32 arithmetic alone, no primitive screening
+    tmix_np = timed(1)
+    Emix_np = float(0.5 * (P * (J - 0.5 * K)).sum())
+    basis.set_prim_tau(-1.0)
 
     # per-kernel times from HIP events on the launch stream
     basis.set_tuning(shapes[0])
@@ -212,6 +216,8 @@ def main():
         "mixed_builds_per_s": 1.0 / tmix,
         "mixed_speedup": t64 / tmix,
         "dE_2e_mixed_vs_fp64": abs(Emix - E64),
+        "mixed_no_prim_screen": {"builds_per_s": 1.0 / tmix_np, "speedup": t64 / tmix_np,
+                                 "dE_2e_vs_fp64": abs(Emix_np - E64)},
         "quartets_per_s_fp64": funcq / t64,
         "shell_quartets_per_s_fp64": shellq / t64,
         "prim_quartets_per_s_fp64": primq / t64,

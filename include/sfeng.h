@@ -116,6 +116,12 @@ int sf_schwarz(sf_basis *b, double *Q);
  * in FP32.  tau = 0 keeps every quartet (the unscreened set of
  * build_eri_parallel, hf.py:37-43).  Replaces hf.py:33-61's enumeration. */
 int sf_screen(sf_basis *b, double tau, double tau_mixed);
+/* Primitive screening of the FP32 quartets (mixed precision only; no
+ * reference counterpart): a primitive quartet whose primitive-pair Schwarz
+ * product Qp_ab Qp_cd is below tau_prim is skipped.  Default (tau_prim < 0):
+ * 1e-8 * tau_mixed, about the FP32 resolution of the largest FP32 quartet.
+ * 0 disables it.  FP64 quartets never skip primitives. */
+int sf_set_prim_tau(sf_basis *b, double tau_prim);
 /* counts[3*c + {0,1,2}] = surviving / FP64 / FP32 shell quartets of
  * quartet class c (0..20); *funcq = surviving function-level quartets. */
 int sf_quartet_counts(const sf_basis *b, long long *counts, long long *funcq,
@@ -157,8 +163,9 @@ int sf_jk_graph(sf_basis *b, const double *P, double *J, double *K, int precisio
 /* Launch-shape autotuner (no reference counterpart).  For every quartet
  * class and precision segment (FP64 / FP32 part), times 64/128/256-thread
  * blocks with one thread per quartet or persistent grids of 2 or 8 blocks
- * per compute unit on the current screened lists (density P, device) and
- * keeps the fastest; sf_jk and sf_jk_graph use the choice.
+ * per compute unit on the current screened lists (density P, device; each
+ * candidate timed on the first 2^20 quartets of the segment) and keeps the
+ * fastest; sf_jk and sf_jk_graph use the choice.
  * shapes[4c + 2seg + {0,1}] receives (block, grid; 0 = one per quartet),
  * ms[2c + seg] the chosen shape's time; either may be NULL. */
 int sf_autotune(sf_basis *b, const double *P, int precision, void *stream, int *shapes,

@@ -103,12 +103,18 @@ struct QuartetShape {
 // LDS).  The ket primitive loop is the outer one: in the kernels each lane
 // has its own ket pair (loaded once per quartet) while the bra pair is
 // shared by the wavefront's lanes (one broadcast load per bra primitive).
-template <int LA, int LB, int LC, int LD, typename R>
+// With PS (primitive screening), bq/kq are the primitive pairs' own Schwarz
+// factors, sorted descending within each shell pair, and primitive quartets
+// with bq * kq < tau_prim are skipped (both loops stop at the first one).
+template <int LA, int LB, int LC, int LD, typename R, bool PS = false>
 SF_HD SF_INLINE void eri_shell_quartet(const PrimPair* __restrict__ bra, int nbra,
                                        const PrimPair* __restrict__ ket, int nket,
                                        const double* A, const double* B, const double* C,
                                        const double* D, R* __restrict__ acc,
-                                       const NodeTable& nt = global_node_table<QuartetShape<LA, LB, LC, LD>::NR>())
+                                       const NodeTable& nt = global_node_table<QuartetShape<LA, LB, LC, LD>::NR>(),
+                                       const double* __restrict__ bq = nullptr,
+                                       const double* __restrict__ kq = nullptr,
+                                       double tau_prim = 0.0)
 {
     using S = QuartetShape<LA, LB, LC, LD>;
     constexpr int NR = S::NR, NAB = S::NAB, NCD = S::NCD;
@@ -124,10 +130,18 @@ SF_HD SF_INLINE void eri_shell_quartet(const PrimPair* __restrict__ bra, int nbr
         for (int e = 1; e <= LD; ++e) CDp[k][e] = CDp[k][e - 1] * cd;
     }
     for (int ik = 0; ik < nket; ++ik) {
+        double qk = 0.0;
+        if constexpr (PS) {
+            qk = kq[ik];
+            if (nbra == 0 || qk * bq[0] < tau_prim) break;
+        }
         const PrimPair pk = ket[ik];
         const double q = pk.p;
         const double QC[3] = {pk.Px - C[0], pk.Py - C[1], pk.Pz - C[2]};
         for (int ib = 0; ib < nbra; ++ib) {
+            if constexpr (PS) {
+                if (bq[ib] * qk < tau_prim) break;
+            }
             const PrimPair pb = bra[ib];
             const double p = pb.p;
             const double PA[3] = {pb.Px - A[0], pb.Py - A[1], pb.Pz - A[2]};
@@ -284,12 +298,15 @@ SF_HD SF_INLINE void dir_component(R g00, R c00, R c00p, R b10, R b01, R b00, R 
 // (fa fb | fc fd) for ONE bra component pair (FA of shell A, FB of shell B)
 // and all ket components, accumulated into acc[NC*ND].  Used where the full
 // class does not fit in registers: one wavefront per bra component pair.
-template <int LA, int LB, int LC, int LD, int FA, int FB, typename R>
+template <int LA, int LB, int LC, int LD, int FA, int FB, typename R, bool PS = false>
 SF_HD SF_INLINE void eri_bra_component(const PrimPair* __restrict__ bra, int nbra,
                                        const PrimPair* __restrict__ ket, int nket,
                                        const double* A, const double* B, const double* C,
                                        const double* D, R* __restrict__ acc,
-                                       const NodeTable& nt = global_node_table<QuartetShape<LA, LB, LC, LD>::NR>())
+                                       const NodeTable& nt = global_node_table<QuartetShape<LA, LB, LC, LD>::NR>(),
+                                       const double* __restrict__ bq = nullptr,
+                                       const double* __restrict__ kq = nullptr,
+                                       double tau_prim = 0.0)
 {
     using S = QuartetShape<LA, LB, LC, LD>;
     constexpr int NR = S::NR;
@@ -306,10 +323,18 @@ SF_HD SF_INLINE void eri_bra_component(const PrimPair* __restrict__ bra, int nbr
         for (int e = 1; e <= LD; ++e) CDp[k][e] = CDp[k][e - 1] * cd;
     }
     for (int ik = 0; ik < nket; ++ik) {
+        double qk = 0.0;
+        if constexpr (PS) {
+            qk = kq[ik];
+            if (nbra == 0 || qk * bq[0] < tau_prim) break;
+        }
         const PrimPair pk = ket[ik];
         const double q = pk.p;
         const double QC[3] = {pk.Px - C[0], pk.Py - C[1], pk.Pz - C[2]};
         for (int ib = 0; ib < nbra; ++ib) {
+            if constexpr (PS) {
+                if (bq[ib] * qk < tau_prim) break;
+            }
             const PrimPair pb = bra[ib];
             const double p = pb.p;
             const double PA[3] = {pb.Px - A[0], pb.Py - A[1], pb.Pz - A[2]};

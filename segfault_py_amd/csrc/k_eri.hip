@@ -28,12 +28,12 @@
 namespace sf {
 
 using SchwarzLaunch = int (*)(const int*, const int*, const int*, const int*, int, const PrimPair*,
-                              const double*, double*, hipStream_t);
+                              const double*, double*, double*, hipStream_t);
 #define X(id, a, b, c, d)                                                                       \
     int launch_class_##id(int fp32, int mode, const ClassParams& cp, const Tune& tn,           \
                           hipStream_t s);                                                      \
     int launch_schwarz_##id(const int*, const int*, const int*, const int*, int,               \
-                            const PrimPair*, const double*, double*, hipStream_t);             \
+                            const PrimPair*, const double*, double*, double*, hipStream_t);    \
     int launch_grad_##id(const ClassParams& cp, double* gout, hipStream_t s);
 SF_FOR_EACH_CLASS(X)
 #undef X
@@ -137,9 +137,12 @@ __global__ __launch_bounds__(256) void k_eri_class(const ClassParams cp)
     R acc[S::NINT];
 #pragma unroll
     for (int k = 0; k < S::NINT; ++k) acc[k] = (R)0;
-    eri_shell_quartet<LA, LB, LC, LD, R>(cp.prim + cp.bra.poff[i], cp.bra.np[i],
-                                         cp.prim + cp.ket.poff[j], cp.ket.np[j], A, B, C, D, acc,
-                                         nt);
+    // FP32 quartets also skip primitive quartets below FP32 resolution
+    constexpr bool PS = std::is_same<R, float>::value;
+    eri_shell_quartet<LA, LB, LC, LD, R, PS>(cp.prim + cp.bra.poff[i], cp.bra.np[i],
+                                             cp.prim + cp.ket.poff[j], cp.ket.np[j], A, B, C, D,
+                                             acc, nt, cp.primq + cp.bra.poff[i],
+                                             cp.primq + cp.ket.poff[j], cp.tau_prim);
     if constexpr (MODE == kModeRaw) {
 #pragma unroll
         for (int k = 0; k < S::NINT; ++k) cp.out[k * cp.qstride + qa] = (double)acc[k];
@@ -162,12 +165,15 @@ __global__ __launch_bounds__(256) void k_eri_class(const ClassParams cp)
     }
 }
 
-// Schwarz factor Q_ab = sqrt(max_{i in a, j in b} |(ij|ij)|) for one pair class.
+// Schwarz factor Q_ab = sqrt(max_{i in a, j in b} |(ij|ij)|) for one pair
+// class, and the same factor Qp of every primitive pair of the shell pair
+// taken on its own (the bound of its share of any quartet).
 template <int LA, int LB>
 __global__ __launch_bounds__(256) void k_schwarz(const int* __restrict__ sa, const int* __restrict__ sb,
                                                  const int* __restrict__ poff, const int* __restrict__ np,
                                                  int n, const PrimPair* __restrict__ prim,
-                                                 const double* __restrict__ xyz, double* __restrict__ Q)
+                                                 const double* __restrict__ xyz, double* __restrict__ Q,
+                                                 double* __restrict__ Qp)
 {
     using S = QuartetShape<LA, LB, LA, LB>;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -175,18 +181,28 @@ __global__ __launch_bounds__(256) void k_schwarz(const int* __restrict__ sa, con
     const int a = sa[i], b = sb[i];
     const double A[3] = {xyz[3 * a], xyz[3 * a + 1], xyz[3 * a + 2]};
     const double B[3] = {xyz[3 * b], xyz[3 * b + 1], xyz[3 * b + 2]};
+    auto diag_max = [](const double* acc) {
+        double m = 0.0;
+#pragma unroll
+        for (int fa = 0; fa < S::NA; ++fa)
+#pragma unroll
+            for (int fb = 0; fb < S::NB; ++fb)
+                m = fmax(m, fabs(acc[((fa * S::NB + fb) * S::NA + fa) * S::NB + fb]));
+        return sqrt(m);
+    };
     double acc[S::NINT];
 #pragma unroll
     for (int k = 0; k < S::NINT; ++k) acc[k] = 0.0;
     eri_shell_quartet<LA, LB, LA, LB, double>(prim + poff[i], np[i], prim + poff[i], np[i], A, B, A,
                                               B, acc);
-    double m = 0.0;
+    Q[i] = diag_max(acc);
+    for (int k = 0; k < np[i]; ++k) {
+        const PrimPair* pk = prim + poff[i] + k;
 #pragma unroll
-    for (int fa = 0; fa < S::NA; ++fa)
-#pragma unroll
-        for (int fb = 0; fb < S::NB; ++fb)
-            m = fmax(m, fabs(acc[((fa * S::NB + fb) * S::NA + fa) * S::NB + fb]));
-    Q[i] = sqrt(m);
+        for (int e = 0; e < S::NINT; ++e) acc[e] = 0.0;
+        eri_shell_quartet<LA, LB, LA, LB, double>(pk, 1, pk, 1, A, B, A, B, acc);
+        Qp[poff[i] + k] = diag_max(acc);
+    }
 }
 
 
@@ -222,9 +238,10 @@ __global__ __launch_bounds__(256) void k_eri_split(const ClassParams cp, double*
     for (int k = 0; k < NK; ++k) acc[k] = (R)0;
     static_switch<0, NS>(s, [&](auto sc) {
         constexpr int FA = decltype(sc)::value / S::NB, FB = decltype(sc)::value % S::NB;
-        eri_bra_component<LA, LB, LC, LD, FA, FB, R>(cp.prim + cp.bra.poff[i], cp.bra.np[i],
-                                                     cp.prim + cp.ket.poff[j], cp.ket.np[j], A, B,
-                                                     C, D, acc, nt);
+        constexpr bool PS = std::is_same<R, float>::value;
+        eri_bra_component<LA, LB, LC, LD, FA, FB, R, PS>(
+            cp.prim + cp.bra.poff[i], cp.bra.np[i], cp.prim + cp.ket.poff[j], cp.ket.np[j], A, B, C,
+            D, acc, nt, cp.primq + cp.bra.poff[i], cp.primq + cp.ket.poff[j], cp.tau_prim);
     });
 #pragma unroll
     for (int k = 0; k < NK; ++k) raw[(long long)(s * NK + k) * cp.qstride + qa] = (double)acc[k];
@@ -410,12 +427,13 @@ static int launch_class(int fp32, int mode, const ClassParams& cp, const Tune& t
 
 template <int LA, int LB, int LC, int LD>
 static int launch_schwarz_t(const int* sa, const int* sb, const int* poff, const int* np, int n,
-                            const PrimPair* prim, const double* xyz, double* Q, hipStream_t s)
+                            const PrimPair* prim, const double* xyz, double* Q, double* Qp,
+                            hipStream_t s)
 {
     if constexpr (LA == LC && LB == LD) {
         if (n <= 0) return kOk;
         hipLaunchKernelGGL((k_schwarz<LA, LB>), dim3((n + 127) / 128), dim3(128), 0, s, sa, sb, poff,
-                           np, n, prim, xyz, Q);
+                           np, n, prim, xyz, Q, Qp);
         SF_HIP(hipGetLastError());
         return kOk;
     } else {
@@ -431,9 +449,10 @@ static int launch_schwarz_t(const int* sa, const int* sb, const int* poff, const
         return launch_class<a, b, c, d>(fp32, mode, cp, tn, s);                                \
     }                                                                                          \
     int launch_schwarz_##id(const int* sa, const int* sb, const int* poff, const int* np, int n, \
-                            const PrimPair* prim, const double* xyz, double* Q, hipStream_t s)  \
+                            const PrimPair* prim, const double* xyz, double* Q, double* Qp,     \
+                            hipStream_t s)                                                     \
     {                                                                                          \
-        return launch_schwarz_t<a, b, c, d>(sa, sb, poff, np, n, prim, xyz, Q, s);             \
+        return launch_schwarz_t<a, b, c, d>(sa, sb, poff, np, n, prim, xyz, Q, Qp, s);         \
     }                                                                                          \
     int launch_grad_##id(const ClassParams& cp, double* gout, hipStream_t s)                   \
     {                                                                                          \
@@ -637,7 +656,8 @@ void class_tile_offsets(int qcls, int off[6])
 }
 
 int launch_schwarz(int pc, const int* sa, const int* sb, const int* poff, const int* np, int n,
-                   const PrimPair* prim, const double* shell_xyz, double* Q, hipStream_t s)
+                   const PrimPair* prim, const double* shell_xyz, double* Q, double* Qp,
+                   hipStream_t s)
 {
     // the diagonal quartet class (pc|pc) holds the pair class's Schwarz kernel
     static const SchwarzLaunch diag[kNumPairClass] = {launch_schwarz_0, launch_schwarz_2,
@@ -647,7 +667,7 @@ int launch_schwarz(int pc, const int* sa, const int* sb, const int* poff, const 
         set_error(kEUnsupported, "pair class %d", pc);
         return kEUnsupported;
     }
-    return diag[pc](sa, sb, poff, np, n, prim, shell_xyz, Q, s);
+    return diag[pc](sa, sb, poff, np, n, prim, shell_xyz, Q, Qp, s);
 }
 
 #endif  // SF_CLASS

@@ -147,7 +147,8 @@ struct sf_basis {
     long long extent = 0;                  // elements spanned by the layout
     std::vector<double> xyz, expn, coef, fnorm;
     std::vector<PrimPair> prim;
-    DevBuf d_prim, d_xyz, d_foff, d_fnorm, d_zero;
+    std::vector<double> prim_q;            // Schwarz factor of each primitive pair
+    DevBuf d_prim, d_xyz, d_foff, d_fnorm, d_zero, d_prim_q;
     DevBuf d_row, d_col, d_ld, d_sh_mol, d_sh_loc, d_mol_ns, d_mol_kbase;
     DevBuf d_mol_off, d_mol_ld, d_mol_n, d_mol_fbase, d_mol_nuc;
     // nuclei (point charges) for the one-electron potential
@@ -161,6 +162,10 @@ struct sf_basis {
     std::vector<int> pair_slot;  // one molecule: pair (a>=b) -> (pc << 24) | position
     bool have_schwarz = false, screened = false;
     double tau = 0.0, tau_mixed = 0.0;
+    // primitive-quartet threshold of the FP32 quartets: tau_prim_set < 0
+    // means 1e-8 * tau_mixed (about FP32 resolution of the largest FP32
+    // quartet), else the value set by sf_set_prim_tau
+    double tau_prim_set = -1.0, tau_prim = 0.0;
     QuartetClassData qc[kNumQuartetClass];
     long long funcq = 0, primq = 0;
     // digestion plan: destination-sorted tile slots
@@ -620,16 +625,20 @@ extern "C" int sf_basis_nbf(const sf_basis* b, int* nbf)
     return kOk;
 }
 
+// Schwarz factors of every shell pair and of every primitive pair; then the
+// primitive pairs of each shell pair are reordered by their own factor,
+// largest first, so that screened primitive loops can stop early.
 static int compute_schwarz(sf_basis* b)
 {
     if (b->have_schwarz) return kOk;
+    SF_TRY(b->d_prim_q.ensure(std::max<size_t>(1, b->prim.size()) * sizeof(double)));
     for (int p = 0; p < kNumPairClass; ++p) {
         PairClassData& P = b->pc[p];
         const int n = (int)P.sa.size();
         if (!n) continue;
         SF_TRY(launch_schwarz(p, P.d_sa.as<int>(), P.d_sb.as<int>(), P.d_poff.as<int>(),
                               P.d_np.as<int>(), n, b->d_prim.as<PrimPair>(), b->d_xyz.as<double>(),
-                              P.d_Q.as<double>(), 0));
+                              P.d_Q.as<double>(), b->d_prim_q.as<double>(), 0));
     }
     SF_HIP(hipDeviceSynchronize());
     for (int p = 0; p < kNumPairClass; ++p) {
@@ -638,6 +647,29 @@ static int compute_schwarz(sf_basis* b)
         if (!P.Q.empty())
             SF_HIP(hipMemcpy(P.Q.data(), P.d_Q.p, P.Q.size() * sizeof(double), hipMemcpyDeviceToHost));
     }
+    b->prim_q.resize(b->prim.size());
+    if (!b->prim.empty())
+        SF_HIP(hipMemcpy(b->prim_q.data(), b->d_prim_q.p, b->prim_q.size() * sizeof(double),
+                         hipMemcpyDeviceToHost));
+    std::vector<int> order;
+    std::vector<PrimPair> pp;
+    std::vector<double> qq;
+    for (auto& P : b->pc)
+        for (size_t i = 0; i < P.sa.size(); ++i) {
+            const int o = P.poff[i], n = P.np[i];
+            order.resize(n);
+            std::iota(order.begin(), order.end(), 0);
+            std::stable_sort(order.begin(), order.end(),
+                             [&](int x, int y) { return b->prim_q[o + x] > b->prim_q[o + y]; });
+            pp.assign(b->prim.begin() + o, b->prim.begin() + o + n);
+            qq.assign(b->prim_q.begin() + o, b->prim_q.begin() + o + n);
+            for (int k = 0; k < n; ++k) {
+                b->prim[o + k] = pp[order[k]];
+                b->prim_q[o + k] = qq[order[k]];
+            }
+        }
+    SF_TRY(b->d_prim.upload(b->prim));
+    SF_TRY(b->d_prim_q.upload(b->prim_q.empty() ? std::vector<double>(1, 0.0) : b->prim_q));
     b->have_schwarz = true;
     return kOk;
 }
@@ -679,6 +711,7 @@ extern "C" int sf_screen(sf_basis* b, double tau, double tau_mixed)
     b->grad_ready = false;
     b->tau = tau;
     b->tau_mixed = tau_mixed;
+    b->tau_prim = b->tau_prim_set >= 0.0 ? b->tau_prim_set : 1e-8 * tau_mixed;
     const int nmol = b->nmol;
     // largest Schwarz factor of each molecule
     std::vector<double> qmax(nmol, 0.0);
@@ -895,6 +928,18 @@ extern "C" int sf_screen(sf_basis* b, double tau, double tau_mixed)
     return kOk;
 }
 
+extern "C" int sf_set_prim_tau(sf_basis* b, double tau_prim)
+{
+    if (!b || tau_prim != tau_prim) {
+        set_error(kEInval, "sf_set_prim_tau: bad arguments");
+        return kEInval;
+    }
+    b->tau_prim_set = tau_prim < 0.0 ? -1.0 : tau_prim;
+    if (b->screened) b->tau_prim = b->tau_prim_set >= 0.0 ? b->tau_prim_set : 1e-8 * b->tau_mixed;
+    b->drop_graph();
+    return kOk;
+}
+
 extern "C" int sf_quartet_counts(const sf_basis* b, long long* counts, long long* funcq,
                                  long long* primq)
 {
@@ -973,6 +1018,8 @@ static ClassParams class_params(sf_basis* b, int c, const double* Ps)
     cp.qstride = Q.nq_all;
     cp.same = Q.same;
     cp.P = Ps;
+    cp.primq = b->d_prim_q.as<double>();
+    cp.tau_prim = b->tau_prim;
     cp.out = nullptr;
     cp.raw = Q.d_raw.as<double>();
     cp.pos = Q.d_pos.as<unsigned>();
@@ -1203,6 +1250,8 @@ extern "C" int sf_jk_graph(sf_basis* b, const double* P, double* J, double* K, i
 // (grid 0) or a persistent grid of 2 or 8 blocks per compute unit.  The
 // class kernel runs once untimed (caches, code load), then each candidate
 // once between HIP events on the stream; the fastest wins.
+static constexpr long long kTuneQuartets = 1LL << 20;
+
 extern "C" int sf_autotune(sf_basis* b, const double* P, int precision, void* stream, int* shapes,
                            double* ms)
 {
@@ -1232,6 +1281,10 @@ extern "C" int sf_autotune(sf_basis* b, const double* P, int precision, void* st
             set_segment(b, c, precision, seg, cp, b->d_zero.as<int>());
             if (ms) ms[2 * c + seg] = 0.0;
             if (Q.nq_all == 0 || cp.nq == 0) continue;
+            // candidates are timed on a prefix of the segment: the ranking of
+            // launch shapes does not need every quartet, and this bounds the
+            // tuning time for the largest classes
+            cp.nq = std::min(cp.nq, kTuneQuartets);
             // work items of one launch: quartets, or waves x component slices
             const long long items = class_split(c) ? ((cp.nq + 63) / 64) * 64 * class_nslice(c) : cp.nq;
             Tune best = b->tune[c][seg];

@@ -82,6 +82,8 @@ struct ClassParams {
     long long qstride;         // quartets in the class (plane stride of out)
     int same;                  // bra and ket pair classes coincide
     const double* P;           // density in the shell-normalised basis
+    const double* primq;       // Schwarz factor of each primitive pair (same index as prim)
+    double tau_prim;           // FP32 segments: skip primitive quartets with Qp_b Qp_k < tau_prim
     double* out;               // kModeRaw: SoA planes out[k * qstride + q]
     double* raw;               // split classes: raw integral planes (NINT * qstride)
     const unsigned* pos;       // kModeJK: pos[t * qstride + q] = slot of tile t in `tiles`
@@ -133,7 +135,8 @@ struct KeyMap {
 int launch_eri_class(int qcls, int fp32, int mode, const ClassParams& cp, const Tune& tn,
                      hipStream_t s);
 int launch_schwarz(int pc, const int* sa, const int* sb, const int* poff, const int* np, int n,
-                   const PrimPair* prim, const double* shell_xyz, double* Q, hipStream_t s);
+                   const PrimPair* prim, const double* shell_xyz, double* Q, double* Qp,
+                   hipStream_t s);
 int class_ntile(int qcls);
 int class_nint(int qcls);
 int class_split(int qcls);

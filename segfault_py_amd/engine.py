@@ -89,6 +89,7 @@ def lib():
     L.sf_jk_graph.argtypes = [vp, vp, vp, vp, ctypes.c_int, vp]
     L.sf_autotune.argtypes = [vp, vp, ctypes.c_int, vp, _i32p, _f64p]
     L.sf_set_tuning.argtypes = [vp, _i32p]
+    L.sf_set_prim_tau.argtypes = [vp, ctypes.c_double]
     _LIB = L
     return L
 
@@ -105,7 +106,7 @@ def exported_symbols() -> list[str]:
             "sf_schwarz", "sf_screen", "sf_quartet_counts", "sf_pair_list", "sf_quartet_list",
             "sf_eri_dense", "sf_jk", "sf_jk_profile", "sf_basis_set_nuclei", "sf_one_electron",
             "sf_grad", "sf_basis_create_batch", "sf_basis_set_layout", "sf_basis_layout",
-            "sf_jk_graph", "sf_autotune", "sf_set_tuning"]
+            "sf_jk_graph", "sf_autotune", "sf_set_tuning", "sf_set_prim_tau"]
 
 
 def _torch():
@@ -362,6 +363,12 @@ class Basis:
                                 _stream_ptr(torch), shapes, ms))
         torch.cuda.synchronize()
         return shapes.reshape(NUM_QUARTET_CLASSES, 2, 2), ms.reshape(NUM_QUARTET_CLASSES, 2)
+
+    def set_prim_tau(self, tau_prim: float) -> "Basis":
+        """Primitive-quartet threshold of the FP32 quartets (< 0: default
+        1e-8 * tau_mixed, 0: off); see sf_set_prim_tau."""
+        check(lib().sf_set_prim_tau(self._h, float(tau_prim)))
+        return self
 
     def set_tuning(self, shapes):
         check(lib().sf_set_tuning(self._h, np.ascontiguousarray(shapes, dtype=np.int32).reshape(-1)))

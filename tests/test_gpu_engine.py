@@ -158,6 +158,28 @@ def test_jk_deterministic_and_mixed_close(gpu):
     assert counts[:, 2].sum() > 0
 
 
+def test_mixed_primitive_screening(gpu):
+    """FP32 quartets skip primitive quartets below tau_prim (default
+    1e-8 * tau_mixed): against the same FP32 build without that screening,
+    J and K move by less than 1e-9 and the two-electron energy by less than
+    1e-8 Eh; FP64 builds are unaffected by the setting (bitwise)."""
+    from segfault_py_amd import engine, workloads as W
+    mol = W.config_molecule(3)
+    b = engine.Basis(mol).screen(1e-12, 1e-5)
+    _, P = _density(mol.nbf, 6, "cuda")
+    J64, K64 = b.jk(P, 0)
+    Jd, Kd = b.jk(P, 1)                   # default tau_prim = 1e-13
+    b.set_prim_tau(0.0)
+    J0, K0 = b.jk(P, 1)
+    J64b, K64b = b.jk(P, 0)
+    b.set_prim_tau(-1.0)
+    assert gpu.equal(J64, J64b) and gpu.equal(K64, K64b)
+    assert float((Jd - J0).abs().max()) < 1e-9 and float((Kd - K0).abs().max()) < 1e-9
+    e = lambda J, K: float(0.5 * (P * (J - 0.5 * K)).sum())  # noqa: E731
+    assert abs(e(Jd, Kd) - e(J0, K0)) < 1e-8
+    assert abs(e(Jd, Kd) - e(J64, K64)) < 1e-6
+
+
 @pytest.mark.parametrize("name,group", [("water", "others"), ("methane", "alkane")])
 def test_scf_energy_with_gpu_jk(gpu, oracle, name, group):
     """SCF with GPU J/K (one-electron integrals from the oracle as inputs)
