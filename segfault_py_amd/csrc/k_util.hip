@@ -229,31 +229,39 @@ int launch_plan_positions(const unsigned int* keys, const unsigned long long* va
     return kOk;
 }
 
-// One wavefront per destination shell pair (four per 256-thread block).  The
-// tiles of a destination are contiguous (NE doubles each), so lanes read
-// them coalesced; a fixed strided assignment of tiles to lanes and a fixed
-// xor butterfly make the sum bitwise reproducible without atomics.
+// Destination reduction, pass 1: one wavefront per chunk (four per
+// 256-thread block).  A chunk's tiles are contiguous (NE doubles each), so
+// lanes read them coalesced; a fixed strided assignment of tiles to lanes
+// and a fixed xor butterfly make the sum bitwise reproducible without
+// atomics.  A chunk that is its destination's only one writes the result.
 template <int N1, int N2>
-__global__ __launch_bounds__(256) void k_reduce(const DestList dl,
-                                                const long long* __restrict__ seg_begin,
-                                                const long long* __restrict__ seg_end,
-                                                const unsigned long long* __restrict__ key_base,
-                                                const double* __restrict__ tiles, const ShellMap map,
-                                                double* __restrict__ X, double* __restrict__ Y)
+__device__ __forceinline__ void store_block(const DestList& dl, int d, const ShellMap& map,
+                                            const double* acc, double* X, double* Y)
+{
+    const int s1 = dl.s1[d], s2 = dl.s2[d];
+    double* dst = (dl.xy[d] == 0 ? X : Y) + map.row[s1] + map.col[s2];
+    const long long ld = map.ld[s1];
+#pragma unroll
+    for (int k = 0; k < N1 * N2; ++k) dst[(k / N2) * ld + k % N2] = acc[k];
+}
+
+template <int N1, int N2>
+__global__ __launch_bounds__(256) void k_reduce_chunks(const DestList dl,
+                                                       const double* __restrict__ tiles,
+                                                       const ShellMap map, double* __restrict__ X,
+                                                       double* __restrict__ Y)
 {
     constexpr int NE = N1 * N2;
     const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (w >= dl.n) return;
+    if (w >= dl.nch) return;
     const int lane = threadIdx.x & 63;
-    const int key = dl.key[w];
-    const int s1 = dl.s1[w], s2 = dl.s2[w];
-    const long long cnt = seg_end[key] - seg_begin[key];
-    const double* src = tiles + key_base[key];
+    const int cnt = dl.ch_n[w];
+    const double* src = tiles + dl.ch_src[w];
     double acc[NE];
 #pragma unroll
     for (int k = 0; k < NE; ++k) acc[k] = 0.0;
-    for (long long e = lane; e < cnt; e += 64) {
-        const double* t = src + e * NE;
+    for (int e = lane; e < cnt; e += 64) {
+        const double* t = src + (long long)e * NE;
 #pragma unroll
         for (int k = 0; k < NE; ++k) acc[k] += t[k];
     }
@@ -262,24 +270,56 @@ __global__ __launch_bounds__(256) void k_reduce(const DestList dl,
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) acc[k] += __shfl_xor(acc[k], o, 64);
     if (lane == 0) {
-        double* dst = (dl.xy[w] == 0 ? X : Y) + map.row[s1] + map.col[s2];
-        const long long ld = map.ld[s1];
+        const int slot = dl.ch_part[w];
+        if (slot < 0) {
+            store_block<N1, N2>(dl, dl.ch_dest[w], map, acc, X, Y);
+        } else {
 #pragma unroll
-        for (int k = 0; k < NE; ++k) dst[(k / N2) * ld + k % N2] = acc[k];
+            for (int k = 0; k < NE; ++k) dl.part[(long long)slot * NE + k] = acc[k];
+        }
     }
 }
 
-int launch_reduce(int n1, int n2, const DestList& dl, const long long* seg_begin,
-                  const long long* seg_end, const unsigned long long* key_base,
-                  const double* tiles, const ShellMap& map, double* X, double* Y, hipStream_t s)
+// Pass 2: one wavefront per destination with several chunks adds their
+// partial sums (lanes strided over the chunks, fixed butterfly).
+template <int N1, int N2>
+__global__ __launch_bounds__(256) void k_reduce_partials(const DestList dl, const ShellMap map,
+                                                         double* __restrict__ X,
+                                                         double* __restrict__ Y)
 {
-    if (dl.n <= 0) return kOk;
-    dim3 g((dl.n + 3) / 4), b(256);
+    constexpr int NE = N1 * N2;
+    const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (w >= dl.nm) return;
+    const int lane = threadIdx.x & 63;
+    const double* src = dl.part + (long long)dl.m_pbase[w] * NE;
+    const int cnt = dl.m_nch[w];
+    double acc[NE];
+#pragma unroll
+    for (int k = 0; k < NE; ++k) acc[k] = 0.0;
+    for (int e = lane; e < cnt; e += 64) {
+#pragma unroll
+        for (int k = 0; k < NE; ++k) acc[k] += src[(long long)e * NE + k];
+    }
+#pragma unroll
+    for (int k = 0; k < NE; ++k)
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) acc[k] += __shfl_xor(acc[k], o, 64);
+    if (lane == 0) store_block<N1, N2>(dl, dl.m_dest[w], map, acc, X, Y);
+}
+
+int launch_reduce(int n1, int n2, const DestList& dl, const double* tiles, const ShellMap& map,
+                  double* X, double* Y, hipStream_t s)
+{
+    if (dl.nch <= 0) return kOk;
+    dim3 g1((dl.nch + 3) / 4), g2((dl.nm + 3) / 4), b(256);
 #define R_(a, c)                                                                              \
     if (n1 == a && n2 == c) {                                                                 \
-        hipLaunchKernelGGL((k_reduce<a, c>), g, b, 0, s, dl, seg_begin, seg_end, key_base,      \
-                           tiles, map, X, Y);                                                 \
+        hipLaunchKernelGGL((k_reduce_chunks<a, c>), g1, b, 0, s, dl, tiles, map, X, Y);       \
         SF_HIP(hipGetLastError());                                                            \
+        if (dl.nm > 0) {                                                                      \
+            hipLaunchKernelGGL((k_reduce_partials<a, c>), g2, b, 0, s, dl, map, X, Y);        \
+            SF_HIP(hipGetLastError());                                                        \
+        }                                                                                     \
         return kOk;                                                                           \
     }
     R_(1, 1) R_(1, 3) R_(1, 6) R_(3, 1) R_(3, 3) R_(3, 6) R_(6, 1) R_(6, 3) R_(6, 6)

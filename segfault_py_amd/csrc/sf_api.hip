@@ -170,8 +170,11 @@ struct sf_basis {
     long long funcq = 0, primq = 0;
     // digestion plan: destination-sorted tile slots
     DevBuf d_segb, d_sege, d_key_base, d_tiles;
-    DevBuf d_dkey[9], d_ds1[9], d_ds2[9], d_dxy[9];
-    int ndest[9] = {0};
+    // reduction lists per tile shape g = 3 L1 + L2 (see DestList)
+    DevBuf d_ds1[9], d_ds2[9], d_dxy[9], d_ch_src[9], d_ch_n[9], d_ch_dest[9], d_ch_part[9];
+    DevBuf d_m_dest[9], d_m_pbase[9], d_m_nch[9], d_part;
+    int ndest[9] = {0}, nchunk[9] = {0}, nmulti[9] = {0};
+    long long part_off[9] = {0};
     unsigned long long ntile_total = 0;
     DevBuf d_Ps, d_X, d_Y;
     // launch shapes per quartet class and precision segment (sf_autotune)
@@ -880,7 +883,10 @@ extern "C" int sf_screen(sf_basis* b, double tau, double tau_mixed)
     // slot layout: destinations in key order, each a run of NE-double tiles
     std::vector<unsigned long long> key_base(nkeys);
     std::vector<int> key_ne(nkeys);
-    std::vector<int> dkey[9], ds1[9], ds2[9], dxy[9];
+    std::vector<int> ds1[9], ds2[9], dxy[9], ch_n[9], ch_dest[9], ch_part[9];
+    std::vector<unsigned long long> ch_src[9];
+    std::vector<int> m_dest[9], m_pbase[9], m_nch[9];
+    int npart[9] = {0};
     unsigned long long acc = 0;
     for (int m = 0; m < nmol_; ++m) {
         const int s0 = b->mol_shell[m];
@@ -892,12 +898,27 @@ extern "C" int sf_screen(sf_basis* b, double tau, double tau_mixed)
             key_ne[k] = cart_n(b->L[s1]) * cart_n(b->L[s2]);
             key_base[k] = acc;
             if (sege[k] <= segb[k]) continue;
-            acc += (unsigned long long)(sege[k] - segb[k]) * key_ne[k];
+            const long long cnt = sege[k] - segb[k];
+            acc += (unsigned long long)cnt * key_ne[k];
             const int g = 3 * b->L[s1] + b->L[s2];
-            dkey[g].push_back((int)k);
+            const int d = (int)ds1[g].size();
             ds1[g].push_back(s1);
             ds2[g].push_back(s2);
             dxy[g].push_back(xy);
+            // chunks of at most kReduceChunk tiles; several -> partial slots
+            const int nch = (int)((cnt + kReduceChunk - 1) / kReduceChunk);
+            if (nch > 1) {
+                m_dest[g].push_back(d);
+                m_pbase[g].push_back(npart[g]);
+                m_nch[g].push_back(nch);
+            }
+            for (int c = 0; c < nch; ++c) {
+                ch_src[g].push_back(key_base[k] + (unsigned long long)c * kReduceChunk * key_ne[k]);
+                ch_n[g].push_back((int)std::min<long long>(kReduceChunk, cnt - (long long)c * kReduceChunk));
+                ch_dest[g].push_back(d);
+                ch_part[g].push_back(nch > 1 ? npart[g] + c : -1);
+            }
+            if (nch > 1) npart[g] += nch;
         }
     }
     if (acc >= (1ULL << 32)) {
@@ -916,13 +937,26 @@ extern "C" int sf_screen(sf_basis* b, double tau, double tau_mixed)
     d_keys2.release();
     d_vals2.release();
     SF_TRY(b->d_tiles.ensure(std::max<size_t>(1, acc) * sizeof(double)));
+    long long part_total = 0;
+    const int nf3[3] = {1, 3, 6};
     for (int g = 0; g < 9; ++g) {
-        b->ndest[g] = (int)dkey[g].size();
-        SF_TRY(b->d_dkey[g].upload(dkey[g]));
+        b->ndest[g] = (int)ds1[g].size();
+        b->nchunk[g] = (int)ch_n[g].size();
+        b->nmulti[g] = (int)m_dest[g].size();
+        b->part_off[g] = part_total;
+        part_total += (long long)npart[g] * nf3[g / 3] * nf3[g % 3];
         SF_TRY(b->d_ds1[g].upload(ds1[g]));
         SF_TRY(b->d_ds2[g].upload(ds2[g]));
         SF_TRY(b->d_dxy[g].upload(dxy[g]));
+        SF_TRY(b->d_ch_src[g].upload(ch_src[g]));
+        SF_TRY(b->d_ch_n[g].upload(ch_n[g]));
+        SF_TRY(b->d_ch_dest[g].upload(ch_dest[g]));
+        SF_TRY(b->d_ch_part[g].upload(ch_part[g]));
+        SF_TRY(b->d_m_dest[g].upload(m_dest[g]));
+        SF_TRY(b->d_m_pbase[g].upload(m_pbase[g]));
+        SF_TRY(b->d_m_nch[g].upload(m_nch[g]));
     }
+    SF_TRY(b->d_part.ensure((size_t)std::max(1LL, part_total) * sizeof(double)));
     SF_HIP(hipDeviceSynchronize());
     b->screened = true;
     return kOk;
@@ -1163,15 +1197,21 @@ static int run_jk(sf_basis* b, const double* P, double* J, double* K, int precis
     const int nf[3] = {1, 3, 6};
     for (int g = 0; g < 9; ++g) {
         DestList dl;
-        dl.key = b->d_dkey[g].as<int>();
         dl.s1 = b->d_ds1[g].as<int>();
         dl.s2 = b->d_ds2[g].as<int>();
         dl.xy = b->d_dxy[g].as<int>();
-        dl.n = b->ndest[g];
-        SF_TRY(launch_reduce(nf[g / 3], nf[g % 3], dl, b->d_segb.as<long long>(),
-                             b->d_sege.as<long long>(), b->d_key_base.as<unsigned long long>(),
-                             b->d_tiles.as<double>(), b->map(), b->d_X.as<double>(),
-                             b->d_Y.as<double>(), s));
+        dl.ch_src = b->d_ch_src[g].as<unsigned long long>();
+        dl.ch_n = b->d_ch_n[g].as<int>();
+        dl.ch_dest = b->d_ch_dest[g].as<int>();
+        dl.ch_part = b->d_ch_part[g].as<int>();
+        dl.nch = b->nchunk[g];
+        dl.m_dest = b->d_m_dest[g].as<int>();
+        dl.m_pbase = b->d_m_pbase[g].as<int>();
+        dl.m_nch = b->d_m_nch[g].as<int>();
+        dl.nm = b->nmulti[g];
+        dl.part = b->d_part.as<double>() + b->part_off[g];
+        SF_TRY(launch_reduce(nf[g / 3], nf[g % 3], dl, b->d_tiles.as<double>(), b->map(),
+                             b->d_X.as<double>(), b->d_Y.as<double>(), s));
     }
     SF_TRY(launch_finalize(b->d_X.as<double>(), b->d_Y.as<double>(), b->d_fnorm.as<double>(), ml, J,
                            K, s));

@@ -110,15 +110,29 @@ struct MolLayout {
     int nmax;
 };
 
-// Digestion destinations of one tile shape: key (plan segment), the two
-// shells of the destination block and which matrix it belongs to (0 = X
-// for J, 1 = Y for K).
+// Digestion destinations of one tile shape (N1 x N2 blocks): the two
+// shells of each destination block and which matrix it belongs to (0 = X
+// for J, 1 = Y for K).  A destination's tiles (a contiguous run in the
+// destination-sorted tile buffer) are summed in chunks of at most
+// kReduceChunk tiles, one wavefront per chunk; a destination with several
+// chunks gets partial sums (slots ch_part) that a second pass adds in chunk
+// order.  Both passes have a fixed order: the result is reproducible bit
+// for bit.
+constexpr int kReduceChunk = 256;
 struct DestList {
-    const int* key;
     const int* s1;
     const int* s2;
     const int* xy;
-    int n;
+    const unsigned long long* ch_src;  // per chunk: first double of its first tile
+    const int* ch_n;                   // tiles in the chunk
+    const int* ch_dest;                // destination of the chunk
+    const int* ch_part;                // partial slot, -1 = the chunk is its whole destination
+    int nch;
+    const int* m_dest;                 // destinations with several chunks
+    const int* m_pbase;                //   their first partial slot
+    const int* m_nch;                  //   and chunk count
+    int nm;
+    double* part;                      // partial sums, N1*N2 doubles per slot
 };
 
 // Shell -> molecule bookkeeping for plan keys: molecule of each shell, its
@@ -171,9 +185,8 @@ int launch_segments(const unsigned int* keys, long long n, long long* seg_begin,
 int launch_plan_positions(const unsigned int* keys, const unsigned long long* vals, long long n,
                           const long long* seg_begin, const unsigned long long* key_base,
                           const int* key_ne, const ClassTab* tab, hipStream_t s);
-int launch_reduce(int n1, int n2, const DestList& dl, const long long* seg_begin,
-                  const long long* seg_end, const unsigned long long* key_base,
-                  const double* tiles, const ShellMap& map, double* X, double* Y, hipStream_t s);
+int launch_reduce(int n1, int n2, const DestList& dl, const double* tiles, const ShellMap& map,
+                  double* X, double* Y, hipStream_t s);
 int launch_scale_density(const double* P, const double* fnorm, const MolLayout& ml, double* Ps,
                          hipStream_t s);
 int launch_finalize(const double* X, const double* Y, const double* fnorm, const MolLayout& ml,
