@@ -172,6 +172,13 @@ int sf_autotune(sf_basis *b, const double *P, int precision, void *stream, int *
                 double *ms);
 /* Set the shapes directly, in the layout sf_autotune reports. */
 int sf_set_tuning(sf_basis *b, const int *shapes);
+/* Integral engine of quartet class qcls (0..20, or -1 for all): 0 = Rys
+ * quadrature (default; the method of rys.eri_rys, core.f90:917-1002),
+ * 1 = McMurchie-Davidson (the method of mmd.eri, core.f90:100-198) with the
+ * Hermite-to-Cartesian E.R.E transforms on the FP64 matrix cores.  Engine 1
+ * computes in FP64 in both precision segments.  Used by sf_jk, sf_jk_graph,
+ * sf_autotune and sf_eri_dense. */
+int sf_set_engine(sf_basis *b, int qcls, int engine);
 
 /* Nuclear gradient of the RHF electronic energy (host output grad[3*natom],
  * hartree/bohr, atom order of the basis): sum P dH/dR - sum W dS/dR +

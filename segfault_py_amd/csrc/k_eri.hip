@@ -13,6 +13,7 @@
 // Replaces the per-quartet Python fan-out + f2py call of hf.py:18-61 and
 // the einsum digestion of hf.py:300; the integral math follows
 // rys.eri_rys (src/hf/core/core.f90:917-1002) -- see eri_core.hpp.
+#include "eri_mmd.hpp"
 #include "sf_internal.hpp"
 
 // Built once per quartet class with -DSF_CLASS=<id> (that class's kernels
@@ -34,7 +35,8 @@ using SchwarzLaunch = int (*)(const int*, const int*, const int*, const int*, in
                           hipStream_t s);                                                      \
     int launch_schwarz_##id(const int*, const int*, const int*, const int*, int,               \
                             const PrimPair*, const double*, double*, double*, hipStream_t);    \
-    int launch_grad_##id(const ClassParams& cp, double* gout, hipStream_t s);
+    int launch_grad_##id(const ClassParams& cp, double* gout, hipStream_t s);                  \
+    int launch_mmd_##id(int mode, const ClassParams& cp, const Tune& tn, hipStream_t s);
 SF_FOR_EACH_CLASS(X)
 #undef X
 
@@ -425,6 +427,57 @@ static int launch_class(int fp32, int mode, const ClassParams& cp, const Tune& t
     }
 }
 
+// McMurchie-Davidson engine (eri_mmd.hpp): one wavefront per shell
+// quartet, blocks stride over the segment; raw integral planes.
+template <int LA, int LB, int LC, int LD>
+__global__ __launch_bounds__(64) void k_eri_mmd(const ClassParams cp, double* __restrict__ out)
+{
+    using M = MmdShape<LA, LB, LC, LD>;
+    __shared__ double sR[2][herm_n(M::L)];
+    __shared__ double sEab[3][M::EAB];
+    __shared__ double sEcd[3][M::ECD];
+    __shared__ double sT[M::TR * M::TC];
+    for (long long q = blockIdx.x; q < cp.nq; q += gridDim.x) {
+        const QuartetIdx ix = locate(cp, q, q);
+        const int i = ix.i, j = ix.j;
+        const int a = cp.bra.sa[i], b = cp.bra.sb[i];
+        const int c = cp.ket.sa[j], d = cp.ket.sb[j];
+        const double A[3] = {cp.shell_xyz[3 * a], cp.shell_xyz[3 * a + 1], cp.shell_xyz[3 * a + 2]};
+        const double B[3] = {cp.shell_xyz[3 * b], cp.shell_xyz[3 * b + 1], cp.shell_xyz[3 * b + 2]};
+        const double C[3] = {cp.shell_xyz[3 * c], cp.shell_xyz[3 * c + 1], cp.shell_xyz[3 * c + 2]};
+        const double D[3] = {cp.shell_xyz[3 * d], cp.shell_xyz[3 * d + 1], cp.shell_xyz[3 * d + 2]};
+        mmd_shell_quartet<LA, LB, LC, LD>(cp.prim + cp.bra.poff[i], cp.bra.np[i],
+                                          cp.prim + cp.ket.poff[j], cp.ket.np[j], A, B, C, D, out,
+                                          cp.qstride, ix.qa, sR, sEab, sEcd, sT);
+    }
+}
+
+// MMD integrals into raw planes, then (J/K mode) the raw-plane digestion.
+template <int LA, int LB, int LC, int LD>
+static int launch_mmd_t(int mode, const ClassParams& cp, const Tune& tn, hipStream_t s)
+{
+    if (cp.nq <= 0) return kOk;
+    SF_TRY_K(check_block(tn));
+    double* raw = (mode == kModeRaw) ? cp.out : cp.raw;
+    if (!raw) {
+        set_error(kEState, "MMD engine without a raw integral buffer");
+        return kEState;
+    }
+    long long nb = cp.nq;
+    const long long cap = tn.grid > 0 ? tn.grid : 16384;
+    if (nb > cap) nb = cap;
+    hipLaunchKernelGGL((k_eri_mmd<LA, LB, LC, LD>), dim3((unsigned)nb), dim3(64), 0, s, cp, raw);
+    SF_HIP(hipGetLastError());
+    if (mode == kModeJK) {
+        unsigned nb2 = 0;
+        SF_TRY_K(grid_blocks(((cp.nq + 63) / 64) * 6, 4, Tune(), &nb2));
+        hipLaunchKernelGGL((k_digest_split<LA, LB, LC, LD>), dim3(nb2), dim3(256), 0, s, cp,
+                           (const double*)raw);
+        SF_HIP(hipGetLastError());
+    }
+    return kOk;
+}
+
 template <int LA, int LB, int LC, int LD>
 static int launch_schwarz_t(const int* sa, const int* sb, const int* poff, const int* np, int n,
                             const PrimPair* prim, const double* xyz, double* Q, double* Qp,
@@ -457,6 +510,10 @@ static int launch_schwarz_t(const int* sa, const int* sb, const int* poff, const
     int launch_grad_##id(const ClassParams& cp, double* gout, hipStream_t s)                   \
     {                                                                                          \
         return launch_grad_t<a, b, c, d>(cp, gout, s);                                         \
+    }                                                                                          \
+    int launch_mmd_##id(int mode, const ClassParams& cp, const Tune& tn, hipStream_t s)        \
+    {                                                                                          \
+        return launch_mmd_t<a, b, c, d>(mode, cp, tn, s);                                      \
     }
 #define SF_PICK(id, a, b, c, d) SF_PICK_##id(id, a, b, c, d)
 #if SF_CLASS == 0
@@ -577,6 +634,18 @@ int launch_eri_class(int qcls, int fp32, int mode, const ClassParams& cp, const 
     switch (qcls) {
 #define X(id, a, b, c, d) \
     case id: return launch_class_##id(fp32, mode, cp, tn, s);
+        SF_FOR_EACH_CLASS(X)
+#undef X
+    }
+    set_error(kEUnsupported, "quartet class %d", qcls);
+    return kEUnsupported;
+}
+
+int launch_eri_mmd(int qcls, int mode, const ClassParams& cp, const Tune& tn, hipStream_t s)
+{
+    switch (qcls) {
+#define X(id, a, b, c, d) \
+    case id: return launch_mmd_##id(mode, cp, tn, s);
         SF_FOR_EACH_CLASS(X)
 #undef X
     }

@@ -179,6 +179,9 @@ struct sf_basis {
     DevBuf d_Ps, d_X, d_Y;
     // launch shapes per quartet class and precision segment (sf_autotune)
     Tune tune[kNumQuartetClass][2];
+    // integral engine per quartet class: 0 = Rys quadrature, 1 = McMurchie-
+    // Davidson with MFMA transforms (FP64 in both precision segments)
+    int engine[kNumQuartetClass] = {0};
     JKGraph graph;
     hipStream_t cap_stream = nullptr;
 
@@ -801,7 +804,7 @@ extern "C" int sf_screen(sf_basis* b, double tau, double tau_mixed)
             SF_TRY(Q.d_base32.upload(Q.base32));
             SF_TRY(Q.d_k032.upload(Q.n64.empty() ? std::vector<int>(1, 0) : Q.n64));
             SF_TRY(Q.d_pos.ensure((size_t)std::max(1LL, Q.nq_all) * 6 * sizeof(unsigned)));
-            if (class_split(c))
+            if (class_split(c) || b->engine[c] == 1)
                 SF_TRY(Q.d_raw.ensure((size_t)std::max(1LL, Q.nq_all) * class_nint(c) * sizeof(double)));
             else
                 Q.d_raw.release();
@@ -1035,6 +1038,31 @@ extern "C" int sf_quartet_list(const sf_basis* b, int c, int* nbra, int* n_all, 
     return kOk;
 }
 
+// One class/segment launch with the class's engine.
+static int launch_class_engine(sf_basis* b, int c, int seg, int mode, const ClassParams& cp,
+                               const Tune& tn, hipStream_t s)
+{
+    if (b->engine[c] == 1) return launch_eri_mmd(c, mode, cp, tn, s);
+    return launch_eri_class(c, seg, mode, cp, tn, s);
+}
+
+extern "C" int sf_set_engine(sf_basis* b, int qcls, int engine)
+{
+    if (!b || qcls < -1 || qcls >= kNumQuartetClass || engine < 0 || engine > 1) {
+        set_error(kEInval, "sf_set_engine: bad arguments");
+        return kEInval;
+    }
+    for (int c = 0; c < kNumQuartetClass; ++c) {
+        if (qcls >= 0 && c != qcls) continue;
+        b->engine[c] = engine;
+        QuartetClassData& Q = b->qc[c];
+        if (b->screened && engine == 1)
+            SF_TRY(Q.d_raw.ensure((size_t)std::max(1LL, Q.nq_all) * class_nint(c) * sizeof(double)));
+    }
+    b->drop_graph();
+    return kOk;
+}
+
 static ClassParams class_params(sf_basis* b, int c, const double* Ps)
 {
     QuartetClassData& Q = b->qc[c];
@@ -1112,7 +1140,7 @@ extern "C" int sf_eri_dense(sf_basis* b, double* eri)
         ClassParams cp = class_params(b, c, nullptr);
         set_segment(b, c, 0, 0, cp, b->d_zero.as<int>());
         cp.out = d_dense.as<double>();
-        SF_TRY(launch_eri_class(c, 0, kModeRaw, cp, Tune(), 0));
+        SF_TRY(launch_class_engine(b, c, 0, kModeRaw, cp, Tune(), 0));
         std::vector<double> raw((size_t)Q.nq_all * nint);
         SF_HIP(hipMemcpy(raw.data(), d_dense.p, raw.size() * sizeof(double), hipMemcpyDeviceToHost));
         const PairClassData& B = b->pc[Q.pcb];
@@ -1187,7 +1215,7 @@ static int run_jk(sf_basis* b, const double* P, double* J, double* K, int precis
         ClassParams cp = class_params(b, c, b->d_Ps.as<double>());
         for (int seg = 0; seg <= precision; ++seg) {
             set_segment(b, c, precision, seg, cp, b->d_zero.as<int>());
-            SF_TRY(launch_eri_class(c, seg, kModeJK, cp, b->tune[c][seg], s));
+            SF_TRY(launch_class_engine(b, c, seg, kModeJK, cp, b->tune[c][seg], s));
             SF_TRY(mark());
             slot.push_back(2 * c + seg);
         }
@@ -1331,7 +1359,7 @@ extern "C" int sf_autotune(sf_basis* b, const double* P, int precision, void* st
             float best_ms = 1e30f;
             const int blocks[3] = {64, 128, 256};
             const int mults[3] = {0, 2, 8};
-            rc = launch_eri_class(c, seg, kModeJK, cp, Tune(), s);  // untimed warm-up
+            rc = launch_class_engine(b, c, seg, kModeJK, cp, Tune(), s);  // untimed warm-up
             for (int bi = 0; bi < 3 && rc == kOk; ++bi)
                 for (int mi = 0; mi < 3 && rc == kOk; ++mi) {
                     Tune t;
@@ -1344,7 +1372,7 @@ extern "C" int sf_autotune(sf_basis* b, const double* P, int precision, void* st
                         rc = kEHip;
                         break;
                     }
-                    rc = launch_eri_class(c, seg, kModeJK, cp, t, s);
+                    rc = launch_class_engine(b, c, seg, kModeJK, cp, t, s);
                     if (rc) break;
                     float el = 0.f;
                     if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||

@@ -148,6 +148,9 @@ struct KeyMap {
 // Launchers (k_eri.hip).
 int launch_eri_class(int qcls, int fp32, int mode, const ClassParams& cp, const Tune& tn,
                      hipStream_t s);
+// McMurchie-Davidson engine (MFMA E.R.E), FP64: raw planes (kModeRaw into
+// cp.out) or raw planes in cp.raw plus digestion (kModeJK).
+int launch_eri_mmd(int qcls, int mode, const ClassParams& cp, const Tune& tn, hipStream_t s);
 int launch_schwarz(int pc, const int* sa, const int* sb, const int* poff, const int* np, int n,
                    const PrimPair* prim, const double* shell_xyz, double* Q, double* Qp,
                    hipStream_t s);
