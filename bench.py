@@ -190,6 +190,15 @@ def main():
     kernel_ms = {"eri_" + "".join(map(str, w["ls"])): round(t, 4) for t, w in top}
     kernel_ms["reduce+finalize"] = round(float(ms[43]), 4)
     kernel_ms["scale_density"] = round(float(ms[42]), 4)
+    # the McMurchie-Davidson engine (MFMA E.R.E) on the same build, per class
+    basis.set_engine(1)
+    basis.jk_profile(P, 0, out=(J, K))
+    _, _, ms_mmd = basis.jk_profile(P, 0, out=(J, K))
+    basis.set_engine(0)
+    engines = {"eri_" + "".join(map(str, w["ls"])): {"rys_ms": round(float(ms[2 * w["class"]]), 4),
+                                                      "mmd_mfma_ms": round(float(ms_mmd[2 * w["class"]]), 4)}
+               for w in work if w["shellq"] > 0}
+    engines["build_ms"] = {"rys": round(float(sum(ms[:44])), 3), "mmd_mfma": round(float(sum(ms_mmd[:44])), 3)}
     achieved = dom["flops"] / (dom_ms * 1e-3) / 1e12
     total_flops = sum(w["flops"] for w in work)
     shellq = sum(w["shellq"] for w in work)
@@ -224,6 +233,7 @@ def main():
         "quartets_per_build": {"function": funcq, "shell": shellq, "primitive": primq},
         "build_tflops_fp64": total_flops / t64 / 1e12,
         "kernel_ms_fp64": kernel_ms,
+        "engines_ms": engines,
         "setup_s": setup_s,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
