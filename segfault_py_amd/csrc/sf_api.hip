@@ -809,19 +809,25 @@ extern "C" int sf_screen(sf_basis* b, double tau, double tau_mixed)
             else
                 Q.d_raw.release();
             total_entries += 6 * Q.nq_all;
-            // function-level unique quartets and primitive quartets
+            // function-level unique quartets and primitive quartets, from
+            // prefix sums over the ket list (a bra's kets are a contiguous run)
             const int la = pair_la(pcb), lb = pair_lb(pcb), lc = pair_la(pck), ld = pair_lb(pck);
             const long long nA = cart_n(la), nB = cart_n(lb), nC = cart_n(lc), nD = cart_n(ld);
+            const size_t nk = K.s_sa.size();
+            std::vector<long long> cum_f(nk + 1, 0), cum_p(nk + 1, 0);
+            for (size_t j = 0; j < nk; ++j) {
+                const bool eqcd = K.s_sa[j] == K.s_sb[j];
+                cum_f[j + 1] = cum_f[j] + (eqcd ? nC * (nC + 1) / 2 : nC * nD);
+                cum_p[j + 1] = cum_p[j] + K.s_np[j];
+            }
             for (int i = 0; i < Q.nbra; ++i) {
                 const bool eqab = B.s_sa[i] == B.s_sb[i];
                 const long long pab = eqab ? nA * (nA + 1) / 2 : nA * nB;
-                for (int jr = 0; jr < Q.n_all[i]; ++jr) {
-                    const int j = Q.ket0[i] + jr;
-                    const bool eqcd = K.s_sa[j] == K.s_sb[j];
-                    const long long pcd = eqcd ? nC * (nC + 1) / 2 : nC * nD;
-                    b->funcq += (Q.same && i == j) ? pab * (pab + 1) / 2 : pab * pcd;
-                    b->primq += (long long)B.s_np[i] * K.s_np[j];
-                }
+                const int j0 = Q.ket0[i], j1 = Q.ket0[i] + Q.n_all[i];
+                b->funcq += pab * (cum_f[j1] - cum_f[j0]);
+                b->primq += (long long)B.s_np[i] * (cum_p[j1] - cum_p[j0]);
+                // the diagonal quartet (i|i) has pab (pab + 1) / 2 unique function quartets
+                if (Q.same && i >= j0 && i < j1) b->funcq -= pab * pab - pab * (pab + 1) / 2;
             }
         }
     // digestion plan: (destination, entry) pairs sorted by destination;
