@@ -60,6 +60,24 @@ struct PrimPair {
     double eb;   // exponent b_j of the second shell
 };
 
+// Primitive pair i of an array given by any pointer type (generic, or the
+// constant address space through which wave-uniform data is read with
+// scalar loads); field by field, so no address-space-qualified copy.
+template <typename Ptr>
+SF_HD SF_INLINE PrimPair load_prim(Ptr ptr, int i)
+{
+    PrimPair r;
+    r.p = ptr[i].p;
+    r.Px = ptr[i].Px;
+    r.Py = ptr[i].Py;
+    r.Pz = ptr[i].Pz;
+    r.K = ptr[i].K;
+    r.ip = ptr[i].ip;
+    r.Kp = ptr[i].Kp;
+    r.eb = ptr[i].eb;
+    return r;
+}
+
 // Where the four shells (a, b, c, d) of a quartet sit in an N x N matrix of
 // their molecule: component x of shell k and component y of shell k' are at
 // row[k] + x * ld + col[k'] + y.  For one molecule of nbf functions,
@@ -106,13 +124,14 @@ struct QuartetShape {
 // With PS (primitive screening), bq/kq are the primitive pairs' own Schwarz
 // factors, sorted descending within each shell pair, and primitive quartets
 // with bq * kq < tau_prim are skipped (both loops stop at the first one).
-template <int LA, int LB, int LC, int LD, typename R, bool PS = false>
-SF_HD SF_INLINE void eri_shell_quartet(const PrimPair* __restrict__ bra, int nbra,
+template <int LA, int LB, int LC, int LD, typename R, bool PS = false,
+          typename BraPtr = const PrimPair*, typename BqPtr = const double*>
+SF_HD SF_INLINE void eri_shell_quartet(BraPtr bra, int nbra,
                                        const PrimPair* __restrict__ ket, int nket,
                                        const double* A, const double* B, const double* C,
                                        const double* D, R* __restrict__ acc,
                                        const NodeTable& nt = global_node_table<QuartetShape<LA, LB, LC, LD>::NR>(),
-                                       const double* __restrict__ bq = nullptr,
+                                       BqPtr bq = nullptr,
                                        const double* __restrict__ kq = nullptr,
                                        double tau_prim = 0.0)
 {
@@ -142,7 +161,7 @@ SF_HD SF_INLINE void eri_shell_quartet(const PrimPair* __restrict__ bra, int nbr
             if constexpr (PS) {
                 if (bq[ib] * qk < tau_prim) break;
             }
-            const PrimPair pb = bra[ib];
+            const PrimPair pb = load_prim(bra, ib);
             const double p = pb.p;
             const double PA[3] = {pb.Px - A[0], pb.Py - A[1], pb.Pz - A[2]};
             const double pq = p + q;

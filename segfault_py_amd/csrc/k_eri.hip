@@ -115,21 +115,16 @@ __device__ __forceinline__ QuartetMap quartet_map(const ShellMap& m, int a, int 
     return qm;
 }
 
-// One thread per shell quartet; the blocks stride over the segment, so a
-// capped grid (Tune::grid) makes them persistent.
-template <int LA, int LB, int LC, int LD, typename R, int MODE>
-__global__ __launch_bounds__(256) void k_eri_class(const ClassParams cp)
+// One shell quartet (bra i, ket j, class-wide index qa): integrals, then
+// digestion tiles or raw planes.  When the caller passes a wave-uniform i
+// (readfirstlane), the bra pair's primitives, centres and density rows are
+// fetched with scalar loads.
+template <int LA, int LB, int LC, int LD, typename R, int MODE, bool UNI>
+__device__ __forceinline__ void class_quartet(const ClassParams& cp, const NodeTable& nt, int i,
+                                              int j, long long qa)
 {
     using S = QuartetShape<LA, LB, LC, LD>;
-    __shared__ double s_tab[node_table_lds(S::NR) > 0 ? node_table_lds(S::NR) : 1];
-    const NodeTable nt = stage_node_table<S::NR>(s_tab);
-    const long long stride = (long long)gridDim.x * blockDim.x;
-    const long long lane0 = (long long)blockIdx.x * blockDim.x + (threadIdx.x & ~63u);
-    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x, qw = lane0; q < cp.nq;
-         q += stride, qw += stride) {
-    const QuartetIdx ix = locate(cp, q, qw);
-    const int i = ix.i, j = ix.j;
-    const long long qa = ix.qa;
+    if constexpr (UNI) i = __builtin_amdgcn_readfirstlane(i);
     const int a = cp.bra.sa[i], b = cp.bra.sb[i];
     const int c = cp.ket.sa[j], d = cp.ket.sb[j];
     const double A[3] = {cp.shell_xyz[3 * a], cp.shell_xyz[3 * a + 1], cp.shell_xyz[3 * a + 2]};
@@ -141,10 +136,22 @@ __global__ __launch_bounds__(256) void k_eri_class(const ClassParams cp)
     for (int k = 0; k < S::NINT; ++k) acc[k] = (R)0;
     // FP32 quartets also skip primitive quartets below FP32 resolution
     constexpr bool PS = std::is_same<R, float>::value;
-    eri_shell_quartet<LA, LB, LC, LD, R, PS>(cp.prim + cp.bra.poff[i], cp.bra.np[i],
-                                             cp.prim + cp.ket.poff[j], cp.ket.np[j], A, B, C, D,
-                                             acc, nt, cp.primq + cp.bra.poff[i],
-                                             cp.primq + cp.ket.poff[j], cp.tau_prim);
+    int bpo = cp.bra.poff[i], bnp = cp.bra.np[i];
+    if constexpr (UNI) {
+        // the bra's primitive pairs are read-only for the whole launch: read
+        // them through the constant address space, i.e. with scalar loads
+        bpo = __builtin_amdgcn_readfirstlane(bpo);
+        bnp = __builtin_amdgcn_readfirstlane(bnp);
+        using CPrim = const __attribute__((address_space(4))) PrimPair*;
+        using CDbl = const __attribute__((address_space(4))) double*;
+        eri_shell_quartet<LA, LB, LC, LD, R, PS, CPrim, CDbl>(
+            (CPrim)(cp.prim + bpo), bnp, cp.prim + cp.ket.poff[j], cp.ket.np[j], A, B, C, D, acc,
+            nt, (CDbl)(cp.primq + bpo), cp.primq + cp.ket.poff[j], cp.tau_prim);
+    } else {
+        eri_shell_quartet<LA, LB, LC, LD, R, PS>(cp.prim + bpo, bnp, cp.prim + cp.ket.poff[j],
+                                                 cp.ket.np[j], A, B, C, D, acc, nt, cp.primq + bpo,
+                                                 cp.primq + cp.ket.poff[j], cp.tau_prim);
+    }
     if constexpr (MODE == kModeRaw) {
 #pragma unroll
         for (int k = 0; k < S::NINT; ++k) cp.out[k * cp.qstride + qa] = (double)acc[k];
@@ -164,6 +171,28 @@ __global__ __launch_bounds__(256) void k_eri_class(const ClassParams cp)
             for (int e = 0; e < S::tile_n1(T) * S::tile_n2(T); ++e) dst[e] = t[S::tile_off(T) + e];
         }
     }
+}
+
+// One thread per shell quartet; the blocks stride over the segment, so a
+// capped grid (Tune::grid) makes them persistent.  Consecutive quartets walk
+// the kets of one bra, so most wavefronts have a single bra: those take the
+// wave-uniform path.
+template <int LA, int LB, int LC, int LD, typename R, int MODE>
+__global__ __launch_bounds__(256) void k_eri_class(const ClassParams cp)
+{
+    using S = QuartetShape<LA, LB, LC, LD>;
+    __shared__ double s_tab[node_table_lds(S::NR) > 0 ? node_table_lds(S::NR) : 1];
+    const NodeTable nt = stage_node_table<S::NR>(s_tab);
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    const long long lane0 = (long long)blockIdx.x * blockDim.x + (threadIdx.x & ~63u);
+    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x, qw = lane0; q < cp.nq;
+         q += stride, qw += stride) {
+        const QuartetIdx ix = locate(cp, q, qw);
+        const int iw = __builtin_amdgcn_readfirstlane(ix.i);
+        if (__all(ix.i == iw))
+            class_quartet<LA, LB, LC, LD, R, MODE, true>(cp, nt, iw, ix.j, ix.qa);
+        else
+            class_quartet<LA, LB, LC, LD, R, MODE, false>(cp, nt, ix.i, ix.j, ix.qa);
     }
 }
 
