@@ -298,9 +298,10 @@ def alkane_workload(steps: int) -> dict:
 
 def gradient_workload() -> dict:
     """Config 4: ten waters (30 atoms, s/p/d basis): SCF + analytic RHF
-    gradient (derivative-ERI kernels) in FP64 and with the mixed-precision
-    SCF density; max gradient difference mixed vs FP64 (the finite-difference
-    checks are in tests/test_gpu_grad.py)."""
+    gradient (derivative-ERI kernels) all in FP64, and all in mixed precision
+    (SCF and derivative quartets below tau_mixed in FP32); max gradient
+    difference mixed vs FP64 (the finite-difference checks are in
+    tests/test_gpu_grad.py)."""
     import torch
     from segfault_py_amd import engine, grad as G, hf, workloads as W
 
@@ -312,7 +313,7 @@ def gradient_workload() -> dict:
         r = hf.scf_diis(mol, basis=b, precision=prec, tol=1e-10)
         torch.cuda.synchronize()
         t = time.perf_counter()
-        g[name] = G.rhf_gradient(mol, r, b)
+        g[name] = G.rhf_gradient(mol, r, b, prec)
         torch.cuda.synchronize()
         out[f"gradient_s_{name}"] = time.perf_counter() - t
         out[f"energy_{name}"] = r.energy

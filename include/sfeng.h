@@ -185,9 +185,12 @@ int sf_set_engine(sf_basis *b, int qcls, int engine);
  * 1/2 sum Gamma d(pq|rs)/dR for the density P and energy-weighted density
  * W = 2 sum_occ e_i C_i C_i^T (device buffers, N x N).  Derivative ERIs
  * over the screened quartet list of sf_screen, deterministic reductions.
+ * precision 0 = all FP64; 1 = derivative quartets below tau_mixed in FP32
+ * (nodes, weights and every sum in FP64), as sf_jk.
  * No counterpart in the reference (single-point energies only); nuclear
  * repulsion is not included.  Requires sf_basis_set_nuclei for every atom. */
-int sf_grad(sf_basis *b, const double *P, const double *W, double *grad, void *stream);
+int sf_grad(sf_basis *b, const double *P, const double *W, double *grad, int precision,
+            void *stream);
 
 #ifdef __cplusplus
 }

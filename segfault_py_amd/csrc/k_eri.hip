@@ -35,7 +35,7 @@ using SchwarzLaunch = int (*)(const int*, const int*, const int*, const int*, in
                           hipStream_t s);                                                      \
     int launch_schwarz_##id(const int*, const int*, const int*, const int*, int,               \
                             const PrimPair*, const double*, double*, double*, hipStream_t);    \
-    int launch_grad_##id(const ClassParams& cp, double* gout, hipStream_t s);                  \
+    int launch_grad_##id(int fp32, const ClassParams& cp, double* gout, hipStream_t s);        \
     int launch_mmd_##id(int mode, const ClassParams& cp, const Tune& tn, hipStream_t s);
 SF_FOR_EACH_CLASS(X)
 #undef X
@@ -317,7 +317,7 @@ __global__ __launch_bounds__(256) void k_digest_split(const ClassParams cp,
 // density factor fac * [4 P_ab P_cd - P_ac P_bd - P_ad P_bc] (the 8 images of
 // 1/2 sum Gamma_pqrs (pq|rs)^x with Gamma = P_pq P_rs - P_pr P_qs / 2) and
 // writes its 9 partial derivatives to gout[(s*9 + k) * qstride + q].
-template <int LA, int LB, int LC, int LD>
+template <int LA, int LB, int LC, int LD, typename R>
 __global__ __launch_bounds__(256) void k_grad_split(const ClassParams cp, double* __restrict__ gout)
 {
     using S = QuartetShape<LA, LB, LC, LD>;
@@ -357,16 +357,16 @@ __global__ __launch_bounds__(256) void k_grad_split(const ClassParams cp, double
                     fac * (4.0 * pab * P[qm.row[2] + fc * n + qm.col[3] + fd] -
                            Pa[qm.col[2] + fc] * Pb[qm.col[3] + fd] -
                            Pa[qm.col[3] + fd] * Pb[qm.col[2] + fc]);
-        grad_bra_component<LA, LB, LC, LD, FA, FB, double>(cp.prim + cp.bra.poff[i], cp.bra.np[i],
-                                                           cp.prim + cp.ket.poff[j], cp.ket.np[j],
-                                                           A, B, C, D, Df, g);
+        grad_bra_component<LA, LB, LC, LD, FA, FB, R>(cp.prim + cp.bra.poff[i], cp.bra.np[i],
+                                                      cp.prim + cp.ket.poff[j], cp.ket.np[j], A,
+                                                      B, C, D, Df, g);
     });
 #pragma unroll
     for (int k = 0; k < 9; ++k) gout[(long long)(s * 9 + k) * cp.qstride + qa] = g[k];
 }
 
 template <int LA, int LB, int LC, int LD>
-static int launch_grad_t(const ClassParams& cp, double* gout, hipStream_t s)
+static int launch_grad_t(int fp32, const ClassParams& cp, double* gout, hipStream_t s)
 {
     using S = QuartetShape<LA, LB, LC, LD>;
     if (cp.nq <= 0) return kOk;
@@ -376,7 +376,12 @@ static int launch_grad_t(const ClassParams& cp, double* gout, hipStream_t s)
         set_error(kEUnsupported, "quartet segment too large (%lld)", cp.nq);
         return kEUnsupported;
     }
-    hipLaunchKernelGGL((k_grad_split<LA, LB, LC, LD>), dim3((unsigned)nb), dim3(256), 0, s, cp, gout);
+    if (fp32)
+        hipLaunchKernelGGL((k_grad_split<LA, LB, LC, LD, float>), dim3((unsigned)nb), dim3(256), 0, s,
+                           cp, gout);
+    else
+        hipLaunchKernelGGL((k_grad_split<LA, LB, LC, LD, double>), dim3((unsigned)nb), dim3(256), 0,
+                           s, cp, gout);
     SF_HIP(hipGetLastError());
     return kOk;
 }
@@ -536,9 +541,9 @@ static int launch_schwarz_t(const int* sa, const int* sb, const int* poff, const
     {                                                                                          \
         return launch_schwarz_t<a, b, c, d>(sa, sb, poff, np, n, prim, xyz, Q, Qp, s);         \
     }                                                                                          \
-    int launch_grad_##id(const ClassParams& cp, double* gout, hipStream_t s)                   \
+    int launch_grad_##id(int fp32, const ClassParams& cp, double* gout, hipStream_t s)         \
     {                                                                                          \
-        return launch_grad_t<a, b, c, d>(cp, gout, s);                                         \
+        return launch_grad_t<a, b, c, d>(fp32, cp, gout, s);                                   \
     }                                                                                          \
     int launch_mmd_##id(int mode, const ClassParams& cp, const Tune& tn, hipStream_t s)        \
     {                                                                                          \
@@ -693,11 +698,11 @@ int class_ntile(int qcls)
     return 0;
 }
 
-int launch_grad_class(int qcls, const ClassParams& cp, double* gout, hipStream_t s)
+int launch_grad_class(int qcls, int fp32, const ClassParams& cp, double* gout, hipStream_t s)
 {
     switch (qcls) {
 #define X(id, a, b, c, d) \
-    case id: return launch_grad_##id(cp, gout, s);
+    case id: return launch_grad_##id(fp32, cp, gout, s);
         SF_FOR_EACH_CLASS(X)
 #undef X
     }

@@ -1515,9 +1515,10 @@ static int prepare_grad(sf_basis* b)
     return kOk;
 }
 
-extern "C" int sf_grad(sf_basis* b, const double* P, const double* W, double* grad, void* stream)
+extern "C" int sf_grad(sf_basis* b, const double* P, const double* W, double* grad, int precision,
+                       void* stream)
 {
-    if (!b || !P || !W || !grad) {
+    if (!b || !P || !W || !grad || precision < 0 || precision > 1) {
         set_error(kEInval, "sf_grad: null argument");
         return kEInval;
     }
@@ -1543,8 +1544,12 @@ extern "C" int sf_grad(sf_basis* b, const double* P, const double* W, double* gr
         QuartetClassData& Q = b->qc[c];
         if (Q.nq_all == 0) continue;
         ClassParams cp = class_params(b, c, b->d_Ps.as<double>());
-        set_segment(b, c, 0, 0, cp, b->d_zero.as<int>());
-        SF_TRY(launch_grad_class(c, cp, b->d_gout.as<double>(), s));
+        // every segment writes its quartets' partials at their class-wide
+        // index, so the reductions below see one array either way
+        for (int seg = 0; seg <= precision; ++seg) {
+            set_segment(b, c, precision, seg, cp, b->d_zero.as<int>());
+            SF_TRY(launch_grad_class(c, seg, cp, b->d_gout.as<double>(), s));
+        }
         SF_TRY(launch_grad_reduce_bra(b->d_gout.as<double>(), class_nslice(c), Q.nq_all,
                                       Q.d_base_all.as<long long>(), Q.nbra,
                                       b->d_gpart.as<double>() + b->gpart_bra[c], s));

@@ -158,7 +158,7 @@ int class_ntile(int qcls);
 int class_nint(int qcls);
 int class_split(int qcls);
 int class_nslice(int qcls);
-int launch_grad_class(int qcls, const ClassParams& cp, double* gout, hipStream_t s);
+int launch_grad_class(int qcls, int fp32, const ClassParams& cp, double* gout, hipStream_t s);
 int launch_grad_reduce_bra(const double* gout, int nslice, long long qstride,
                            const long long* base, int nbra, double* part, hipStream_t s);
 int launch_grad_reduce_ket(const double* gout, int nslice, long long qstride,

@@ -78,7 +78,7 @@ def lib():
     L.sf_jk.argtypes = [vp, vp, vp, vp, ctypes.c_int, vp]
     L.sf_basis_set_nuclei.argtypes = [vp, ctypes.c_int, _f64p, _f64p]
     L.sf_one_electron.argtypes = [vp, vp, vp, vp, vp]
-    L.sf_grad.argtypes = [vp, vp, vp, _f64p, vp]
+    L.sf_grad.argtypes = [vp, vp, vp, _f64p, ctypes.c_int, vp]
     L.sf_jk_profile.argtypes = [vp, vp, vp, vp, ctypes.c_int, vp, _f64p]
     L.sf_basis_create_batch.argtypes = [ctypes.c_int, _i32p, ctypes.c_int, _i32p, _i32p, _f64p,
                                         _f64p, _f64p, ctypes.c_int, _i32p, _i32p,
@@ -268,15 +268,16 @@ class Basis:
                                     _stream_ptr(torch)))
         return S, T, V
 
-    def gradient(self, P, W) -> np.ndarray:
+    def gradient(self, P, W, precision: int = 0) -> np.ndarray:
         """Electronic nuclear gradient (natom x 3, Eh/bohr) for density P and
-        energy-weighted density W (float64 CUDA tensors)."""
+        energy-weighted density W (float64 CUDA tensors); precision 1 runs
+        the derivative quartets below tau_mixed in FP32."""
         torch = _torch()
         P = P.contiguous()
         W = W.contiguous()
         g = np.zeros(3 * self.mol.natom)
         check(lib().sf_grad(self._h, ctypes.c_void_p(P.data_ptr()), ctypes.c_void_p(W.data_ptr()), g,
-                            _stream_ptr(torch)))
+                            int(precision), _stream_ptr(torch)))
         return g.reshape(-1, 3)
 
     def schwarz(self) -> np.ndarray:
@@ -424,7 +425,7 @@ class BatchBasis(Basis):
         check(lib().sf_basis_set_layout(self._h, off, ld))
         self.shape = (self.nmol, self.nmax, self.nmax)
 
-    def gradient(self, P, W):
+    def gradient(self, P, W, precision: int = 0):
         raise EngineError(5, "gradients are computed per molecule (Basis)")
 
     def eri_dense(self):

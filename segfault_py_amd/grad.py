@@ -35,10 +35,11 @@ def nuclear_repulsion_gradient(mol) -> np.ndarray:
     return g
 
 
-def rhf_gradient(mol, result: hf.SCFResult, basis: engine.Basis) -> np.ndarray:
-    """Total RHF gradient (natom x 3, Eh/bohr) at a converged SCF result."""
+def rhf_gradient(mol, result: hf.SCFResult, basis: engine.Basis, precision: int = 0) -> np.ndarray:
+    """Total RHF gradient (natom x 3, Eh/bohr) at a converged SCF result;
+    precision 1 runs the derivative quartets below tau_mixed in FP32."""
     W = result.energy_weighted_density()
-    g_el = basis.gradient(result.P, W)
+    g_el = basis.gradient(result.P, W, precision)
     return g_el + nuclear_repulsion_gradient(mol)
 
 
@@ -71,7 +72,7 @@ class RHFEnergy:
                 o = RHFEnergy.options
                 basis = engine.Basis(mol).screen(o["tau"], o["tau_mixed"])
                 res = hf.scf_diis(mol, tol=o["tol"], basis=basis, precision=o["precision"])
-                g = rhf_gradient(mol, res, basis)
+                g = rhf_gradient(mol, res, basis, o["precision"])
                 ctx.save_for_backward(torch.tensor(g, dtype=coords.dtype, device=coords.device))
                 return torch.tensor(res.energy, dtype=coords.dtype, device=coords.device)
 

@@ -29,10 +29,12 @@ def _fd(mol, comps, energy=_energy):
 
 
 def _analytic(mol, precision=0):
+    """SCF and gradient in the same precision (mixed: quartets and
+    derivative quartets below tau_mixed = 1e-5 in FP32)."""
     from segfault_py_amd import engine, grad as G, hf
     b = engine.Basis(mol).screen(1e-12, 1e-5)
     r = hf.scf_diis(mol, tol=1e-11, basis=b, precision=precision)
-    return G.rhf_gradient(mol, r, b)
+    return G.rhf_gradient(mol, r, b, precision)
 
 
 @pytest.mark.parametrize("fixture", ["water_sp", "water_ccpvdz"])
@@ -83,3 +85,7 @@ def test_cluster_gradient_fp64_and_mixed(gpu):
     fd = _fd(mol, [(0, 0), (1, 1), (2, 2), (17, 0)], energy=lambda m: _energy(m, tol=1e-10))
     err = max(abs(g64[a, k] - v) for (a, k), v in fd.items())
     assert err < 1e-5, err
+    # the mixed gradient against finite differences of the mixed energy
+    fdm = _fd(mol, [(0, 0), (17, 0)], energy=lambda m: _energy(m, tol=1e-10, precision=1))
+    errm = max(abs(gmx[a, k] - v) for (a, k), v in fdm.items())
+    assert errm < 1e-5, errm
