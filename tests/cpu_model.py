@@ -28,6 +28,7 @@ def harness():
         H = ctypes.CDLL(build_harness())
         H.h_eri.argtypes = [ctypes.c_int] * 4 + [_f64p, ctypes.c_int, _f64p, ctypes.c_int, _f64p,
                                                  _f64p, _f64p, _f64p, _f64p, ctypes.c_int]
+        H.h_rys_kernel.argtypes = [ctypes.c_int, ctypes.c_double, _f64p, _f64p]
         H.h_digest.argtypes = [ctypes.c_int] * 4 + [_f64p, _f64p, ctypes.c_int, _i32p, _f64p]
         H.h_eri_split.argtypes = H.h_eri.argtypes
         H.h_digest_tiles.argtypes = [ctypes.c_int] * 4 + [_f64p, ctypes.c_double, _f64p,

@@ -7,6 +7,19 @@ using namespace sf;
 
 extern "C" void h_boys(int mmax, double X, double* F) { boys_grid(mmax, X, F); }
 extern "C" void h_boys01(double X, double* F) { boys01(X, F[0], F[1]); }
+// The node evaluation the ERI kernels use (one node from F_0, F_1 of the
+// Boys grid, more from the Chebyshev tables), with the global tables.
+extern "C" int h_rys_kernel(int n, double X, double* u, double* w)
+{
+    switch (n) {
+    case 1: rys_nodes_tab<1>(X, global_node_table<1>(), u, w); return 0;
+    case 2: rys_nodes_tab<2>(X, global_node_table<2>(), u, w); return 0;
+    case 3: rys_nodes_tab<3>(X, global_node_table<3>(), u, w); return 0;
+    case 4: rys_nodes_tab<4>(X, global_node_table<4>(), u, w); return 0;
+    case 5: rys_nodes_tab<5>(X, global_node_table<5>(), u, w); return 0;
+    }
+    return 1;
+}
 
 extern "C" int h_rys(int n, double X, double* u, double* w)
 {

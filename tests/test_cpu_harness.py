@@ -30,6 +30,31 @@ def test_rys_tables_match_mpmath():
     assert worst_u < 1e-14 and worst_w < 1e-14, (worst_u, worst_w)
 
 
+def test_kernel_node_rules_match_mpmath():
+    """The node rules inside the ERI kernels (one node from F0/F1 of the
+    Boys grid, two to five from the Chebyshev tables) against 50-digit
+    mpmath nodes and weights; the one-node rule also against the Chebyshev
+    rule over a dense range of X."""
+    H = CM.harness()
+    worst_u = worst_w = 0.0
+    for pt in _golden("rys_mpmath.json")["points"]:
+        n = pt["n"]
+        u, w = np.zeros(n), np.zeros(n)
+        assert H.h_rys_kernel(n, float(pt["X"]), u, w) == 0
+        ur = np.array([float(v) for v in pt["u"]])
+        wr = np.array([float(v) for v in pt["w"]])
+        worst_u = max(worst_u, float(np.max(np.abs(u - ur) / ur)))
+        worst_w = max(worst_w, float(np.max(np.abs(w - wr) / wr)))
+    assert worst_u < 1e-14 and worst_w < 1e-14, (worst_u, worst_w)
+    rng = np.random.default_rng(0)
+    X = np.concatenate([rng.uniform(0, 50, 4000), np.geomspace(1e-12, 1e4, 200)])
+    for x in X:
+        u, w, uc, wc = np.zeros(1), np.zeros(1), np.zeros(1), np.zeros(1)
+        H.h_rys_kernel(1, float(x), u, w)
+        H.h_rys(1, float(x), uc, wc)
+        assert np.max(np.abs(u - uc) / uc) < 5e-14 and np.max(np.abs(w - wc) / wc) < 5e-14, x
+
+
 def test_boys_grid_matches_mpmath():
     H = CM.harness()
     worst = 0.0
