@@ -53,6 +53,8 @@ def parse():
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the 256-molecule batched SCF measurement")
     ap.add_argument("--batch-size", type=int, default=256)
+    ap.add_argument("--only-batch", action="store_true",
+                    help="run only the batched SCF workload and print its JSON")
     ap.add_argument("--workload", type=int, default=1, help="workload index (1 = 20 waters)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (for HBM counter scaling)")
@@ -144,6 +146,9 @@ def main():
         sys.exit(2)
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+        return
+    if args.only_batch:
+        print(json.dumps({"batch": batch_workload(args.batch_size)}))
         return
     import torch
     from segfault_py_amd import engine, metrics, workloads as W
