@@ -9,12 +9,15 @@
 #include <rocprim/device/device_radix_sort.hpp>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
 #include <numeric>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/sfeng.h"
@@ -78,6 +81,46 @@ struct DevBuf {
         int rc_ = (x);           \
         if (rc_ != kOk) return rc_; \
     } while (0)
+
+// Opt-in wall-clock report of setup stages: with SF_SETUP_TIMING set in
+// the environment, one line per stage on stderr.
+struct StageTimer {
+    bool on;
+    const char* what;
+    std::chrono::steady_clock::time_point t;
+    explicit StageTimer(const char* w)
+        : on(std::getenv("SF_SETUP_TIMING") != nullptr), what(w), t(std::chrono::steady_clock::now())
+    {
+    }
+    void mark(const char* stage)
+    {
+        if (!on) return;
+        const auto n = std::chrono::steady_clock::now();
+        fprintf(stderr, "[sf %s] %s %.3f ms\n", what, stage,
+                std::chrono::duration<double, std::milli>(n - t).count());
+        t = n;
+    }
+};
+
+// Runs f(lo, hi) over [0, n) split into contiguous ranges on up to 16 host
+// threads (setup loops over independent molecules or shell pairs).  Small n
+// runs inline.
+template <class F>
+static void parallel_ranges(long long n, long long min_per_thread, F&& f)
+{
+    const long long hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    const long long nt = std::max(1LL, std::min(hw, n / std::max(1LL, min_per_thread)));
+    if (nt <= 1) {
+        if (n > 0) f(0LL, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    th.reserve(nt - 1);
+    for (long long t = 1; t < nt; ++t)
+        th.emplace_back([&f, t, n, nt] { f(n * t / nt, n * (t + 1) / nt); });
+    f(0LL, n / nt);
+    for (auto& x : th) x.join();
+}
 
 static double dfact2(int n)
 {
@@ -379,6 +422,7 @@ static int create_basis(int nmol, const int* mol_shell, int nshell, const int* L
                 return kEInval;
             }
     }
+    StageTimer tm("create");
     sf_basis* b = new (std::nothrow) sf_basis();
     if (!b) {
         set_error(kENoMem, "sf_basis_create: host allocation");
@@ -429,6 +473,7 @@ static int create_basis(int nmol, const int* mol_shell, int nshell, const int* L
         b->mol_nbf.push_back(off - b->mol_fbase[m]);
     }
     b->nbf = off;
+    tm.mark("shells");
     // plan keys: 2 ns^2 per molecule (X and Y destinations)
     std::vector<unsigned> kbase(nmol);
     std::vector<int> mol_ns(nmol);
@@ -450,70 +495,99 @@ static int create_basis(int nmol, const int* mol_shell, int nshell, const int* L
     // largest |Kp| of the molecule) are dropped, which changes no integral by
     // more than ~1e-15.
     const double kPrimTol = 1e-20;
+    // molecules are independent: each fills its own lists on a host thread,
+    // then the lists are concatenated in molecule order
+    struct MolPairs {
+        std::vector<int> sa[kNumPairClass], sb[kNumPairClass], poff[kNumPairClass], np[kNumPairClass];
+        std::vector<PrimPair> prim;
+    };
+    std::vector<MolPairs> mp(nmol);
     if (nmol == 1) b->pair_slot.resize((size_t)nshell * (nshell + 1) / 2);
+    const sf_basis* cb = b;
+    parallel_ranges(nmol, 4, [&](long long m0, long long m1) {
+        for (long long m = m0; m < m1; ++m) {
+            MolPairs& R = mp[m];
+            const int s0 = mol_shell[m], s1 = mol_shell[m + 1];
+            double kp_max = 0.0;
+            for (int a = s0; a < s1; ++a)
+                for (int c = s0; c <= a; ++c) {
+                    const double* A = &cb->xyz[3 * a];
+                    const double* B = &cb->xyz[3 * c];
+                    const double AB2 = (A[0] - B[0]) * (A[0] - B[0]) + (A[1] - B[1]) * (A[1] - B[1]) +
+                                       (A[2] - B[2]) * (A[2] - B[2]);
+                    for (int i = 0; i < nprim[a]; ++i)
+                        for (int j = 0; j < nprim[c]; ++j) {
+                            const double ea = cb->expn[cb->pstart[a] + i], eb = cb->expn[cb->pstart[c] + j];
+                            const double p = ea + eb;
+                            const double K = cb->coef[cb->pstart[a] + i] * cb->coef[cb->pstart[c] + j] *
+                                             std::exp(-ea * eb / p * AB2);
+                            kp_max = std::max(kp_max, std::fabs(K / p));
+                        }
+                }
+            for (int a = s0; a < s1; ++a)
+                for (int c = s0; c <= a; ++c) {
+                    int sa = a, sb = c;
+                    if (L[sb] > L[sa]) std::swap(sa, sb);
+                    const int pcl = pair_class(L[sa], L[sb]);
+                    R.sa[pcl].push_back(sa);
+                    R.sb[pcl].push_back(sb);
+                    R.poff[pcl].push_back((int)R.prim.size());
+                    if (nmol == 1)
+                        b->pair_slot[(size_t)a * (a + 1) / 2 + c] = (pcl << 24) | (int)(R.sa[pcl].size() - 1);
+                    const double* A = &cb->xyz[3 * sa];
+                    const double* B = &cb->xyz[3 * sb];
+                    const double AB2 = (A[0] - B[0]) * (A[0] - B[0]) + (A[1] - B[1]) * (A[1] - B[1]) +
+                                       (A[2] - B[2]) * (A[2] - B[2]);
+                    int kept = 0;
+                    for (int i = 0; i < nprim[sa]; ++i)
+                        for (int j = 0; j < nprim[sb]; ++j) {
+                            const double ea = cb->expn[cb->pstart[sa] + i], eb = cb->expn[cb->pstart[sb] + j];
+                            const double p = ea + eb;
+                            PrimPair pp;
+                            pp.p = p;
+                            pp.Px = (ea * A[0] + eb * B[0]) / p;
+                            pp.Py = (ea * A[1] + eb * B[1]) / p;
+                            pp.Pz = (ea * A[2] + eb * B[2]) / p;
+                            pp.K = cb->coef[cb->pstart[sa] + i] * cb->coef[cb->pstart[sb] + j] *
+                                   std::exp(-ea * eb / p * AB2);
+                            pp.ip = 1.0 / p;
+                            pp.Kp = pp.K / p;
+                            pp.eb = eb;
+                            if (34.986836655249725 * std::fabs(pp.Kp) * kp_max / std::sqrt(p) < kPrimTol)
+                                continue;
+                            R.prim.push_back(pp);
+                            ++kept;
+                        }
+                    R.np[pcl].push_back(kept);
+                }
+        }
+    });
+    {
+        size_t nprim_all = 0;
+        for (const auto& R : mp) nprim_all += R.prim.size();
+        if (nprim_all >= (1u << 31)) {
+            set_error(kEUnsupported, "too many primitive pairs");
+            delete b;
+            return kEUnsupported;
+        }
+        b->prim.reserve(nprim_all);
+    }
     for (auto& P : b->pc) P.mbeg.push_back(0);
     for (int m = 0; m < nmol; ++m) {
-        const int s0 = mol_shell[m], s1 = mol_shell[m + 1];
-        double kp_max = 0.0;
-        for (int a = s0; a < s1; ++a)
-            for (int c = s0; c <= a; ++c) {
-                const double* A = &b->xyz[3 * a];
-                const double* B = &b->xyz[3 * c];
-                const double AB2 = (A[0] - B[0]) * (A[0] - B[0]) + (A[1] - B[1]) * (A[1] - B[1]) +
-                                   (A[2] - B[2]) * (A[2] - B[2]);
-                for (int i = 0; i < nprim[a]; ++i)
-                    for (int j = 0; j < nprim[c]; ++j) {
-                        const double ea = b->expn[b->pstart[a] + i], eb = b->expn[b->pstart[c] + j];
-                        const double p = ea + eb;
-                        const double K = b->coef[b->pstart[a] + i] * b->coef[b->pstart[c] + j] *
-                                         std::exp(-ea * eb / p * AB2);
-                        kp_max = std::max(kp_max, std::fabs(K / p));
-                    }
-            }
-        for (int a = s0; a < s1; ++a)
-            for (int c = s0; c <= a; ++c) {
-                int sa = a, sb = c;
-                if (L[sb] > L[sa]) std::swap(sa, sb);
-                const int pcl = pair_class(L[sa], L[sb]);
-                PairClassData& P = b->pc[pcl];
-                P.sa.push_back(sa);
-                P.sb.push_back(sb);
-                P.poff.push_back((int)b->prim.size());
-                if (nmol == 1)
-                    b->pair_slot[(size_t)a * (a + 1) / 2 + c] = (pcl << 24) | (int)(P.sa.size() - 1);
-                const double* A = &b->xyz[3 * sa];
-                const double* B = &b->xyz[3 * sb];
-                const double AB2 = (A[0] - B[0]) * (A[0] - B[0]) + (A[1] - B[1]) * (A[1] - B[1]) +
-                                   (A[2] - B[2]) * (A[2] - B[2]);
-                int kept = 0;
-                for (int i = 0; i < nprim[sa]; ++i)
-                    for (int j = 0; j < nprim[sb]; ++j) {
-                        const double ea = b->expn[b->pstart[sa] + i], eb = b->expn[b->pstart[sb] + j];
-                        const double p = ea + eb;
-                        PrimPair pp;
-                        pp.p = p;
-                        pp.Px = (ea * A[0] + eb * B[0]) / p;
-                        pp.Py = (ea * A[1] + eb * B[1]) / p;
-                        pp.Pz = (ea * A[2] + eb * B[2]) / p;
-                        pp.K = b->coef[b->pstart[sa] + i] * b->coef[b->pstart[sb] + j] *
-                               std::exp(-ea * eb / p * AB2);
-                        pp.ip = 1.0 / p;
-                        pp.Kp = pp.K / p;
-                        pp.eb = eb;
-                        if (34.986836655249725 * std::fabs(pp.Kp) * kp_max / std::sqrt(p) < kPrimTol)
-                            continue;
-                        b->prim.push_back(pp);
-                        ++kept;
-                    }
-                P.np.push_back(kept);
-            }
-        for (auto& P : b->pc) P.mbeg.push_back((int)P.sa.size());
+        MolPairs& R = mp[m];
+        const int base = (int)b->prim.size();
+        b->prim.insert(b->prim.end(), R.prim.begin(), R.prim.end());
+        for (int p = 0; p < kNumPairClass; ++p) {
+            PairClassData& P = b->pc[p];
+            P.sa.insert(P.sa.end(), R.sa[p].begin(), R.sa[p].end());
+            P.sb.insert(P.sb.end(), R.sb[p].begin(), R.sb[p].end());
+            P.np.insert(P.np.end(), R.np[p].begin(), R.np[p].end());
+            for (int o : R.poff[p]) P.poff.push_back(base + o);
+            P.mbeg.push_back((int)P.sa.size());
+        }
+        std::vector<PrimPair>().swap(R.prim);
     }
-    if (b->prim.size() >= (1u << 31)) {
-        set_error(kEUnsupported, "too many primitive pairs");
-        delete b;
-        return kEUnsupported;
-    }
+    tm.mark("primitive pairs");
     // default layout: molecule blocks packed back to back, ld = n
     long long lo = 0;
     for (int m = 0; m < nmol; ++m) {
@@ -546,6 +620,7 @@ static int create_basis(int nmol, const int* mol_shell, int nshell, const int* L
         delete b;
         return rc;
     }
+    tm.mark("uploads");
     *out = b;
     return kOk;
 }
@@ -657,23 +732,26 @@ static int compute_schwarz(sf_basis* b)
     if (!b->prim.empty())
         SF_HIP(hipMemcpy(b->prim_q.data(), b->d_prim_q.p, b->prim_q.size() * sizeof(double),
                          hipMemcpyDeviceToHost));
-    std::vector<int> order;
-    std::vector<PrimPair> pp;
-    std::vector<double> qq;
+    // shell pairs own disjoint primitive ranges: sorted on several threads
     for (auto& P : b->pc)
-        for (size_t i = 0; i < P.sa.size(); ++i) {
-            const int o = P.poff[i], n = P.np[i];
-            order.resize(n);
-            std::iota(order.begin(), order.end(), 0);
-            std::stable_sort(order.begin(), order.end(),
-                             [&](int x, int y) { return b->prim_q[o + x] > b->prim_q[o + y]; });
-            pp.assign(b->prim.begin() + o, b->prim.begin() + o + n);
-            qq.assign(b->prim_q.begin() + o, b->prim_q.begin() + o + n);
-            for (int k = 0; k < n; ++k) {
-                b->prim[o + k] = pp[order[k]];
-                b->prim_q[o + k] = qq[order[k]];
+        parallel_ranges((long long)P.sa.size(), 256, [&](long long i0, long long i1) {
+            std::vector<int> order;
+            std::vector<PrimPair> pp;
+            std::vector<double> qq;
+            for (long long i = i0; i < i1; ++i) {
+                const int o = P.poff[i], n = P.np[i];
+                order.resize(n);
+                std::iota(order.begin(), order.end(), 0);
+                std::stable_sort(order.begin(), order.end(),
+                                 [&](int x, int y) { return b->prim_q[o + x] > b->prim_q[o + y]; });
+                pp.assign(b->prim.begin() + o, b->prim.begin() + o + n);
+                qq.assign(b->prim_q.begin() + o, b->prim_q.begin() + o + n);
+                for (int k = 0; k < n; ++k) {
+                    b->prim[o + k] = pp[order[k]];
+                    b->prim_q[o + k] = qq[order[k]];
+                }
             }
-        }
+        });
     SF_TRY(b->d_prim.upload(b->prim));
     SF_TRY(b->d_prim_q.upload(b->prim_q.empty() ? std::vector<double>(1, 0.0) : b->prim_q));
     b->have_schwarz = true;
@@ -711,7 +789,9 @@ extern "C" int sf_screen(sf_basis* b, double tau, double tau_mixed)
         set_error(kEInval, "sf_screen: bad arguments");
         return kEInval;
     }
+    StageTimer tm("screen");
     SF_TRY(compute_schwarz(b));
+    tm.mark("schwarz");
     b->drop_graph();
     b->screened = false;
     b->grad_ready = false;
@@ -725,15 +805,21 @@ extern "C" int sf_screen(sf_basis* b, double tau, double tau_mixed)
         for (int m = 0; m < nmol; ++m)
             for (int i = P.mbeg[m]; i < P.mbeg[m + 1]; ++i) qmax[m] = std::max(qmax[m], P.Q[i]);
     // significant pairs of each molecule, sorted by Q descending (ties: original order)
+    std::vector<std::vector<int>> sig(nmol);
     for (auto& P : b->pc) {
         P.s_sa.clear(); P.s_sb.clear(); P.s_poff.clear(); P.s_np.clear(); P.s_Q.clear();
         P.s_mol.clear(); P.s_mbeg.assign(1, 0);
+        parallel_ranges(nmol, 8, [&](long long m0, long long m1) {
+            for (long long m = m0; m < m1; ++m) {
+                std::vector<int>& idx = sig[m];
+                idx.clear();
+                for (int i = P.mbeg[m]; i < P.mbeg[m + 1]; ++i)
+                    if (P.Q[i] * qmax[m] >= tau) idx.push_back(i);
+                std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return P.Q[x] > P.Q[y]; });
+            }
+        });
         for (int m = 0; m < nmol; ++m) {
-            std::vector<int> idx;
-            for (int i = P.mbeg[m]; i < P.mbeg[m + 1]; ++i)
-                if (P.Q[i] * qmax[m] >= tau) idx.push_back(i);
-            std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return P.Q[x] > P.Q[y]; });
-            for (int i : idx) {
+            for (int i : sig[m]) {
                 P.s_sa.push_back(P.sa[i]);
                 P.s_sb.push_back(P.sb[i]);
                 P.s_poff.push_back(P.poff[i]);
@@ -749,6 +835,7 @@ extern "C" int sf_screen(sf_basis* b, double tau, double tau_mixed)
         SF_TRY(P.ds_np.upload(P.s_np));
         SF_TRY(P.ds_Q.upload(P.s_Q));
     }
+    tm.mark("pair lists");
     // quartet lists per class (GPU builder), prefix sums on the host
     size_t maxbra = 1;
     for (auto& P : b->pc) maxbra = std::max(maxbra, P.s_sa.size());
@@ -830,6 +917,7 @@ extern "C" int sf_screen(sf_basis* b, double tau, double tau_mixed)
                 if (Q.same && i >= j0 && i < j1) b->funcq -= pab * pab - pab * (pab + 1) / 2;
             }
         }
+    tm.mark("quartet lists");
     // digestion plan: (destination, entry) pairs sorted by destination;
     // destinations are numbered per molecule (KeyMap)
     const int nmol_ = b->nmol;
@@ -865,6 +953,8 @@ extern "C" int sf_screen(sf_basis* b, double tau, double tau_mixed)
     }
     DevBuf d_tab;
     SF_TRY(d_tab.upload(tab));
+    SF_HIP(hipDeviceSynchronize());
+    tm.mark("plan entries");
     const int nbits = key_bits(nkeys);
     size_t tmp_bytes = 0;
     SF_HIP(rocprim::radix_sort_pairs(nullptr, tmp_bytes, d_keys.as<unsigned>(),
@@ -889,51 +979,122 @@ extern "C" int sf_screen(sf_basis* b, double tau, double tau_mixed)
     std::vector<long long> segb(nkeys), sege(nkeys);
     SF_HIP(hipMemcpy(segb.data(), b->d_segb.p, nkeys * sizeof(long long), hipMemcpyDeviceToHost));
     SF_HIP(hipMemcpy(sege.data(), b->d_sege.p, nkeys * sizeof(long long), hipMemcpyDeviceToHost));
-    // slot layout: destinations in key order, each a run of NE-double tiles
+    tm.mark("sort + segments");
+    // slot layout: destinations in key order, each a run of NE-double tiles.
+    // Two passes over the molecules on host threads: counts per molecule and
+    // destination group, then (after prefix sums) the lists in place.
     std::vector<unsigned long long> key_base(nkeys);
     std::vector<int> key_ne(nkeys);
     std::vector<int> ds1[9], ds2[9], dxy[9], ch_n[9], ch_dest[9], ch_part[9];
     std::vector<unsigned long long> ch_src[9];
     std::vector<int> m_dest[9], m_pbase[9], m_nch[9];
-    int npart[9] = {0};
-    unsigned long long acc = 0;
-    for (int m = 0; m < nmol_; ++m) {
+    struct SlotCount {
+        unsigned long long acc;
+        long long nd[9], nch[9], nm[9], npart[9];
+    };
+    std::vector<SlotCount> cnt_m((size_t)nmol_ + 1, SlotCount{});
+    auto key_shells = [&](int m, unsigned long long k, int& s1, int& s2, int& xy) {
         const int s0 = b->mol_shell[m];
         const unsigned long long ns = b->mol_shell[m + 1] - s0, n2 = ns * ns;
-        for (unsigned long long k = mkb[m]; k < mkb[m + 1]; ++k) {
-            const unsigned long long r = k - mkb[m];
-            const int xy = (int)(r / n2);
-            const int s1 = s0 + (int)((r % n2) / ns), s2 = s0 + (int)(r % ns);
-            key_ne[k] = cart_n(b->L[s1]) * cart_n(b->L[s2]);
-            key_base[k] = acc;
-            if (sege[k] <= segb[k]) continue;
-            const long long cnt = sege[k] - segb[k];
-            acc += (unsigned long long)cnt * key_ne[k];
-            const int g = 3 * b->L[s1] + b->L[s2];
-            const int d = (int)ds1[g].size();
-            ds1[g].push_back(s1);
-            ds2[g].push_back(s2);
-            dxy[g].push_back(xy);
-            // chunks of at most kReduceChunk tiles; several -> partial slots
-            const int nch = (int)((cnt + kReduceChunk - 1) / kReduceChunk);
-            if (nch > 1) {
-                m_dest[g].push_back(d);
-                m_pbase[g].push_back(npart[g]);
-                m_nch[g].push_back(nch);
+        const unsigned long long r = k - mkb[m];
+        xy = (int)(r / n2);
+        s1 = s0 + (int)((r % n2) / ns);
+        s2 = s0 + (int)(r % ns);
+    };
+    parallel_ranges(nmol_, 4, [&](long long m0, long long m1) {
+        for (long long m = m0; m < m1; ++m) {
+            SlotCount c{};
+            for (unsigned long long k = mkb[m]; k < mkb[m + 1]; ++k) {
+                if (sege[k] <= segb[k]) continue;
+                int s1, s2, xy;
+                key_shells((int)m, k, s1, s2, xy);
+                const long long n = sege[k] - segb[k];
+                const int g = 3 * b->L[s1] + b->L[s2];
+                const long long nch = (n + kReduceChunk - 1) / kReduceChunk;
+                c.acc += (unsigned long long)n * (cart_n(b->L[s1]) * cart_n(b->L[s2]));
+                c.nd[g] += 1;
+                c.nch[g] += nch;
+                if (nch > 1) {
+                    c.nm[g] += 1;
+                    c.npart[g] += nch;
+                }
             }
-            for (int c = 0; c < nch; ++c) {
-                ch_src[g].push_back(key_base[k] + (unsigned long long)c * kReduceChunk * key_ne[k]);
-                ch_n[g].push_back((int)std::min<long long>(kReduceChunk, cnt - (long long)c * kReduceChunk));
-                ch_dest[g].push_back(d);
-                ch_part[g].push_back(nch > 1 ? npart[g] + c : -1);
-            }
-            if (nch > 1) npart[g] += nch;
+            cnt_m[m + 1] = c;
+        }
+    });
+    // exclusive prefix sums: cnt_m[m] becomes molecule m's starting offsets
+    for (int m = 0; m < nmol_; ++m) {
+        SlotCount& n = cnt_m[m + 1];
+        const SlotCount& p = cnt_m[m];
+        n.acc += p.acc;
+        for (int g = 0; g < 9; ++g) {
+            n.nd[g] += p.nd[g];
+            n.nch[g] += p.nch[g];
+            n.nm[g] += p.nm[g];
+            n.npart[g] += p.npart[g];
         }
     }
+    const unsigned long long acc = cnt_m[nmol_].acc;
     if (acc >= (1ULL << 32)) {
         set_error(kEUnsupported, "digestion plan of %llu doubles exceeds 32-bit slots", acc);
         return kEUnsupported;
     }
+    int npart[9];
+    for (int g = 0; g < 9; ++g) {
+        const SlotCount& t = cnt_m[nmol_];
+        if (t.nd[g] >= (1LL << 31) || t.nch[g] >= (1LL << 31) || t.npart[g] >= (1LL << 31)) {
+            set_error(kEUnsupported, "digestion plan: too many destinations or chunks");
+            return kEUnsupported;
+        }
+        npart[g] = (int)t.npart[g];
+        ds1[g].resize(t.nd[g]);
+        ds2[g].resize(t.nd[g]);
+        dxy[g].resize(t.nd[g]);
+        ch_src[g].resize(t.nch[g]);
+        ch_n[g].resize(t.nch[g]);
+        ch_dest[g].resize(t.nch[g]);
+        ch_part[g].resize(t.nch[g]);
+        m_dest[g].resize(t.nm[g]);
+        m_pbase[g].resize(t.nm[g]);
+        m_nch[g].resize(t.nm[g]);
+    }
+    parallel_ranges(nmol_, 4, [&](long long m0, long long m1) {
+        for (long long m = m0; m < m1; ++m) {
+            SlotCount o = cnt_m[m];
+            for (unsigned long long k = mkb[m]; k < mkb[m + 1]; ++k) {
+                int s1, s2, xy;
+                key_shells((int)m, k, s1, s2, xy);
+                key_ne[k] = cart_n(b->L[s1]) * cart_n(b->L[s2]);
+                key_base[k] = o.acc;
+                if (sege[k] <= segb[k]) continue;
+                const long long n = sege[k] - segb[k];
+                o.acc += (unsigned long long)n * key_ne[k];
+                const int g = 3 * b->L[s1] + b->L[s2];
+                const int d = (int)o.nd[g]++;
+                ds1[g][d] = s1;
+                ds2[g][d] = s2;
+                dxy[g][d] = xy;
+                // chunks of at most kReduceChunk tiles; several -> partial slots
+                const int nch = (int)((n + kReduceChunk - 1) / kReduceChunk);
+                const int pb = (int)o.npart[g];
+                if (nch > 1) {
+                    const long long j = o.nm[g]++;
+                    m_dest[g][j] = d;
+                    m_pbase[g][j] = pb;
+                    m_nch[g][j] = nch;
+                    o.npart[g] += nch;
+                }
+                for (int c = 0; c < nch; ++c) {
+                    const long long j = o.nch[g]++;
+                    ch_src[g][j] = key_base[k] + (unsigned long long)c * kReduceChunk * key_ne[k];
+                    ch_n[g][j] = (int)std::min<long long>(kReduceChunk, n - (long long)c * kReduceChunk);
+                    ch_dest[g][j] = d;
+                    ch_part[g][j] = nch > 1 ? pb + c : -1;
+                }
+            }
+        }
+    });
+    tm.mark("slot layout + chunks");
     b->ntile_total = acc;
     SF_TRY(b->d_key_base.upload(key_base));
     DevBuf d_key_ne;
@@ -967,6 +1128,7 @@ extern "C" int sf_screen(sf_basis* b, double tau, double tau_mixed)
     }
     SF_TRY(b->d_part.ensure((size_t)std::max(1LL, part_total) * sizeof(double)));
     SF_HIP(hipDeviceSynchronize());
+    tm.mark("positions + buffers");
     b->screened = true;
     return kOk;
 }

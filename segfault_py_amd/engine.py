@@ -187,23 +187,26 @@ def eri_components(mol, quartets) -> np.ndarray:
 # ------------------------------------------------------------ the basis --
 def _shell_arrays(mols):
     """Concatenated shell arrays of a list of molecules (AO order kept)."""
+    # per atom: its shells' radial data (Atom.shell_*), centred on the atom
     L, nprim, center, expns, coefs, atom = [], [], [], [], [], []
     mol_shell, mol_atom = [0], [0]
     for mol in mols:
-        shells = mol.shells()
-        for s in shells:
-            L.append(s.L)
-            nprim.append(s.nprim)
-            center.append(np.asarray(s.center, dtype=np.float64))
-            expns.append(np.asarray(s.expns, dtype=np.float64))
-            coefs.append(np.asarray(s.coefs, dtype=np.float64))
-            atom.append(mol_atom[-1] + s.atom)
-        mol_shell.append(mol_shell[-1] + len(shells))
+        nsh = 0
+        for ia, a in enumerate(mol.atoms):
+            k = len(a.shell_L)
+            L += a.shell_L
+            nprim += a.shell_nprim
+            center.append(np.broadcast_to(np.asarray(a.xyz, dtype=np.float64), (k, 3)))
+            expns += a.shell_expns
+            coefs += a.shell_coefs
+            atom += [mol_atom[-1] + ia] * k
+            nsh += k
+        mol_shell.append(mol_shell[-1] + nsh)
         mol_atom.append(mol_atom[-1] + mol.natom)
     cat = lambda xs: np.ascontiguousarray(np.concatenate(xs), dtype=np.float64)  # noqa: E731
-    return (np.array(L, dtype=np.int32), np.array(nprim, dtype=np.int32), cat(center), cat(expns),
-            cat(coefs), np.array(atom, dtype=np.int32), np.array(mol_shell, dtype=np.int32),
-            np.array(mol_atom, dtype=np.int32))
+    return (np.array(L, dtype=np.int32), np.array(nprim, dtype=np.int32),
+            cat(center).reshape(-1), cat(expns), cat(coefs), np.array(atom, dtype=np.int32),
+            np.array(mol_shell, dtype=np.int32), np.array(mol_atom, dtype=np.int32))
 
 
 class Basis:

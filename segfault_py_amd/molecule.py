@@ -167,6 +167,19 @@ class Atom:
             for comp in CART[L]:
                 self.cgfs.append(ContractedGaussianFunction(len(ex), ex, list(cf), letter, comp,
                                                             self.xyz))
+        # per-shell radial data (fixed by element and basis, not by position):
+        # L, exponents and coefficients with the contraction normalisation and
+        # the x^L component's angular factor folded in
+        self.shell_L, self.shell_expns, self.shell_coefs = [], [], []
+        k = 0
+        for L, ex, _cf in self.shell_defs:
+            cg = self.cgfs[k]  # first component (x^L) of this shell
+            self.shell_L.append(L)
+            self.shell_expns.append(np.asarray(ex, dtype=np.float64))
+            self.shell_coefs.append(np.asarray(cg.coefs) * np.asarray(cg.normalize_factors)
+                                    * np.sqrt(factorial2(2 * L - 1)))
+            k += (L + 1) * (L + 2) // 2
+        self.shell_nprim = [len(e) for e in self.shell_expns]
 
     def __len__(self):
         return len(self.cgfs)
@@ -239,17 +252,10 @@ class Molecule:
         out = []
         off = 0
         for ia, atom in enumerate(self.atoms):
-            k = 0
-            for L, ex, _cf in atom.shell_defs:
-                cg = atom.cgfs[k]  # first component (x^L) of this shell
-                ex = np.asarray(ex, dtype=np.float64)
-                f2 = factorial2(2 * L - 1)  # the x^L component's angular factor
-                c = np.asarray(cg.coefs) * np.asarray(cg.normalize_factors) * np.sqrt(f2)
-                out.append(Shell(L=L, center=atom.xyz.copy(), expns=ex, coefs=c, atom=ia,
-                                 offset=off))
-                nf = (L + 1) * (L + 2) // 2
-                off += nf
-                k += nf
+            for L, ex, c in zip(atom.shell_L, atom.shell_expns, atom.shell_coefs):
+                out.append(Shell(L=L, center=atom.xyz.copy(), expns=ex.copy(), coefs=c.copy(),
+                                 atom=ia, offset=off))
+                off += (L + 1) * (L + 2) // 2
         return out
 
     def function_norms(self) -> np.ndarray:
