@@ -120,8 +120,14 @@ int sf_screen(sf_basis *b, double tau, double tau_mixed);
  * reference counterpart): a primitive quartet whose primitive-pair Schwarz
  * product Qp_ab Qp_cd is below tau_prim is skipped.  Default (tau_prim < 0):
  * 1e-8 * tau_mixed, about the FP32 resolution of the largest FP32 quartet.
- * 0 disables it.  FP64 quartets never skip primitives. */
+ * 0 disables it. */
 int sf_set_prim_tau(sf_basis *b, double tau_prim);
+/* Primitive screening of the FP64 quartets (no reference counterpart): a
+ * primitive quartet with Qp_ab Qp_cd < tau_prim is skipped.  Default
+ * (tau_prim < 0): 1e-3 * tau, so the primitive terms dropped from one
+ * quartet stay below the shell-level Schwarz threshold; tau = 0 (no
+ * screening) therefore skips nothing.  0 disables it. */
+int sf_set_prim_tau_fp64(sf_basis *b, double tau_prim);
 /* counts[3*c + {0,1,2}] = surviving / FP64 / FP32 shell quartets of
  * quartet class c (0..20); *funcq = surviving function-level quartets. */
 int sf_quartet_counts(const sf_basis *b, long long *counts, long long *funcq,

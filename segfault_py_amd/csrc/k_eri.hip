@@ -134,8 +134,9 @@ __device__ __forceinline__ void class_quartet(const ClassParams& cp, const NodeT
     R acc[S::NINT];
 #pragma unroll
     for (int k = 0; k < S::NINT; ++k) acc[k] = (R)0;
-    // FP32 quartets also skip primitive quartets below FP32 resolution
-    constexpr bool PS = std::is_same<R, float>::value;
+    // primitive quartets with Qp_b Qp_k < cp.tau_prim are skipped (the
+    // segment's threshold; 0 keeps every one)
+    constexpr bool PS = true;
     int bpo = cp.bra.poff[i], bnp = cp.bra.np[i];
     if constexpr (UNI) {
         // the bra's primitive pairs are read-only for the whole launch: read
@@ -269,7 +270,7 @@ __global__ __launch_bounds__(256) void k_eri_split(const ClassParams cp, double*
     for (int k = 0; k < NK; ++k) acc[k] = (R)0;
     static_switch<0, NS>(s, [&](auto sc) {
         constexpr int FA = decltype(sc)::value / S::NB, FB = decltype(sc)::value % S::NB;
-        constexpr bool PS = std::is_same<R, float>::value;
+        constexpr bool PS = true;
         eri_bra_component<LA, LB, LC, LD, FA, FB, R, PS>(
             cp.prim + cp.bra.poff[i], cp.bra.np[i], cp.prim + cp.ket.poff[j], cp.ket.np[j], A, B, C,
             D, acc, nt, cp.primq + cp.bra.poff[i], cp.primq + cp.ket.poff[j], cp.tau_prim);

@@ -205,10 +205,13 @@ struct sf_basis {
     std::vector<int> pair_slot;  // one molecule: pair (a>=b) -> (pc << 24) | position
     bool have_schwarz = false, screened = false;
     double tau = 0.0, tau_mixed = 0.0;
-    // primitive-quartet threshold of the FP32 quartets: tau_prim_set < 0
-    // means 1e-8 * tau_mixed (about FP32 resolution of the largest FP32
-    // quartet), else the value set by sf_set_prim_tau
+    // primitive-quartet thresholds.  FP32 quartets: tau_prim_set < 0 means
+    // 1e-8 * tau_mixed (about FP32 resolution of the largest FP32 quartet),
+    // else the value set by sf_set_prim_tau.  FP64 quartets: tau_prim64_set
+    // < 0 means 1e-3 * tau (the dropped primitive terms of a quartet stay
+    // below the shell-level Schwarz threshold), else sf_set_prim_tau_fp64's.
     double tau_prim_set = -1.0, tau_prim = 0.0;
+    double tau_prim64_set = -1.0, tau_prim64 = 0.0;
     QuartetClassData qc[kNumQuartetClass];
     long long funcq = 0, primq = 0;
     // digestion plan: destination-sorted tile slots
@@ -798,6 +801,7 @@ extern "C" int sf_screen(sf_basis* b, double tau, double tau_mixed)
     b->tau = tau;
     b->tau_mixed = tau_mixed;
     b->tau_prim = b->tau_prim_set >= 0.0 ? b->tau_prim_set : 1e-8 * tau_mixed;
+    b->tau_prim64 = b->tau_prim64_set >= 0.0 ? b->tau_prim64_set : 1e-3 * tau;
     const int nmol = b->nmol;
     // largest Schwarz factor of each molecule
     std::vector<double> qmax(nmol, 0.0);
@@ -1145,6 +1149,18 @@ extern "C" int sf_set_prim_tau(sf_basis* b, double tau_prim)
     return kOk;
 }
 
+extern "C" int sf_set_prim_tau_fp64(sf_basis* b, double tau_prim)
+{
+    if (!b || tau_prim != tau_prim) {
+        set_error(kEInval, "sf_set_prim_tau_fp64: bad arguments");
+        return kEInval;
+    }
+    b->tau_prim64_set = tau_prim < 0.0 ? -1.0 : tau_prim;
+    if (b->screened) b->tau_prim64 = b->tau_prim64_set >= 0.0 ? b->tau_prim64_set : 1e-3 * b->tau;
+    b->drop_graph();
+    return kOk;
+}
+
 extern "C" int sf_quartet_counts(const sf_basis* b, long long* counts, long long* funcq,
                                  long long* primq)
 {
@@ -1249,7 +1265,7 @@ static ClassParams class_params(sf_basis* b, int c, const double* Ps)
     cp.same = Q.same;
     cp.P = Ps;
     cp.primq = b->d_prim_q.as<double>();
-    cp.tau_prim = b->tau_prim;
+    cp.tau_prim = b->tau_prim64;  // set per segment
     cp.out = nullptr;
     cp.raw = Q.d_raw.as<double>();
     cp.pos = Q.d_pos.as<unsigned>();
@@ -1264,6 +1280,7 @@ static void set_segment(sf_basis* b, int c, int precision, int seg, ClassParams&
                         const int* zeros)
 {
     QuartetClassData& Q = b->qc[c];
+    cp.tau_prim = b->tau_prim64;
     if (precision == 0) {
         cp.seg_base = Q.d_base_all.as<long long>();
         cp.seg_k0 = zeros;
@@ -1276,6 +1293,7 @@ static void set_segment(sf_basis* b, int c, int precision, int seg, ClassParams&
         cp.seg_base = Q.d_base32.as<long long>();
         cp.seg_k0 = Q.d_k032.as<int>();
         cp.nq = Q.nq32;
+        cp.tau_prim = b->tau_prim;
     }
 }
 

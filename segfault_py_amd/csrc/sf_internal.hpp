@@ -83,7 +83,7 @@ struct ClassParams {
     int same;                  // bra and ket pair classes coincide
     const double* P;           // density in the shell-normalised basis
     const double* primq;       // Schwarz factor of each primitive pair (same index as prim)
-    double tau_prim;           // FP32 segments: skip primitive quartets with Qp_b Qp_k < tau_prim
+    double tau_prim;           // skip primitive quartets with Qp_b Qp_k < tau_prim (per segment)
     double* out;               // kModeRaw: SoA planes out[k * qstride + q]
     double* raw;               // split classes: raw integral planes (NINT * qstride)
     const unsigned* pos;       // kModeJK: pos[t * qstride + q] = slot of tile t in `tiles`

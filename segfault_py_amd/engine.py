@@ -90,6 +90,7 @@ def lib():
     L.sf_autotune.argtypes = [vp, vp, ctypes.c_int, vp, _i32p, _f64p]
     L.sf_set_tuning.argtypes = [vp, _i32p]
     L.sf_set_prim_tau.argtypes = [vp, ctypes.c_double]
+    L.sf_set_prim_tau_fp64.argtypes = [vp, ctypes.c_double]
     L.sf_set_engine.argtypes = [vp, ctypes.c_int, ctypes.c_int]
     _LIB = L
     return L
@@ -107,7 +108,8 @@ def exported_symbols() -> list[str]:
             "sf_schwarz", "sf_screen", "sf_quartet_counts", "sf_pair_list", "sf_quartet_list",
             "sf_eri_dense", "sf_jk", "sf_jk_profile", "sf_basis_set_nuclei", "sf_one_electron",
             "sf_grad", "sf_basis_create_batch", "sf_basis_set_layout", "sf_basis_layout",
-            "sf_jk_graph", "sf_autotune", "sf_set_tuning", "sf_set_prim_tau", "sf_set_engine"]
+            "sf_jk_graph", "sf_autotune", "sf_set_tuning", "sf_set_prim_tau", "sf_set_prim_tau_fp64",
+            "sf_set_engine"]
 
 
 def _torch():
@@ -373,6 +375,12 @@ class Basis:
         """Primitive-quartet threshold of the FP32 quartets (< 0: default
         1e-8 * tau_mixed, 0: off); see sf_set_prim_tau."""
         check(lib().sf_set_prim_tau(self._h, float(tau_prim)))
+        return self
+
+    def set_prim_tau_fp64(self, tau_prim: float) -> "Basis":
+        """Primitive-quartet threshold of the FP64 quartets (< 0: default
+        1e-3 * tau, 0: off); see sf_set_prim_tau_fp64."""
+        check(lib().sf_set_prim_tau_fp64(self._h, float(tau_prim)))
         return self
 
     def set_engine(self, engine: int, qcls: int = -1) -> "Basis":

@@ -180,6 +180,37 @@ def test_mixed_primitive_screening(gpu):
     assert abs(e(Jd, Kd) - e(J64, K64)) < 1e-6
 
 
+def test_fp64_primitive_screening(gpu, oracle):
+    """FP64 quartets skip primitive quartets with Qp_ab Qp_cd below
+    tau_prim64 (default 1e-3 * tau).  Each dropped term is bounded by that
+    product (Schwarz inequality per primitive quartet), so a contracted
+    integral moves by at most (primitive quartets) x tau_prim64: checked on
+    the dense tensor against the oracle, and on J/K (< 1e-12, energy <
+    1e-11 Eh) against the build without it.  tau = 0 skips nothing."""
+    from segfault_py_amd import engine, workloads as W
+    mol = W.config_molecule(3)
+    b = engine.Basis(mol).screen(1e-12, 1e-5)
+    _, P = _density(mol.nbf, 7, "cuda")
+    Jd, Kd = b.jk(P, 0)                   # default tau_prim64 = 1e-15
+    b.set_prim_tau_fp64(0.0)
+    J0, K0 = b.jk(P, 0)
+    b.set_prim_tau_fp64(-1.0)
+    assert float((Jd - J0).abs().max()) < 1e-12 and float((Kd - K0).abs().max()) < 1e-12
+    e = lambda J, K: float(0.5 * (P * (J - 0.5 * K)).sum())  # noqa: E731
+    assert abs(e(Jd, Kd) - e(J0, K0)) < 1e-11
+    # dense integrals of a small molecule at tau = 1e-6 (tau_prim64 = 1e-9):
+    # every integral of a surviving quartet within 81 x 1e-9 of the oracle
+    w = W.config_molecule(0)
+    ref = oracle.eri_tensor(w, 8)
+    bw = engine.Basis(w).screen(0.0, 0.0)
+    assert np.max(np.abs(bw.eri_dense() - ref)) < 1e-11   # tau = 0: nothing skipped
+    bw.set_prim_tau_fp64(1e-9)
+    dense = bw.eri_dense()
+    assert np.max(np.abs(dense - ref)) < 81e-9
+    bw.set_prim_tau_fp64(0.0)
+    assert np.max(np.abs(bw.eri_dense() - ref)) < 1e-11
+
+
 @pytest.mark.parametrize("name,group", [("water", "others"), ("methane", "alkane")])
 def test_scf_energy_with_gpu_jk(gpu, oracle, name, group):
     """SCF with GPU J/K (one-electron integrals from the oracle as inputs)
