@@ -164,10 +164,11 @@ def main():
     J = torch.empty_like(P)
     K = torch.empty_like(P)
 
-    shapes = {p: basis.autotune(P, p)[0] for p in (0, 1)}
+    basis.autotune(P, 0)
+    tuned = basis.autotune(P, 1)[0]      # shapes of every slot after both tunings
 
-    def timed(precision):
-        basis.set_tuning(shapes[precision])
+    def timed(precision, shapes=tuned):
+        basis.set_tuning(shapes)
         for _ in range(args.warmup):
             basis.jk(P, precision, out=(J, K))
         torch.cuda.synchronize()
@@ -181,13 +182,19 @@ def main():
     E64 = float(0.5 * (P * (J - 0.5 * K)).sum())
     tmix = timed(1)
     Emix = float(0.5 * (P * (J - 0.5 * K)).sum())
-    basis.set_prim_tau(0.0)              # FP32 arithmetic alone, no primitive screening
-    tmix_np = timed(1)
+    # every small-quartet segment in FP32 arithmetic (no FP64 fall-back) ...
+    fp32_only = tuned.copy()
+    fp32_only[:, 2, 2] = 0
+    tmix32 = timed(1, fp32_only)
+    Emix32 = float(0.5 * (P * (J - 0.5 * K)).sum())
+    # ... and also without the primitive screening of those quartets
+    basis.set_prim_tau(0.0)
+    tmix_np = timed(1, fp32_only)
     Emix_np = float(0.5 * (P * (J - 0.5 * K)).sum())
     basis.set_prim_tau(-1.0)
 
     # per-kernel times from HIP events on the launch stream
-    basis.set_tuning(shapes[0])
+    basis.set_tuning(tuned)
     _, _, ms = basis.jk_profile(P, 0, out=(J, K))
     kern = [(ms[2 * w["class"]], w) for w in work if w["shellq"] > 0]
     dom_ms, dom = max(kern, key=lambda x: x[0])
@@ -230,8 +237,11 @@ def main():
         "mixed_builds_per_s": 1.0 / tmix,
         "mixed_speedup": t64 / tmix,
         "dE_2e_mixed_vs_fp64": abs(Emix - E64),
-        "mixed_no_prim_screen": {"builds_per_s": 1.0 / tmix_np, "speedup": t64 / tmix_np,
-                                 "dE_2e_vs_fp64": abs(Emix_np - E64)},
+        "mixed_fp64_arith_classes": [int(c) for c in np.nonzero(tuned[:, 2, 2])[0]],
+        "mixed_fp32_arith_only": {"builds_per_s": 1.0 / tmix32, "speedup": t64 / tmix32,
+                                  "dE_2e_vs_fp64": abs(Emix32 - E64)},
+        "mixed_fp32_arith_no_prim_screen": {"builds_per_s": 1.0 / tmix_np, "speedup": t64 / tmix_np,
+                                            "dE_2e_vs_fp64": abs(Emix_np - E64)},
         "quartets_per_s_fp64": funcq / t64,
         "shell_quartets_per_s_fp64": shellq / t64,
         "prim_quartets_per_s_fp64": primq / t64,
@@ -362,8 +372,8 @@ def batch_workload(nmol: int) -> dict:
                      "converged": int(sum(res.converged)),
                      "shell_quartets_per_build": int(c[:, 0].sum()),
                      "function_quartets_per_build": int(fq),
-                     "tuned_shapes": res.tuning[0][:, :, :].reshape(-1, 4).tolist()
-                     if res.tuning is not None else None}
+                     "tuned_shapes": res.tuning[0][:, (0,) if prec == 0 else (1, 2), :]
+                     .reshape(21, -1).tolist() if res.tuning is not None else None}
         out[name]["_energies"] = res.energies
         del bb
     e64, emx = out["fp64"].pop("_energies"), out["mixed"].pop("_energies")

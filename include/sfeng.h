@@ -166,17 +166,23 @@ int sf_jk_profile(sf_basis *b, const double *P, double *J, double *K, int precis
  * fixed buffers (the per-iteration line hf.py:300). */
 int sf_jk_graph(sf_basis *b, const double *P, double *J, double *K, int precision,
                 void *stream);
-/* Launch-shape autotuner (no reference counterpart).  For every quartet
- * class and precision segment (FP64 / FP32 part), times 64/128/256-thread
- * blocks with one thread per quartet or persistent grids of 2 or 8 blocks
- * per compute unit on the current screened lists (density P, device; each
- * candidate timed on the first 2^20 quartets of the segment) and keeps the
- * fastest; sf_jk and sf_jk_graph use the choice.
- * shapes[4c + 2seg + {0,1}] receives (block, grid; 0 = one per quartet),
- * ms[2c + seg] the chosen shape's time; either may be NULL. */
+/* Launch-shape autotuner (no reference counterpart).  Three tuning slots
+ * per quartet class c: 0 = the whole class in FP64 builds (precision 0),
+ * 1 / 2 = the FP64 / FP32 segment of mixed builds (precision 1).  For the
+ * slots of `precision`, times 64/128/256-thread blocks with one thread per
+ * quartet or persistent grids of 2 or 8 blocks per compute unit on the
+ * current screened lists (density P, device; each candidate timed on the
+ * first 2^20 quartets of the segment) and keeps the fastest; for slot 2 it
+ * also times the segment with FP64 arithmetic (same primitive threshold;
+ * never less accurate) and keeps that where it is faster.  sf_jk and
+ * sf_jk_graph use the choice.  shapes[9c + 3slot + {0,1,2}] receives
+ * (block, grid; 0 = one per quartet, arith: 1 = FP64 arithmetic in slot 2)
+ * for every slot, ms[3c + slot] the chosen time of each tuned slot (others
+ * untouched); either may be NULL. */
 int sf_autotune(sf_basis *b, const double *P, int precision, void *stream, int *shapes,
                 double *ms);
-/* Set the shapes directly, in the layout sf_autotune reports. */
+/* Set the shapes (all 3 slots of all 21 classes, 189 ints) directly, in
+ * the layout sf_autotune reports. */
 int sf_set_tuning(sf_basis *b, const int *shapes);
 /* Integral engine of quartet class qcls (0..20, or -1 for all): 0 = Rys
  * quadrature (default; the method of rys.eri_rys, core.f90:917-1002),

@@ -224,7 +224,9 @@ struct sf_basis {
     unsigned long long ntile_total = 0;
     DevBuf d_Ps, d_X, d_Y;
     // launch shapes per quartet class and precision segment (sf_autotune)
-    Tune tune[kNumQuartetClass][2];
+    // launch shapes per class and tuning slot (tune_slot: the whole class in
+    // FP64 builds; the FP64 and the FP32 segment of mixed builds)
+    Tune tune[kNumQuartetClass][3];
     // integral engine per quartet class: 0 = Rys quadrature, 1 = McMurchie-
     // Davidson with MFMA transforms (FP64 in both precision segments)
     int engine[kNumQuartetClass] = {0};
@@ -1223,11 +1225,14 @@ extern "C" int sf_quartet_list(const sf_basis* b, int c, int* nbra, int* n_all, 
 }
 
 // One class/segment launch with the class's engine.
+// Tuning slot of segment seg of a build in `precision`.
+static int tune_slot(int precision, int seg) { return precision == 0 ? 0 : 1 + seg; }
+
 static int launch_class_engine(sf_basis* b, int c, int seg, int mode, const ClassParams& cp,
                                const Tune& tn, hipStream_t s)
 {
     if (b->engine[c] == 1) return launch_eri_mmd(c, mode, cp, tn, s);
-    return launch_eri_class(c, seg, mode, cp, tn, s);
+    return launch_eri_class(c, (seg == 1 && !tn.arith64) ? 1 : 0, mode, cp, tn, s);
 }
 
 extern "C" int sf_set_engine(sf_basis* b, int qcls, int engine)
@@ -1401,7 +1406,7 @@ static int run_jk(sf_basis* b, const double* P, double* J, double* K, int precis
         ClassParams cp = class_params(b, c, b->d_Ps.as<double>());
         for (int seg = 0; seg <= precision; ++seg) {
             set_segment(b, c, precision, seg, cp, b->d_zero.as<int>());
-            SF_TRY(launch_class_engine(b, c, seg, kModeJK, cp, b->tune[c][seg], s));
+            SF_TRY(launch_class_engine(b, c, seg, kModeJK, cp, b->tune[c][tune_slot(precision, seg)], s));
             SF_TRY(mark());
             slot.push_back(2 * c + seg);
         }
@@ -1533,7 +1538,7 @@ extern "C" int sf_autotune(sf_basis* b, const double* P, int precision, void* st
         ClassParams cp = class_params(b, c, b->d_Ps.as<double>());
         for (int seg = 0; seg <= precision && rc == kOk; ++seg) {
             set_segment(b, c, precision, seg, cp, b->d_zero.as<int>());
-            if (ms) ms[2 * c + seg] = 0.0;
+            if (ms) ms[3 * c + tune_slot(precision, seg)] = 0.0;
             if (Q.nq_all == 0 || cp.nq == 0) continue;
             // candidates are timed on a prefix of the segment: the ranking of
             // launch shapes does not need every quartet, and this bounds the
@@ -1541,40 +1546,48 @@ extern "C" int sf_autotune(sf_basis* b, const double* P, int precision, void* st
             cp.nq = std::min(cp.nq, kTuneQuartets);
             // work items of one launch: quartets, or waves x component slices
             const long long items = class_split(c) ? ((cp.nq + 63) / 64) * 64 * class_nslice(c) : cp.nq;
-            Tune best = b->tune[c][seg];
+            const int slot = tune_slot(precision, seg);
+            Tune best = b->tune[c][slot];
             float best_ms = 1e30f;
             const int blocks[3] = {64, 128, 256};
             const int mults[3] = {0, 2, 8};
-            rc = launch_class_engine(b, c, seg, kModeJK, cp, Tune(), s);  // untimed warm-up
-            for (int bi = 0; bi < 3 && rc == kOk; ++bi)
-                for (int mi = 0; mi < 3 && rc == kOk; ++mi) {
-                    Tune t;
-                    t.block = blocks[bi];
-                    t.grid = mults[mi] * ncu;
-                    const long long need = (items + t.block - 1) / t.block;
-                    if (t.grid > 0 && t.grid >= need) continue;  // same as grid 0
-                    if (hipEventRecord(e0, s) != hipSuccess) {
-                        set_error(kEHip, "sf_autotune: event record failed");
-                        rc = kEHip;
-                        break;
+            // the FP32 segment also tries FP64 arithmetic (never less accurate)
+            const int narith = (seg == 1 && b->engine[c] == 0) ? 2 : 1;
+            for (int ar = 0; ar < narith && rc == kOk; ++ar) {
+                Tune warm;
+                warm.arith64 = ar;
+                rc = launch_class_engine(b, c, seg, kModeJK, cp, warm, s);  // untimed warm-up
+                for (int bi = 0; bi < 3 && rc == kOk; ++bi)
+                    for (int mi = 0; mi < 3 && rc == kOk; ++mi) {
+                        Tune t;
+                        t.block = blocks[bi];
+                        t.grid = mults[mi] * ncu;
+                        t.arith64 = ar;
+                        const long long need = (items + t.block - 1) / t.block;
+                        if (t.grid > 0 && t.grid >= need) continue;  // same as grid 0
+                        if (hipEventRecord(e0, s) != hipSuccess) {
+                            set_error(kEHip, "sf_autotune: event record failed");
+                            rc = kEHip;
+                            break;
+                        }
+                        rc = launch_class_engine(b, c, seg, kModeJK, cp, t, s);
+                        if (rc) break;
+                        float el = 0.f;
+                        if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                            hipEventElapsedTime(&el, e0, e1) != hipSuccess) {
+                            set_error(kEHip, "sf_autotune: event timing failed");
+                            rc = kEHip;
+                            break;
+                        }
+                        if (el < best_ms) {
+                            best_ms = el;
+                            best = t;
+                        }
                     }
-                    rc = launch_class_engine(b, c, seg, kModeJK, cp, t, s);
-                    if (rc) break;
-                    float el = 0.f;
-                    if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-                        hipEventElapsedTime(&el, e0, e1) != hipSuccess) {
-                        set_error(kEHip, "sf_autotune: event timing failed");
-                        rc = kEHip;
-                        break;
-                    }
-                    if (el < best_ms) {
-                        best_ms = el;
-                        best = t;
-                    }
-                }
+            }
             if (rc) break;
-            b->tune[c][seg] = best;
-            if (ms) ms[2 * c + seg] = best_ms;
+            b->tune[c][slot] = best;
+            if (ms) ms[3 * c + slot] = best_ms;
         }
     }
     (void)hipEventDestroy(e0);
@@ -1582,9 +1595,10 @@ extern "C" int sf_autotune(sf_basis* b, const double* P, int precision, void* st
     if (rc) return rc;
     if (shapes)
         for (int c = 0; c < kNumQuartetClass; ++c)
-            for (int seg = 0; seg < 2; ++seg) {
-                shapes[4 * c + 2 * seg] = b->tune[c][seg].block;
-                shapes[4 * c + 2 * seg + 1] = b->tune[c][seg].grid;
+            for (int k = 0; k < 3; ++k) {
+                shapes[9 * c + 3 * k] = b->tune[c][k].block;
+                shapes[9 * c + 3 * k + 1] = b->tune[c][k].grid;
+                shapes[9 * c + 3 * k + 2] = b->tune[c][k].arith64;
             }
     return kOk;
 }
@@ -1595,17 +1609,18 @@ extern "C" int sf_set_tuning(sf_basis* b, const int* shapes)
         set_error(kEInval, "sf_set_tuning: null argument");
         return kEInval;
     }
-    for (int k = 0; k < 2 * kNumQuartetClass; ++k) {
-        const int block = shapes[2 * k], grid = shapes[2 * k + 1];
-        if (block < 64 || block > 256 || block % 64 != 0 || grid < 0) {
-            set_error(kEInval, "sf_set_tuning: entry %d: block %d grid %d", k, block, grid);
+    for (int k = 0; k < 3 * kNumQuartetClass; ++k) {
+        const int block = shapes[3 * k], grid = shapes[3 * k + 1], ar = shapes[3 * k + 2];
+        if (block < 64 || block > 256 || block % 64 != 0 || grid < 0 || ar < 0 || ar > 1) {
+            set_error(kEInval, "sf_set_tuning: entry %d: block %d grid %d arith %d", k, block, grid, ar);
             return kEInval;
         }
     }
     for (int c = 0; c < kNumQuartetClass; ++c)
-        for (int seg = 0; seg < 2; ++seg) {
-            b->tune[c][seg].block = shapes[4 * c + 2 * seg];
-            b->tune[c][seg].grid = shapes[4 * c + 2 * seg + 1];
+        for (int k = 0; k < 3; ++k) {
+            b->tune[c][k].block = shapes[9 * c + 3 * k];
+            b->tune[c][k].grid = shapes[9 * c + 3 * k + 1];
+            b->tune[c][k].arith64 = shapes[9 * c + 3 * k + 2];
         }
     b->drop_graph();
     return kOk;

@@ -66,6 +66,9 @@ struct ShellMap {
 struct Tune {
     int block = 256;
     int grid = 0;
+    // FP32 segment of a mixed build only: 1 = run its quartets with FP64
+    // arithmetic (same primitive threshold), where that is the faster one
+    int arith64 = 0;
 };
 
 // One launch of a quartet-class kernel over a precision segment.

@@ -361,15 +361,17 @@ class Basis:
 
     def autotune(self, P, precision: int = 0):
         """Pick launch shapes per quartet class (sf_autotune).  Returns
-        (shapes[21, 2, 2] as (block, grid), ms[21, 2])."""
+        (shapes[21, 3, 3] as (block, grid, arith) per tuning slot -- 0: FP64
+        builds, 1/2: FP64/FP32 segment of mixed builds -- and ms[21, 3],
+        zero for the slots not tuned by this call)."""
         torch = _torch()
         P = self._check_matrix(P)
-        shapes = np.zeros(4 * NUM_QUARTET_CLASSES, dtype=np.int32)
-        ms = np.zeros(2 * NUM_QUARTET_CLASSES)
+        shapes = np.zeros(9 * NUM_QUARTET_CLASSES, dtype=np.int32)
+        ms = np.zeros(3 * NUM_QUARTET_CLASSES)
         check(lib().sf_autotune(self._h, ctypes.c_void_p(P.data_ptr()), int(precision),
                                 _stream_ptr(torch), shapes, ms))
         torch.cuda.synchronize()
-        return shapes.reshape(NUM_QUARTET_CLASSES, 2, 2), ms.reshape(NUM_QUARTET_CLASSES, 2)
+        return shapes.reshape(NUM_QUARTET_CLASSES, 3, 3), ms.reshape(NUM_QUARTET_CLASSES, 3)
 
     def set_prim_tau(self, tau_prim: float) -> "Basis":
         """Primitive-quartet threshold of the FP32 quartets (< 0: default

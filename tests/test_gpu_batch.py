@@ -102,16 +102,25 @@ def test_single_molecule_graph_and_tuning(gpu, water_ccpvdz):
     from segfault_py_amd import engine
     b = engine.Basis(water_ccpvdz).screen(1e-12, 1e-5)
     P = gpu.tensor(_density(b.nbf, 3), dtype=gpu.float64, device="cuda")
-    J0, K0 = b.jk(P, 1)
+    Jd, Kd = b.jk(P, 1)
     shapes, ms = b.autotune(P, 1)
-    assert shapes.shape == (21, 2, 2) and np.all(shapes[:, :, 0] % 64 == 0)
-    J1, K1 = b.jk(P, 1)
+    assert shapes.shape == (21, 3, 3) and np.all(shapes[:, :, 0] % 64 == 0)
+    assert set(np.unique(shapes[:, :, 2])) <= {0, 1} and not shapes[:, :2, 2].any()
+    J0, K0 = b.jk(P, 1)
     Jg, Kg = b._zeros(), b._zeros()
     for _ in range(3):
         b.jk_graph(P, (Jg, Kg), 1)
         gpu.cuda.synchronize()
         assert bool((Jg == J0).all()) and bool((Kg == K0).all())
-    assert bool((J1 == J0).all()) and bool((K1 == K0).all())
+    # FP64 arithmetic chosen for some FP32 segments only moves J/K by FP32 rounding
+    assert float((J0 - Jd).abs().max()) < 1e-9 and float((K0 - Kd).abs().max()) < 1e-9
+    # FP64 arithmetic in every FP32 segment: closer to the FP64 build
+    s64 = shapes.copy()
+    s64[:, 2, 2] = 1
+    b.set_tuning(s64)
+    Jx, Kx = b.jk(P, 1)
+    J64, K64 = b.jk(P, 0)
+    assert float((Jx - J64).abs().max()) <= float((Jd - J64).abs().max()) + 1e-12
 
 
 @pytest.mark.parametrize("block,grid", [(64, 1), (128, 3), (256, 7), (64, 0)])
@@ -120,10 +129,10 @@ def test_persistent_launch_shapes_bitwise(batch, block, grid):
     the default build's J and K bit for bit, FP64 and mixed, split classes
     included."""
     mols, bb, singles, P, Ps = batch
-    default = np.tile(np.array([256, 0], dtype=np.int32), 42)
+    default = np.tile(np.array([256, 0, 0], dtype=np.int32), 63)
     bb.set_tuning(default)
     ref = [bb.jk(P, p) for p in (0, 1)]
-    bb.set_tuning(np.tile(np.array([block, grid], dtype=np.int32), 42))
+    bb.set_tuning(np.tile(np.array([block, grid, 0], dtype=np.int32), 63))
     for p in (0, 1):
         J, K = bb.jk(P, p)
         assert bool((J == ref[p][0]).all()) and bool((K == ref[p][1]).all()), p
