@@ -158,7 +158,8 @@ def main():
     t0 = time.perf_counter()
     basis = engine.Basis(mol).screen(tau, tau_mixed)
     setup_s = time.perf_counter() - t0
-    work = metrics.class_work(basis, mol)
+    # flops count the primitive quartets the FP64 kernels evaluate
+    work = metrics.class_work(basis, mol, evaluated=basis.prim_quartet_counts(0))
     nocc = int(mol.numbers.sum()) // 2
     P = torch.tensor(synthetic_density(mol.nbf, nocc), dtype=torch.float64, device="cuda")
     J = torch.empty_like(P)

@@ -132,6 +132,12 @@ int sf_set_prim_tau_fp64(sf_basis *b, double tau_prim);
  * quartet class c (0..20); *funcq = surviving function-level quartets. */
 int sf_quartet_counts(const sf_basis *b, long long *counts, long long *funcq,
                       long long *primq);
+/* Primitive quartets the Rys class kernels evaluate in one build of
+ * `precision` under the current screening and primitive thresholds
+ * (no reference counterpart; a measurement aid): counts[2c + seg], seg 0 =
+ * the whole class (precision 0) or its FP64 segment, seg 1 = its FP32
+ * segment (0 for precision 0).  Runs a counting kernel on the null stream. */
+int sf_prim_quartet_counts(sf_basis *b, int precision, long long *counts);
 /* Sorted significant pairs of pair class pc (0..5): shells, Q and the
  * number of primitive pairs kept after primitive-pair screening (any output
  * pointer may be NULL). */

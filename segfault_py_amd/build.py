@@ -17,8 +17,12 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libsfeng.so")
 # (source, extra defines, object name): k_eri.hip is compiled once per
 # quartet class so the 21 classes build in parallel.
+# The d-shell classes (6..20) are further split into their J/K, gradient
+# and McMurchie-Davidson parts (SF_PART), the heaviest compiles.
 UNITS = ([("k_eri.hip", [], "k_eri_dispatch.o")]
-         + [("k_eri.hip", [f"-DSF_CLASS={c}"], f"k_eri_c{c:02d}.o") for c in range(21)]
+         + [("k_eri.hip", [f"-DSF_CLASS={c}"], f"k_eri_c{c:02d}.o") for c in range(6)]
+         + [("k_eri.hip", [f"-DSF_CLASS={c}", f"-DSF_PART={p}"], f"k_eri_c{c:02d}p{p}.o")
+            for c in range(6, 21) for p in range(3)]
          + [("k_util.hip", [], "k_util.o"), ("k_onee.hip", [], "k_onee.o"),
             ("sf_api.hip", [], "sf_api.o")])
 ARCH = os.environ.get("SF_OFFLOAD_ARCH", "gfx950")

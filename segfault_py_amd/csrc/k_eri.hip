@@ -40,14 +40,6 @@ using SchwarzLaunch = int (*)(const int*, const int*, const int*, const int*, in
 SF_FOR_EACH_CLASS(X)
 #undef X
 
-#ifdef SF_CLASS
-
-#define SF_TRY_K(x)                   \
-    do {                              \
-        const int rc_ = (x);          \
-        if (rc_ != kOk) return rc_;   \
-    } while (0)
-
 __device__ __forceinline__ int find_bra(const long long* __restrict__ base, int n, long long q)
 {
     int lo = 0, hi = n;  // base[lo] <= q < base[hi]
@@ -83,6 +75,14 @@ __device__ __forceinline__ QuartetIdx locate(const ClassParams& cp, long long q,
     r.qa = cp.qall_base[i] + jr;
     return r;
 }
+
+#ifdef SF_CLASS
+
+#define SF_TRY_K(x)                   \
+    do {                              \
+        const int rc_ = (x);          \
+        if (rc_ != kOk) return rc_;   \
+    } while (0)
 
 // Copies the node table of NR nodes into LDS (block-wide; every thread of
 // the block must call it) and returns its descriptor; global table when
@@ -530,7 +530,11 @@ static int launch_schwarz_t(const int* sa, const int* sb, const int* poff, const
     }
 }
 
-#define SF_INST(id, a, b, c, d)                                                                 \
+// With SF_PART defined, a class unit holds one part of its kernels (0: J/K
+// and Schwarz, 1: gradient, 2: McMurchie-Davidson), so the heavy classes
+// compile in parallel; without it, all of them.
+#if !defined(SF_PART) || SF_PART == 0
+#define SF_INST_ERI(id, a, b, c, d)                                                             \
     int launch_class_##id(int fp32, int mode, const ClassParams& cp, const Tune& tn,           \
                           hipStream_t s)                                                       \
     {                                                                                          \
@@ -541,15 +545,30 @@ static int launch_schwarz_t(const int* sa, const int* sb, const int* poff, const
                             hipStream_t s)                                                     \
     {                                                                                          \
         return launch_schwarz_t<a, b, c, d>(sa, sb, poff, np, n, prim, xyz, Q, Qp, s);         \
-    }                                                                                          \
+    }
+#else
+#define SF_INST_ERI(id, a, b, c, d)
+#endif
+#if !defined(SF_PART) || SF_PART == 1
+#define SF_INST_GRAD(id, a, b, c, d)                                                            \
     int launch_grad_##id(int fp32, const ClassParams& cp, double* gout, hipStream_t s)         \
     {                                                                                          \
         return launch_grad_t<a, b, c, d>(fp32, cp, gout, s);                                   \
-    }                                                                                          \
+    }
+#else
+#define SF_INST_GRAD(id, a, b, c, d)
+#endif
+#if !defined(SF_PART) || SF_PART == 2
+#define SF_INST_MMD(id, a, b, c, d)                                                             \
     int launch_mmd_##id(int mode, const ClassParams& cp, const Tune& tn, hipStream_t s)        \
     {                                                                                          \
         return launch_mmd_t<a, b, c, d>(mode, cp, tn, s);                                      \
     }
+#else
+#define SF_INST_MMD(id, a, b, c, d)
+#endif
+#define SF_INST(id, a, b, c, d) \
+    SF_INST_ERI(id, a, b, c, d) SF_INST_GRAD(id, a, b, c, d) SF_INST_MMD(id, a, b, c, d)
 #define SF_PICK(id, a, b, c, d) SF_PICK_##id(id, a, b, c, d)
 #if SF_CLASS == 0
 #define SF_PICK_0(id, a, b, c, d) SF_INST(id, a, b, c, d)
@@ -662,6 +681,51 @@ SF_FOR_EACH_CLASS(SF_PICK)
 #else  // dispatch tables
 
 static_assert(quartet_class(pair_class(2, 1), pair_class(1, 1)) == 12, "class numbering");
+
+// Primitive quartets the Rys class kernels evaluate in one segment: the
+// same ket-outer / bra-inner loops and threshold breaks as the screened
+// primitive loops of eri_shell_quartet, counted instead of computed.  One
+// partial sum per block (no atomics).
+__global__ __launch_bounds__(256) void k_count_prims(const ClassParams cp,
+                                                     unsigned long long* __restrict__ part)
+{
+    __shared__ unsigned long long s_sum[256];
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    const long long lane0 = (long long)blockIdx.x * blockDim.x + (threadIdx.x & ~63u);
+    unsigned long long cnt = 0;
+    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x, qw = lane0; q < cp.nq;
+         q += stride, qw += stride) {
+        const QuartetIdx ix = locate(cp, q, qw);
+        const double* bq = cp.primq + cp.bra.poff[ix.i];
+        const double* kq = cp.primq + cp.ket.poff[ix.j];
+        const int nb = cp.bra.np[ix.i], nk = cp.ket.np[ix.j];
+        for (int ik = 0; ik < nk; ++ik) {
+            const double qk = kq[ik];
+            if (nb == 0 || qk * bq[0] < cp.tau_prim) break;
+            int ib = 0;
+            while (ib < nb && !(bq[ib] * qk < cp.tau_prim)) ++ib;
+            cnt += (unsigned long long)ib;
+        }
+    }
+    s_sum[threadIdx.x] = cnt;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) s_sum[threadIdx.x] += s_sum[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = s_sum[0];
+}
+
+int launch_count_prims(const ClassParams& cp, int nblock, unsigned long long* part, hipStream_t s)
+{
+    if (nblock <= 0 || nblock > 65535) {
+        set_error(kEInval, "launch_count_prims: %d blocks", nblock);
+        return kEInval;
+    }
+    hipLaunchKernelGGL(k_count_prims, dim3(nblock), dim3(256), 0, s, cp, part);
+    SF_HIP(hipGetLastError());
+    return kOk;
+}
 
 int launch_eri_class(int qcls, int fp32, int mode, const ClassParams& cp, const Tune& tn,
                      hipStream_t s)

@@ -1302,6 +1302,39 @@ static void set_segment(sf_basis* b, int c, int precision, int seg, ClassParams&
     }
 }
 
+extern "C" int sf_prim_quartet_counts(sf_basis* b, int precision, long long* counts)
+{
+    if (!b || !counts || precision < 0 || precision > 1) {
+        set_error(kEInval, "sf_prim_quartet_counts: bad arguments");
+        return kEInval;
+    }
+    if (!b->screened) {
+        set_error(kEState, "sf_prim_quartet_counts: call sf_screen first");
+        return kEState;
+    }
+    const int nblock = 2048;
+    DevBuf d_part;
+    SF_TRY(d_part.ensure(nblock * sizeof(unsigned long long)));
+    std::vector<unsigned long long> part(nblock);
+    for (int c = 0; c < kNumQuartetClass; ++c) {
+        QuartetClassData& Q = b->qc[c];
+        ClassParams cp = class_params(b, c, nullptr);
+        for (int seg = 0; seg < 2; ++seg) {
+            counts[2 * c + seg] = 0;
+            if (seg > precision || Q.nq_all == 0) continue;
+            set_segment(b, c, precision, seg, cp, b->d_zero.as<int>());
+            if (cp.nq == 0) continue;
+            SF_TRY(launch_count_prims(cp, nblock, d_part.as<unsigned long long>(), 0));
+            SF_HIP(hipMemcpy(part.data(), d_part.p, nblock * sizeof(unsigned long long),
+                             hipMemcpyDeviceToHost));
+            unsigned long long t = 0;
+            for (unsigned long long v : part) t += v;
+            counts[2 * c + seg] = (long long)t;
+        }
+    }
+    return kOk;
+}
+
 extern "C" int sf_eri_dense(sf_basis* b, double* eri)
 {
     if (!b || !eri) {

@@ -162,6 +162,8 @@ int class_nint(int qcls);
 int class_split(int qcls);
 int class_nslice(int qcls);
 int launch_grad_class(int qcls, int fp32, const ClassParams& cp, double* gout, hipStream_t s);
+// per-block counts of the primitive quartets the Rys kernels evaluate in a segment
+int launch_count_prims(const ClassParams& cp, int nblock, unsigned long long* part, hipStream_t s);
 int launch_grad_reduce_bra(const double* gout, int nslice, long long qstride,
                            const long long* base, int nbra, double* part, hipStream_t s);
 int launch_grad_reduce_ket(const double* gout, int nslice, long long qstride,

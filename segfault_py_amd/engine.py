@@ -91,6 +91,7 @@ def lib():
     L.sf_set_tuning.argtypes = [vp, _i32p]
     L.sf_set_prim_tau.argtypes = [vp, ctypes.c_double]
     L.sf_set_prim_tau_fp64.argtypes = [vp, ctypes.c_double]
+    L.sf_prim_quartet_counts.argtypes = [vp, ctypes.c_int, _i64p]
     L.sf_set_engine.argtypes = [vp, ctypes.c_int, ctypes.c_int]
     _LIB = L
     return L
@@ -109,7 +110,7 @@ def exported_symbols() -> list[str]:
             "sf_eri_dense", "sf_jk", "sf_jk_profile", "sf_basis_set_nuclei", "sf_one_electron",
             "sf_grad", "sf_basis_create_batch", "sf_basis_set_layout", "sf_basis_layout",
             "sf_jk_graph", "sf_autotune", "sf_set_tuning", "sf_set_prim_tau", "sf_set_prim_tau_fp64",
-            "sf_set_engine"]
+            "sf_set_engine", "sf_prim_quartet_counts"]
 
 
 def _torch():
@@ -303,6 +304,13 @@ class Basis:
         fq, pq = ctypes.c_longlong(), ctypes.c_longlong()
         check(lib().sf_quartet_counts(self._h, c, ctypes.byref(fq), ctypes.byref(pq)))
         return c.reshape(NUM_QUARTET_CLASSES, 3), fq.value, pq.value
+
+    def prim_quartet_counts(self, precision: int = 0) -> np.ndarray:
+        """Primitive quartets the Rys kernels evaluate per class and segment
+        under the current thresholds (sf_prim_quartet_counts): [21, 2]."""
+        c = np.zeros(2 * NUM_QUARTET_CLASSES, dtype=np.int64)
+        check(lib().sf_prim_quartet_counts(self._h, int(precision), c))
+        return c.reshape(NUM_QUARTET_CLASSES, 2)
 
     def pair_list(self, pc: int, with_nprim: bool = False):
         """Sorted significant pairs of pair class pc: (sa, sb, Q[, nprim])."""

@@ -31,11 +31,14 @@ def flops_per_prim_quartet(la, lb, lc, ld) -> int:
             + ncart(la) * ncart(lb) * ncart(lc) * ncart(ld) * (4 * nr + 2))
 
 
-def class_work(basis, mol=None) -> list[dict]:
+def class_work(basis, mol=None, evaluated=None) -> list[dict]:
     """Per quartet class: shell / primitive / unique-function quartet counts
     (all surviving, and the FP64 part) and algorithmic flops.  Primitive
-    quartets count only the primitive pairs kept by primitive screening,
-    i.e. the ones the kernels evaluate."""
+    quartets count the primitive pairs kept by primitive-pair screening;
+    ``evaluated`` ([21, 2] from Basis.prim_quartet_counts(0)), when given,
+    replaces the all-quartet count by the primitive quartets the kernels
+    evaluate after primitive-quartet screening, so flops count only work
+    done."""
     out = []
     pairs = {}
     for pc in range(6):
@@ -68,6 +71,8 @@ def class_work(basis, mol=None) -> list[dict]:
                     diag = n > idx
                     fq = fq - np.where(diag, pab * pab - pab * (pab + 1) // 2, 0)
                 rec["funcq" + key] = int(fq.sum())
+            if evaluated is not None:
+                rec["primq"] = int(evaluated[c][0])
             fpq = flops_per_prim_quartet(la, lb, lc, ld)
             rec["flops"] = float(rec["primq"] * fpq + 12 * rec["funcq"])
             rec["flops64"] = float(rec["primq64"] * fpq + 12 * rec["funcq64"])

@@ -211,6 +211,31 @@ def test_fp64_primitive_screening(gpu, oracle):
     assert np.max(np.abs(bw.eri_dense() - ref)) < 1e-11
 
 
+def test_prim_quartet_counts(gpu):
+    """The evaluated-primitive-quartet counter: with primitive-quartet
+    screening off it equals the static count of the quartet lists (per
+    class, whole build and per mixed segment); with it on it can only
+    drop primitive quartets, and does."""
+    from segfault_py_amd import engine, metrics, workloads as W
+    mol = W.config_molecule(3)
+    b = engine.Basis(mol).screen(1e-12, 1e-5)
+    b.set_prim_tau_fp64(0.0)
+    b.set_prim_tau(0.0)
+    work = metrics.class_work(b, mol)
+    ev = b.prim_quartet_counts(0)
+    evm = b.prim_quartet_counts(1)
+    assert [w["primq"] for w in work] == ev[:, 0].tolist() and not ev[:, 1].any()
+    assert [w["primq64"] for w in work] == evm[:, 0].tolist()
+    assert (evm.sum(axis=1) == ev[:, 0]).all()
+    assert int(ev.sum()) == b.counts()[2]
+    b.set_prim_tau_fp64(-1.0)
+    b.set_prim_tau(-1.0)
+    ev2 = b.prim_quartet_counts(0)
+    evm2 = b.prim_quartet_counts(1)
+    assert (ev2 <= ev).all() and ev2.sum() < ev.sum()
+    assert (evm2 <= evm).all() and evm2[:, 1].sum() < evm[:, 1].sum()
+
+
 @pytest.mark.parametrize("name,group", [("water", "others"), ("methane", "alkane")])
 def test_scf_energy_with_gpu_jk(gpu, oracle, name, group):
     """SCF with GPU J/K (one-electron integrals from the oracle as inputs)
