@@ -11,6 +11,8 @@ import os
 import shutil
 import subprocess
 import sys
+import tempfile
+import time
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
@@ -57,16 +59,31 @@ def build(verbose: bool = False, extra_flags: list[str] | None = None) -> str:
     failed = []
     running = []
     pending = list(cmds)
-    # longest units first: the d-shell classes dominate the build time
-    pending.sort(key=lambda c: -int(c[0][7:9]) if c[0].startswith("k_eri_c") else 0)
+    # longest units first: the d-shell classes (their gradient and J/K parts)
+    # dominate the build time
+    def cost(c):
+        name = c[0]
+        if not name.startswith("k_eri_c"):
+            return 0
+        part = name[10:11] if name[9:10] == "p" else "0"
+        return int(name[7:9]) * (0.1 if part == "2" else 1.0)
+    pending.sort(key=lambda c: -cost(c))
     while pending or running:
         while pending and len(running) < jobs:
             name, cmd = pending.pop(0)
             if verbose:
                 print(" ".join(cmd), flush=True)
-            running.append((name, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))
-        name, p = running.pop(0)
-        out, _ = p.communicate()
+            log = tempfile.TemporaryFile()
+            running.append((name, subprocess.Popen(cmd, stdout=log, stderr=subprocess.STDOUT), log))
+        # refill as soon as any compile ends (not the oldest one)
+        k = next((k for k, r in enumerate(running) if r[1].poll() is not None), None)
+        if k is None:
+            time.sleep(0.1)
+            continue
+        name, p, log = running.pop(k)
+        log.seek(0)
+        out = log.read()
+        log.close()
         if p.returncode != 0:
             failed.append((name, out.decode(errors="replace")))
         elif verbose and out:
