@@ -256,7 +256,8 @@ def main():
                      "traffic": measured_traffic(dom["ls"]),
                      "kernel": f"k_eri_class<{','.join(map(str, dom['ls']))},double,JK>",
                      "kernel_ms": dom_ms,
-                     "note": "FP64 vector-ALU bound; peak = MI355X FP64 (vector = matrix)"},
+                     "note": "ceiling: FP64 compute (MI355X FP64 vector = matrix rate); flops "
+                             "count the primitive quartets evaluated after screening"},
     }
     # full SCF (DIIS, tol 1e-8) in FP64 and in mixed precision on the same basis
     from segfault_py_amd import hf
