@@ -588,44 +588,101 @@ SF_HD SF_INLINE void grad_bra_component(const PrimPair* __restrict__ bra, int nb
 // One digestion tile of a shell quartet from its raw integrals v (plane
 // stride vs): T = 0 X_ab, 1 X_cd, 2 Y_ac, 3 Y_ad, 4 Y_bc, 5 Y_bd (see
 // digest_quartet).  fac is the degeneracy factor.
+// Geometry of digestion tile T (0..5: AB, CD, AC, AD, BC, BD) of a class:
+// N1 x N2 outputs, contracted against the M1 x M2 density block whose rows
+// and columns belong to shells RS and CS (0 a, 1 b, 2 c, 3 d); RI is the
+// integral index (0 a, 1 b, 2 c) that selects the tile row.
+//   T0 X_ab += 2 v P_cd, T1 X_cd += 2 v P_ab, T2 Y_ac += v P_bd,
+//   T3 Y_ad += v P_bc, T4 Y_bc += v P_ad, T5 Y_bd += v P_ac.
 template <int LA, int LB, int LC, int LD, int T>
-SF_HD SF_INLINE void digest_tile(const double* __restrict__ v, long long vs, double fac,
-                                 const double* __restrict__ P, const QuartetMap& qm,
-                                 double* __restrict__ t)
+struct TileGeom {
+    using S = QuartetShape<LA, LB, LC, LD>;
+    static constexpr int N1 = (T == 0 || T == 2 || T == 3) ? S::NA : (T == 1) ? S::NC : S::NB;
+    static constexpr int N2 = (T == 0) ? S::NB : (T == 2 || T == 4) ? S::NC : S::ND;
+    static constexpr int M1 = (T == 0) ? S::NC : (T == 1) ? S::NA : (T == 2 || T == 3) ? S::NB : S::NA;
+    static constexpr int M2 = (T == 0) ? S::ND : (T == 1) ? S::NB : (T == 2) ? S::ND : (T == 3) ? S::NC : (T == 4) ? S::ND : S::NC;
+    static constexpr int RS = (T == 0) ? 2 : (T == 1) ? 0 : (T == 2 || T == 3) ? 1 : 0;
+    static constexpr int CS = (T == 0) ? 3 : (T == 1) ? 1 : (T == 2) ? 3 : (T == 3) ? 2 : (T == 4) ? 3 : 2;
+    static constexpr int RI = (T == 0 || T == 2 || T == 3) ? 0 : (T == 1) ? 2 : 1;
+};
+
+template <int LA, int LB, int LC, int LD, int T>
+SF_HD SF_INLINE void load_tile_density(const double* __restrict__ P, const QuartetMap& qm,
+                                       double (&Pm)[TileGeom<LA, LB, LC, LD, T>::M1]
+                                                    [TileGeom<LA, LB, LC, LD, T>::M2])
+{
+    using G = TileGeom<LA, LB, LC, LD, T>;
+    const double* Pr = P + qm.row[G::RS] + qm.col[G::CS];
+#pragma unroll
+    for (int x = 0; x < G::M1; ++x)
+#pragma unroll
+        for (int y = 0; y < G::M2; ++y) Pm[x][y] = Pr[(long long)x * qm.ld + y];
+}
+
+// Row r of tile T: row[N2] = the outputs whose RI index is r.  Each output
+// sums its terms in the lexicographic order of (a, b, c, d), so a tile built
+// row by row equals the tile built at once bit for bit.  r may be a runtime
+// (wavefront-uniform) value; it only offsets addresses.
+template <int LA, int LB, int LC, int LD, int T>
+SF_HD SF_INLINE void digest_tile_row(const double* __restrict__ v, long long vs, double fac,
+                                     const double (&Pm)[TileGeom<LA, LB, LC, LD, T>::M1]
+                                                       [TileGeom<LA, LB, LC, LD, T>::M2],
+                                     int r, double* __restrict__ row)
 {
     using S = QuartetShape<LA, LB, LC, LD>;
-    constexpr int N1 = (T == 0 || T == 2 || T == 3) ? S::NA : (T == 1) ? S::NC : S::NB;
-    constexpr int N2 = (T == 0) ? S::NB : (T == 2 || T == 4) ? S::NC : S::ND;
-    // density block contracted against: T0 P_cd, T1 P_ab, T2 P_bd, T3 P_bc, T4 P_ad, T5 P_ac
-    constexpr int M1 = (T == 0) ? S::NC : (T == 1) ? S::NA : (T == 2 || T == 3) ? S::NB : S::NA;
-    constexpr int M2 = (T == 0) ? S::ND : (T == 1) ? S::NB : (T == 2) ? S::ND : (T == 3) ? S::NC : (T == 4) ? S::ND : S::NC;
-    // shells (0 a, 1 b, 2 c, 3 d) of the density block's rows and columns
-    constexpr int RS = (T == 0) ? 2 : (T == 1) ? 0 : (T == 2 || T == 3) ? 1 : 0;
-    constexpr int CS = (T == 0) ? 3 : (T == 1) ? 1 : (T == 2) ? 3 : (T == 3) ? 2 : (T == 4) ? 3 : 2;
-    const double* Pr = P + qm.row[RS] + qm.col[CS];
-    double Pm[M1][M2];
+    using G = TileGeom<LA, LB, LC, LD, T>;
 #pragma unroll
-    for (int x = 0; x < M1; ++x)
-#pragma unroll
-        for (int y = 0; y < M2; ++y) Pm[x][y] = Pr[(long long)x * qm.ld + y];
-#pragma unroll
-    for (int e = 0; e < N1 * N2; ++e) t[e] = 0.0;
-#pragma unroll
-    for (int a = 0; a < S::NA; ++a)
+    for (int e = 0; e < G::N2; ++e) row[e] = 0.0;
+    if constexpr (G::RI == 0) {           // T0, T2, T3: a = r
+        const double* va = v + (long long)r * (S::NB * S::NC * S::ND) * vs;
 #pragma unroll
         for (int b = 0; b < S::NB; ++b)
 #pragma unroll
             for (int c = 0; c < S::NC; ++c)
 #pragma unroll
                 for (int d = 0; d < S::ND; ++d) {
-                    const double x = fac * v[(long long)(((a * S::NB + b) * S::NC + c) * S::ND + d) * vs];
-                    if constexpr (T == 0) t[a * N2 + b] += 2.0 * x * Pm[c][d];
-                    else if constexpr (T == 1) t[c * N2 + d] += 2.0 * x * Pm[a][b];
-                    else if constexpr (T == 2) t[a * N2 + c] += x * Pm[b][d];
-                    else if constexpr (T == 3) t[a * N2 + d] += x * Pm[b][c];
-                    else if constexpr (T == 4) t[b * N2 + c] += x * Pm[a][d];
-                    else t[b * N2 + d] += x * Pm[a][c];
+                    const double x = fac * va[(long long)((b * S::NC + c) * S::ND + d) * vs];
+                    if constexpr (T == 0) row[b] += 2.0 * x * Pm[c][d];
+                    else if constexpr (T == 2) row[c] += x * Pm[b][d];
+                    else row[d] += x * Pm[b][c];
                 }
+    } else if constexpr (G::RI == 2) {    // T1: c = r
+#pragma unroll
+        for (int a = 0; a < S::NA; ++a)
+#pragma unroll
+            for (int b = 0; b < S::NB; ++b)
+#pragma unroll
+                for (int d = 0; d < S::ND; ++d) {
+                    const long long k = (long long)(((a * S::NB + b) * S::NC) * S::ND + d) + (long long)r * S::ND;
+                    const double x = fac * v[k * vs];
+                    row[d] += 2.0 * x * Pm[a][b];
+                }
+    } else {                               // T4, T5: b = r
+#pragma unroll
+        for (int a = 0; a < S::NA; ++a)
+#pragma unroll
+            for (int c = 0; c < S::NC; ++c)
+#pragma unroll
+                for (int d = 0; d < S::ND; ++d) {
+                    const long long k = (long long)((a * S::NB * S::NC + c) * S::ND + d) +
+                                        (long long)r * (S::NC * S::ND);
+                    const double x = fac * v[k * vs];
+                    if constexpr (T == 4) row[c] += x * Pm[a][d];
+                    else row[d] += x * Pm[a][c];
+                }
+    }
+}
+
+template <int LA, int LB, int LC, int LD, int T>
+SF_HD SF_INLINE void digest_tile(const double* __restrict__ v, long long vs, double fac,
+                                 const double* __restrict__ P, const QuartetMap& qm,
+                                 double* __restrict__ t)
+{
+    using G = TileGeom<LA, LB, LC, LD, T>;
+    double Pm[G::M1][G::M2];
+    load_tile_density<LA, LB, LC, LD, T>(P, qm, Pm);
+#pragma unroll
+    for (int r = 0; r < G::N1; ++r) digest_tile_row<LA, LB, LC, LD, T>(v, vs, fac, Pm, r, t + r * G::N2);
 }
 
 // J/K digestion of one shell quartet.  v[NINT] are the integrals in the

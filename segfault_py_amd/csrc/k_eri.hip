@@ -280,20 +280,40 @@ __global__ __launch_bounds__(256) void k_eri_split(const ClassParams cp, double*
     }
 }
 
-// Split classes, pass 2: one wavefront per (64 quartets, digestion tile).
+// Split classes, pass 2: one wavefront per (64 quartets, digestion tile),
+// or, with `rows` (classes with few quartets, where that leaves most of the
+// chip idle), per (64 quartets, tile, tile row).  Same results either way.
+template <int LA, int LB, int LC, int LD>
+struct TileRows {
+    static constexpr int n1(int T)
+    {
+        using S = QuartetShape<LA, LB, LC, LD>;
+        return (T == 0 || T == 2 || T == 3) ? S::NA : (T == 1) ? S::NC : S::NB;
+    }
+    static constexpr int base(int T) { return T == 0 ? 0 : base(T - 1) + n1(T - 1); }
+    static constexpr int total = base(5) + n1(5);
+};
+
 template <int LA, int LB, int LC, int LD>
 __global__ __launch_bounds__(256) void k_digest_split(const ClassParams cp,
-                                                      const double* __restrict__ raw)
+                                                      const double* __restrict__ raw, int rows)
 {
-    using S = QuartetShape<LA, LB, LC, LD>;
-    const long long nwork = ((cp.nq + 63) >> 6) * 6;
+    using TR = TileRows<LA, LB, LC, LD>;
+    const int per = rows ? TR::total : 6;
+    const long long nwork = ((cp.nq + 63) >> 6) * per;
     const long long wstride = ((long long)gridDim.x * blockDim.x) >> 6;
     for (long long gw = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6; gw < nwork;
          gw += wstride) {
-    const int t = __builtin_amdgcn_readfirstlane((int)(gw % 6));
-    const long long q = (gw / 6) * 64 + (threadIdx.x & 63);
+    const int w = __builtin_amdgcn_readfirstlane((int)(gw % per));
+    int t = w, r = 0;
+    if (rows) {
+        t = 0;
+        while (t < 5 && w >= TR::base(t + 1)) ++t;
+        r = w - TR::base(t);
+    }
+    const long long q = (gw / per) * 64 + (threadIdx.x & 63);
     if (q >= cp.nq) continue;
-    const QuartetIdx ix = locate(cp, q, (gw / 6) * 64);
+    const QuartetIdx ix = locate(cp, q, (gw / per) * 64);
     const int i = ix.i, j = ix.j;
     const long long qa = ix.qa;
     const int a = cp.bra.sa[i], b = cp.bra.sb[i];
@@ -303,12 +323,23 @@ __global__ __launch_bounds__(256) void k_digest_split(const ClassParams cp,
     const QuartetMap qm = quartet_map(cp.map, a, b, c, d);
     static_switch<0, 6>(t, [&](auto tc) {
         constexpr int T = decltype(tc)::value;
-        constexpr int NE = S::tile_n1(T) * S::tile_n2(T);
-        double tile[NE];
-        digest_tile<LA, LB, LC, LD, T>(raw + qa, cp.qstride, fac, cp.P, qm, tile);
+        using G = TileGeom<LA, LB, LC, LD, T>;
+        double Pm[G::M1][G::M2];
+        load_tile_density<LA, LB, LC, LD, T>(cp.P, qm, Pm);
         double* dst = cp.tiles + cp.pos[T * cp.qstride + qa];
+        double row[G::N2];
+        if (rows) {
+            digest_tile_row<LA, LB, LC, LD, T>(raw + qa, cp.qstride, fac, Pm, r, row);
 #pragma unroll
-        for (int e = 0; e < NE; ++e) dst[e] = tile[e];
+            for (int e = 0; e < G::N2; ++e) dst[r * G::N2 + e] = row[e];
+        } else {
+#pragma unroll
+            for (int rr = 0; rr < G::N1; ++rr) {
+                digest_tile_row<LA, LB, LC, LD, T>(raw + qa, cp.qstride, fac, Pm, rr, row);
+#pragma unroll
+                for (int e = 0; e < G::N2; ++e) dst[rr * G::N2 + e] = row[e];
+            }
+        }
     });
     }
 }
@@ -419,7 +450,10 @@ static int launch_split(int fp32, int mode, const ClassParams& cp, const Tune& t
     const int wpb = tn.block / 64;
     unsigned nb1 = 0, nb2 = 0;
     SF_TRY_K(grid_blocks(waves * S::NA * S::NB, wpb, tn, &nb1));
-    SF_TRY_K(grid_blocks(waves * 6, wpb, tn, &nb2));
+    // digestion by tile rows when by tiles would give fewer than 1536
+    // wavefronts (6 per CU): measured faster below, slower above
+    const int rows = (waves * 6 < 1536) ? 1 : 0;
+    SF_TRY_K(grid_blocks(waves * (rows ? TileRows<LA, LB, LC, LD>::total : 6), wpb, tn, &nb2));
     double* raw = (mode == kModeRaw) ? cp.out : cp.raw;
     if (!raw) {
         set_error(kEState, "split class without a raw buffer");
@@ -434,7 +468,7 @@ static int launch_split(int fp32, int mode, const ClassParams& cp, const Tune& t
     SF_HIP(hipGetLastError());
     if (mode == kModeJK) {
         hipLaunchKernelGGL((k_digest_split<LA, LB, LC, LD>), dim3(nb2), dim3(tn.block), 0, s, cp,
-                           (const double*)raw);
+                           (const double*)raw, rows);
         SF_HIP(hipGetLastError());
     }
     return kOk;
@@ -505,9 +539,11 @@ static int launch_mmd_t(int mode, const ClassParams& cp, const Tune& tn, hipStre
     SF_HIP(hipGetLastError());
     if (mode == kModeJK) {
         unsigned nb2 = 0;
-        SF_TRY_K(grid_blocks(((cp.nq + 63) / 64) * 6, 4, Tune(), &nb2));
+        const long long waves = (cp.nq + 63) / 64;
+        const int rows = (waves * 6 < 1536) ? 1 : 0;
+        SF_TRY_K(grid_blocks(waves * (rows ? TileRows<LA, LB, LC, LD>::total : 6), 4, Tune(), &nb2));
         hipLaunchKernelGGL((k_digest_split<LA, LB, LC, LD>), dim3(nb2), dim3(256), 0, s, cp,
-                           (const double*)raw);
+                           (const double*)raw, rows);
         SF_HIP(hipGetLastError());
     }
     return kOk;
