@@ -260,10 +260,23 @@ __global__ __launch_bounds__(256) void k_reduce_chunks(const DestList dl,
     double acc[NE];
 #pragma unroll
     for (int k = 0; k < NE; ++k) acc[k] = 0.0;
-    for (int e = lane; e < cnt; e += 64) {
-        const double* t = src + (long long)e * NE;
+    // each lane adds tiles lane, lane + 64, lane + 128, ... in that order;
+    // U tiles' loads are issued before their adds
+    constexpr int U = reduce_unroll(NE);
+    for (int e0 = lane; e0 < cnt; e0 += 64 * U) {
+        double v[U][NE];
 #pragma unroll
-        for (int k = 0; k < NE; ++k) acc[k] += t[k];
+        for (int u = 0; u < U; ++u) {
+            const int e = e0 + 64 * u;
+            const double* t = src + (long long)e * NE;
+#pragma unroll
+            for (int k = 0; k < NE; ++k) v[u][k] = (e < cnt) ? t[k] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (e0 + 64 * u < cnt)
+#pragma unroll
+                for (int k = 0; k < NE; ++k) acc[k] += v[u][k];
     }
 #pragma unroll
     for (int k = 0; k < NE; ++k)

@@ -1016,7 +1016,8 @@ extern "C" int sf_screen(sf_basis* b, double tau, double tau_mixed)
                 key_shells((int)m, k, s1, s2, xy);
                 const long long n = sege[k] - segb[k];
                 const int g = 3 * b->L[s1] + b->L[s2];
-                const long long nch = (n + kReduceChunk - 1) / kReduceChunk;
+                const long long cw = reduce_chunk_tiles(cart_n(b->L[s1]) * cart_n(b->L[s2]));
+                const long long nch = (n + cw - 1) / cw;
                 c.acc += (unsigned long long)n * (cart_n(b->L[s1]) * cart_n(b->L[s2]));
                 c.nd[g] += 1;
                 c.nch[g] += nch;
@@ -1080,8 +1081,9 @@ extern "C" int sf_screen(sf_basis* b, double tau, double tau_mixed)
                 ds1[g][d] = s1;
                 ds2[g][d] = s2;
                 dxy[g][d] = xy;
-                // chunks of at most kReduceChunk tiles; several -> partial slots
-                const int nch = (int)((n + kReduceChunk - 1) / kReduceChunk);
+                // chunks of at most reduce_chunk_tiles(NE) tiles; several -> partial slots
+                const long long cw = reduce_chunk_tiles(key_ne[k]);
+                const int nch = (int)((n + cw - 1) / cw);
                 const int pb = (int)o.npart[g];
                 if (nch > 1) {
                     const long long j = o.nm[g]++;
@@ -1092,8 +1094,8 @@ extern "C" int sf_screen(sf_basis* b, double tau, double tau_mixed)
                 }
                 for (int c = 0; c < nch; ++c) {
                     const long long j = o.nch[g]++;
-                    ch_src[g][j] = key_base[k] + (unsigned long long)c * kReduceChunk * key_ne[k];
-                    ch_n[g][j] = (int)std::min<long long>(kReduceChunk, n - (long long)c * kReduceChunk);
+                    ch_src[g][j] = key_base[k] + (unsigned long long)c * cw * key_ne[k];
+                    ch_n[g][j] = (int)std::min<long long>(cw, n - (long long)c * cw);
                     ch_dest[g][j] = d;
                     ch_part[g][j] = nch > 1 ? pb + c : -1;
                 }

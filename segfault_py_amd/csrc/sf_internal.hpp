@@ -117,11 +117,15 @@ struct MolLayout {
 // shells of each destination block and which matrix it belongs to (0 = X
 // for J, 1 = Y for K).  A destination's tiles (a contiguous run in the
 // destination-sorted tile buffer) are summed in chunks of at most
-// kReduceChunk tiles, one wavefront per chunk; a destination with several
-// chunks gets partial sums (slots ch_part) that a second pass adds in chunk
-// order.  Both passes have a fixed order: the result is reproducible bit
-// for bit.
-constexpr int kReduceChunk = 256;
+// reduce_chunk_tiles(NE) tiles of NE = N1 N2 doubles, one wavefront per
+// chunk; a destination with several chunks gets partial sums (slots
+// ch_part) that a second pass adds in chunk order.  Both passes have a
+// fixed order: the result is reproducible bit for bit.  Small tiles take
+// long chunks (more loads in flight per wavefront), large ones short
+// chunks (more wavefronts, fewer registers).
+constexpr int reduce_chunk_tiles(int ne) { return ne == 1 ? 1024 : ne <= 3 ? 512 : 256; }
+// tiles whose loads a lane issues before adding them
+constexpr int reduce_unroll(int ne) { return ne <= 3 ? 4 : 1; }
 struct DestList {
     const int* s1;
     const int* s2;
