@@ -106,6 +106,11 @@ struct QuartetShape {
     // classes with more than 81 integrals per quartet run one wavefront per
     // bra component pair (eri_bra_component) and digest in a second kernel
     static constexpr bool SPLIT = NINT > 81;
+    // split classes whose NB*NC*ND integrals of one bra component of A fit
+    // in registers run one wavefront per (64 quartets, component of A)
+    // (eri_bra_row), the others one per (64 quartets, component pair)
+    static constexpr bool SPLIT_ROWS = SPLIT && NB * NC * ND <= 54;
+    static constexpr int NSLICE_ERI = SPLIT_ROWS ? NA : NA * NB;
     static constexpr int tile_n1(int T) { return (T == 0 || T == 2 || T == 3) ? NA : (T == 1) ? NC : NB; }
     static constexpr int tile_n2(int T) { return (T == 0) ? NB : (T == 2 || T == 4) ? NC : ND; }
     static constexpr int tile_off(int T)
@@ -392,6 +397,152 @@ SF_HD SF_INLINE void eri_bra_component(const PrimPair* __restrict__ bra, int nbr
                         acc[fc * S::ND + fd] += Ix[cart_l(LC, fc, 0)][cart_l(LD, fd, 0)] *
                                                 Iy[cart_l(LC, fc, 1)][cart_l(LD, fd, 1)] *
                                                 Iz[cart_l(LC, fc, 2)][cart_l(LD, fd, 2)];
+            }
+        }
+    }
+}
+
+// One Cartesian direction for fixed bra exponent AK and every BK <= LB:
+// I[bk][ic][id] = I(AK, bk, ic, id).  The 2-D recurrence rows up to AK+bk
+// and the transfers are the same operations, in the same order, as
+// dir_component<AK, bk> (identical values up to the compiler's choice of
+// fused multiply-adds); here the recurrence and the ket transfer are done
+// once for all bk.
+template <int AK, int LB, int LC, int LD, typename R>
+SF_HD SF_INLINE void dir_row(R g00, R c00, R c00p, R b10, R b01, R b00, R ab, const R* CDp,
+                             R (&I)[LB + 1][LC + 1][LD + 1])
+{
+    constexpr int NN = AK + LB + 1, NM = LC + LD + 1;
+    R G[NN][NM];
+    G[0][0] = g00;
+    if (NN > 1) G[1][0] = c00 * g00;
+#pragma unroll
+    for (int n = 1; n < NN - 1; ++n) G[n + 1][0] = (R)n * b10 * G[n - 1][0] + c00 * G[n][0];
+    if (NM > 1) {
+#pragma unroll
+        for (int n = 0; n < NN; ++n) {
+            G[n][1] = c00p * G[n][0];
+            if (n > 0) G[n][1] += (R)n * b00 * G[n - 1][0];
+        }
+#pragma unroll
+        for (int m = 1; m < NM - 1; ++m)
+#pragma unroll
+            for (int n = 0; n < NN; ++n) {
+                R g = (R)m * b01 * G[n][m - 1] + c00p * G[n][m];
+                if (n > 0) g += (R)n * b00 * G[n - 1][m];
+                G[n][m + 1] = g;
+            }
+    }
+    R H[LB + 1][LC + 1][LD + 1];
+#pragma unroll
+    for (int nn = 0; nn <= LB; ++nn)
+#pragma unroll
+        for (int ic = 0; ic <= LC; ++ic)
+#pragma unroll
+            for (int id = 0; id <= LD; ++id) {
+                R s = (R)0;
+#pragma unroll
+                for (int m = 0; m <= id; ++m)
+                    s += (R)binom_c(id, m) * CDp[id - m] * G[AK + nn][ic + m];
+                H[nn][ic][id] = s;
+            }
+    R abp[LB + 1];
+    abp[0] = (R)1;
+#pragma unroll
+    for (int e = 1; e <= LB; ++e) abp[e] = abp[e - 1] * ab;
+#pragma unroll
+    for (int bk = 0; bk <= LB; ++bk)
+#pragma unroll
+        for (int ic = 0; ic <= LC; ++ic)
+#pragma unroll
+            for (int id = 0; id <= LD; ++id) {
+                R s = (R)0;
+#pragma unroll
+                for (int nn = 0; nn <= bk; ++nn) s += (R)binom_c(bk, nn) * abp[bk - nn] * H[nn][ic][id];
+                I[bk][ic][id] = s;
+            }
+}
+
+// (fa fb | fc fd) for ONE bra component FA of shell A, every component fb
+// of shell B and every ket component, accumulated into acc[NB*NC*ND]
+// (index (fb*NC + fc)*ND + fd).  The same sums as eri_bra_component<FA, fb>
+// for each fb, with the nodes, the 2-D recurrence and the ket transfer
+// evaluated once per FA instead of once per (FA, fb).
+template <int LA, int LB, int LC, int LD, int FA, typename R, bool PS = false>
+SF_HD SF_INLINE void eri_bra_row(const PrimPair* __restrict__ bra, int nbra,
+                                 const PrimPair* __restrict__ ket, int nket,
+                                 const double* A, const double* B, const double* C,
+                                 const double* D, R* __restrict__ acc,
+                                 const NodeTable& nt = global_node_table<QuartetShape<LA, LB, LC, LD>::NR>(),
+                                 const double* __restrict__ bq = nullptr,
+                                 const double* __restrict__ kq = nullptr,
+                                 double tau_prim = 0.0)
+{
+    using S = QuartetShape<LA, LB, LC, LD>;
+    constexpr int NR = S::NR;
+    constexpr int ax = cart_l(LA, FA, 0), ay = cart_l(LA, FA, 1), az = cart_l(LA, FA, 2);
+    R CDp[3][LD + 1];
+    R AB[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const R cd = (R)(C[k] - D[k]);
+        AB[k] = (R)(A[k] - B[k]);
+        CDp[k][0] = (R)1;
+#pragma unroll
+        for (int e = 1; e <= LD; ++e) CDp[k][e] = CDp[k][e - 1] * cd;
+    }
+    for (int ik = 0; ik < nket; ++ik) {
+        double qk = 0.0;
+        if constexpr (PS) {
+            qk = kq[ik];
+            if (nbra == 0 || qk * bq[0] < tau_prim) break;
+        }
+        const PrimPair pk = ket[ik];
+        const double q = pk.p;
+        const double QC[3] = {pk.Px - C[0], pk.Py - C[1], pk.Pz - C[2]};
+        for (int ib = 0; ib < nbra; ++ib) {
+            if constexpr (PS) {
+                if (bq[ib] * qk < tau_prim) break;
+            }
+            const PrimPair pb = bra[ib];
+            const double p = pb.p;
+            const double PA[3] = {pb.Px - A[0], pb.Py - A[1], pb.Pz - A[2]};
+            const double pq = p + q;
+            const double rs = fast_rsqrt(pq);
+            const double ipq = rs * rs;
+            const double rho = p * q * ipq;
+            const double PQ[3] = {pb.Px - pk.Px, pb.Py - pk.Py, pb.Pz - pk.Pz};
+            const double X = rho * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]);
+            const double pref = 34.986836655249725 * pb.Kp * pk.Kp * rs;
+            double u[NR], w[NR];
+            rys_nodes_tab<NR>(X, nt, u, w);
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                const double t2 = u[r];
+                const R b00 = (R)(0.5 * ipq * t2);
+                const R b10 = (R)(0.5 * pb.ip * (1.0 - q * ipq * t2));
+                const R b01 = (R)(0.5 * pk.ip * (1.0 - p * ipq * t2));
+                R c00[3], c00p[3];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    c00[k] = (R)(PA[k] - q * ipq * PQ[k] * t2);
+                    c00p[k] = (R)(QC[k] + p * ipq * PQ[k] * t2);
+                }
+                R Ix[LB + 1][LC + 1][LD + 1], Iy[LB + 1][LC + 1][LD + 1], Iz[LB + 1][LC + 1][LD + 1];
+                dir_row<ax, LB, LC, LD, R>((R)(pref * w[r]), c00[0], c00p[0], b10, b01, b00, AB[0],
+                                           CDp[0], Ix);
+                dir_row<ay, LB, LC, LD, R>((R)1, c00[1], c00p[1], b10, b01, b00, AB[1], CDp[1], Iy);
+                dir_row<az, LB, LC, LD, R>((R)1, c00[2], c00p[2], b10, b01, b00, AB[2], CDp[2], Iz);
+#pragma unroll
+                for (int fb = 0; fb < S::NB; ++fb)
+#pragma unroll
+                    for (int fc = 0; fc < S::NC; ++fc)
+#pragma unroll
+                        for (int fd = 0; fd < S::ND; ++fd)
+                            acc[(fb * S::NC + fc) * S::ND + fd] +=
+                                Ix[cart_l(LB, fb, 0)][cart_l(LC, fc, 0)][cart_l(LD, fd, 0)] *
+                                Iy[cart_l(LB, fb, 1)][cart_l(LC, fc, 1)][cart_l(LD, fd, 1)] *
+                                Iz[cart_l(LB, fb, 2)][cart_l(LC, fc, 2)][cart_l(LD, fd, 2)];
             }
         }
     }

@@ -246,7 +246,8 @@ template <int LA, int LB, int LC, int LD, typename R>
 __global__ __launch_bounds__(256) void k_eri_split(const ClassParams cp, double* __restrict__ raw)
 {
     using S = QuartetShape<LA, LB, LC, LD>;
-    constexpr int NS = S::NA * S::NB, NK = S::NC * S::ND;
+    constexpr int NS = S::NSLICE_ERI, NK = S::NC * S::ND;
+    constexpr int NOUT = S::SPLIT_ROWS ? S::NB * NK : NK;  // integrals per lane
     __shared__ double s_tab[node_table_lds(S::NR) > 0 ? node_table_lds(S::NR) : 1];
     const NodeTable nt = stage_node_table<S::NR>(s_tab);
     const long long nwork = ((cp.nq + 63) >> 6) * NS;
@@ -265,18 +266,27 @@ __global__ __launch_bounds__(256) void k_eri_split(const ClassParams cp, double*
     const double B[3] = {cp.shell_xyz[3 * b], cp.shell_xyz[3 * b + 1], cp.shell_xyz[3 * b + 2]};
     const double C[3] = {cp.shell_xyz[3 * c], cp.shell_xyz[3 * c + 1], cp.shell_xyz[3 * c + 2]};
     const double D[3] = {cp.shell_xyz[3 * d], cp.shell_xyz[3 * d + 1], cp.shell_xyz[3 * d + 2]};
-    R acc[NK];
+    R acc[NOUT];
 #pragma unroll
-    for (int k = 0; k < NK; ++k) acc[k] = (R)0;
+    for (int k = 0; k < NOUT; ++k) acc[k] = (R)0;
     static_switch<0, NS>(s, [&](auto sc) {
-        constexpr int FA = decltype(sc)::value / S::NB, FB = decltype(sc)::value % S::NB;
         constexpr bool PS = true;
-        eri_bra_component<LA, LB, LC, LD, FA, FB, R, PS>(
-            cp.prim + cp.bra.poff[i], cp.bra.np[i], cp.prim + cp.ket.poff[j], cp.ket.np[j], A, B, C,
-            D, acc, nt, cp.primq + cp.bra.poff[i], cp.primq + cp.ket.poff[j], cp.tau_prim);
+        if constexpr (S::SPLIT_ROWS) {
+            constexpr int FA = decltype(sc)::value;
+            eri_bra_row<LA, LB, LC, LD, FA, R, PS>(
+                cp.prim + cp.bra.poff[i], cp.bra.np[i], cp.prim + cp.ket.poff[j], cp.ket.np[j], A, B,
+                C, D, acc, nt, cp.primq + cp.bra.poff[i], cp.primq + cp.ket.poff[j], cp.tau_prim);
+        } else {
+            constexpr int FA = decltype(sc)::value / S::NB, FB = decltype(sc)::value % S::NB;
+            eri_bra_component<LA, LB, LC, LD, FA, FB, R, PS>(
+                cp.prim + cp.bra.poff[i], cp.bra.np[i], cp.prim + cp.ket.poff[j], cp.ket.np[j], A, B,
+                C, D, acc, nt, cp.primq + cp.bra.poff[i], cp.primq + cp.ket.poff[j], cp.tau_prim);
+        }
     });
+    // raw plane of integral (fa fb | fc fd): ((fa*NB + fb)*NK + fc*ND + fd);
+    // a row slice s = fa covers planes s*NB*NK .. (s+1)*NB*NK - 1
 #pragma unroll
-    for (int k = 0; k < NK; ++k) raw[(long long)(s * NK + k) * cp.qstride + qa] = (double)acc[k];
+    for (int k = 0; k < NOUT; ++k) raw[(long long)(s * NOUT + k) * cp.qstride + qa] = (double)acc[k];
     }
 }
 
@@ -449,7 +459,7 @@ static int launch_split(int fp32, int mode, const ClassParams& cp, const Tune& t
     const long long waves = (cp.nq + 63) / 64;
     const int wpb = tn.block / 64;
     unsigned nb1 = 0, nb2 = 0;
-    SF_TRY_K(grid_blocks(waves * S::NA * S::NB, wpb, tn, &nb1));
+    SF_TRY_K(grid_blocks(waves * S::NSLICE_ERI, wpb, tn, &nb1));
     // digestion by tile rows when by tiles would give fewer than 1536
     // wavefronts (6 per CU): measured faster below, slower above
     const int rows = (waves * 6 < 1536) ? 1 : 0;

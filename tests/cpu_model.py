@@ -31,6 +31,7 @@ def harness():
         H.h_rys_kernel.argtypes = [ctypes.c_int, ctypes.c_double, _f64p, _f64p]
         H.h_digest.argtypes = [ctypes.c_int] * 4 + [_f64p, _f64p, ctypes.c_int, _i32p, _f64p]
         H.h_eri_split.argtypes = H.h_eri.argtypes
+        H.h_eri_split_pairs.argtypes = H.h_eri.argtypes
         H.h_digest_tiles.argtypes = [ctypes.c_int] * 4 + [_f64p, ctypes.c_double, _f64p,
                                                           ctypes.c_int, _i32p, _f64p]
         H.h_rys.argtypes = [ctypes.c_int, ctypes.c_double, _f64p, _f64p]
@@ -64,10 +65,14 @@ def prim_pairs(sa, sb):
     return np.ascontiguousarray(np.array(out).reshape(-1))
 
 
-def shell_quartet(shells, a, b, c, d, fp32=False, split=False):
+def shell_quartet(shells, a, b, c, d, fp32=False, split=False, pairs=False):
+    """Integrals of one shell quartet through the kernels' code paths: the
+    full-class path, the split path (per component of A where the class
+    allows, as the GPU), or with pairs the per-bra-component-pair path."""
     S = [shells[a], shells[b], shells[c], shells[d]]
     out = np.zeros(int(np.prod([s.nfunc for s in S])))
-    fn = harness().h_eri_split if split else harness().h_eri
+    H = harness()
+    fn = H.h_eri_split_pairs if pairs else H.h_eri_split if split else H.h_eri
     rc = fn(S[0].L, S[1].L, S[2].L, S[3].L, prim_pairs(S[0], S[1]),
                          S[0].nprim * S[1].nprim, prim_pairs(S[2], S[3]), S[2].nprim * S[3].nprim,
                          S[0].center, S[1].center, S[2].center, S[3].center, out, int(fp32))

@@ -51,11 +51,37 @@ static void run(const double* bra, int nb, const double* ket, int nk, const doub
     }
 }
 
+// The split kernels' decomposition: per component of A (eri_bra_row) where
+// the class allows it, else per bra component pair; pairs = 1 forces the
+// per-pair path.
 template <int LA, int LB, int LC, int LD>
 static void run_split(const double* bra, int nb, const double* ket, int nk, const double* A,
-                      const double* B, const double* C, const double* D, double* out, int fp32)
+                      const double* B, const double* C, const double* D, double* out, int fp32,
+                      int pairs = 0)
 {
     using S = QuartetShape<LA, LB, LC, LD>;
+    if constexpr (S::SPLIT_ROWS) {
+        if (!pairs) {
+            constexpr int NO = S::NB * S::NC * S::ND;
+            for (int s = 0; s < S::NA; ++s) {
+                static_switch<0, S::NA>(s, [&](auto sc) {
+                    constexpr int FA = decltype(sc)::value;
+                    if (fp32) {
+                        float acc[NO] = {};
+                        eri_bra_row<LA, LB, LC, LD, FA, float>((const PrimPair*)bra, nb,
+                                                               (const PrimPair*)ket, nk, A, B, C, D, acc);
+                        for (int k = 0; k < NO; ++k) out[s * NO + k] = acc[k];
+                    } else {
+                        double acc[NO] = {};
+                        eri_bra_row<LA, LB, LC, LD, FA, double>((const PrimPair*)bra, nb,
+                                                                (const PrimPair*)ket, nk, A, B, C, D, acc);
+                        for (int k = 0; k < NO; ++k) out[s * NO + k] = acc[k];
+                    }
+                });
+            }
+            return;
+        }
+    }
     for (int s = 0; s < S::NA * S::NB; ++s) {
         static_switch<0, S::NA * S::NB>(s, [&](auto sc) {
             constexpr int FA = decltype(sc)::value / S::NB, FB = decltype(sc)::value % S::NB;
@@ -144,6 +170,20 @@ extern "C" int h_eri_split(int la, int lb, int lc, int ld, const double* bra, in
 #define X(a, b, c, d)                                                          \
     if (la == a && lb == b && lc == c && ld == d) {                            \
         run_split<a, b, c, d>(bra, nb, ket, nk, A, B, C, D, out, fp32);        \
+        return 0;                                                              \
+    }
+    SF_CLASSES(X)
+#undef X
+    return 1;
+}
+
+extern "C" int h_eri_split_pairs(int la, int lb, int lc, int ld, const double* bra, int nb,
+                                 const double* ket, int nk, const double* A, const double* B,
+                                 const double* C, const double* D, double* out, int fp32)
+{
+#define X(a, b, c, d)                                                          \
+    if (la == a && lb == b && lc == c && ld == d) {                            \
+        run_split<a, b, c, d>(bra, nb, ket, nk, A, B, C, D, out, fp32, 1);     \
         return 0;                                                              \
     }
     SF_CLASSES(X)

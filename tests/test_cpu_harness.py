@@ -162,7 +162,8 @@ def test_ket_count_builder_prefix_and_caps():
 
 
 def test_split_bra_component_path_equals_full(water_ccpvdz):
-    """eri_bra_component (one wavefront per bra component pair) reproduces
+    """The split path -- eri_bra_row (one wavefront per component of A) or
+    eri_bra_component (one per bra component pair) -- reproduces
     eri_shell_quartet for every class, FP64 and FP32."""
     shells = water_ccpvdz.shells()
     last = {}
@@ -181,6 +182,11 @@ def test_split_bra_component_path_equals_full(water_ccpvdz):
                     assert np.max(np.abs(full - split)) <= 1e-14 * max(1.0, np.max(np.abs(full)))
                     f32 = CM.shell_quartet(shells, *q, fp32=True, split=True)
                     assert np.max(np.abs(f32 - full)) <= 1e-5 * max(1.0, np.max(np.abs(full)))
+                    # per component of A (the kernels' split path where the class
+                    # allows) vs per bra component pair: the same operations in the
+                    # same order, bit for bit on the host (no FMA contraction here)
+                    pairs = CM.shell_quartet(shells, *q, split=True, pairs=True)
+                    assert np.array_equal(split, pairs)
                     n += 1
     assert n == 21
 
