@@ -53,6 +53,8 @@ def parse():
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the 256-molecule batched SCF measurement")
     ap.add_argument("--batch-size", type=int, default=256)
+    ap.add_argument("--builds-only", action="store_true",
+                    help="autotune, then only W + K FP64 J/K builds (for kernel tracing)")
     ap.add_argument("--only-batch", action="store_true",
                     help="run only the batched SCF workload and print its JSON")
     ap.add_argument("--workload", type=int, default=1, help="workload index (1 = 20 waters)")
@@ -158,12 +160,27 @@ def main():
     t0 = time.perf_counter()
     basis = engine.Basis(mol).screen(tau, tau_mixed)
     setup_s = time.perf_counter() - t0
-    # flops count the primitive quartets the FP64 kernels evaluate
-    work = metrics.class_work(basis, mol, evaluated=basis.prim_quartet_counts(0))
     nocc = int(mol.numbers.sum()) // 2
     P = torch.tensor(synthetic_density(mol.nbf, nocc), dtype=torch.float64, device="cuda")
     J = torch.empty_like(P)
     K = torch.empty_like(P)
+    if args.builds_only:
+        # W warm-up and K timed FP64 builds, nothing else: each build starts
+        # with the density-scaling kernel, which splits a kernel trace per build
+        basis.autotune(P, 0)
+        for _ in range(args.warmup):
+            basis.jk(P, 0, out=(J, K))
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            basis.jk(P, 0, out=(J, K))
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t) / args.steps * 1e3
+        print(json.dumps({"builds_only": {"workload": mol.nbf, "ms_per_build": ms,
+                                          "builds": args.steps}}))
+        return
+    # flops count the primitive quartets the FP64 kernels evaluate
+    work = metrics.class_work(basis, mol, evaluated=basis.prim_quartet_counts(0))
 
     basis.autotune(P, 0)
     tuned = basis.autotune(P, 1)[0]      # shapes of every slot after both tunings
