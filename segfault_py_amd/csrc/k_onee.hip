@@ -77,21 +77,75 @@ int launch_one_electron(int pc, const int* sa, const int* sb, const int* poff, c
     return kOk;
 }
 
-// Gradient of the one-electron energy: one thread per shell pair writes its
-// row of 3*natom derivatives (one_electron_grad_pair); rows are summed by
-// k_grad_rows in a fixed order.
+// Gradient of the one-electron energy, in two passes over a pair class:
+//  k_grad_onee_v:  one thread per (shell pair, nucleus c) computes the
+//                  attraction derivatives for c: writes d/dC into the pair's
+//                  row at atom c and d/dA, d/dB into part[(i*natom + c)*6];
+//  k_grad_onee_st: one thread per shell pair adds the overlap/kinetic
+//                  derivatives and the per-nucleus d/dA, d/dB in nucleus
+//                  order into the row at the pair's atoms.
+// Rows are summed by k_grad_rows in a fixed order (one_electron_grad_pair
+// is the same computation on one thread).
 template <int LA, int LB>
-__global__ __launch_bounds__(64) void k_grad_onee(const int* __restrict__ sa, const int* __restrict__ sb,
-                                                  const int* __restrict__ poff, const int* __restrict__ np,
-                                                  int n, const PrimPair* __restrict__ prim,
-                                                  const double* __restrict__ xyz,
-                                                  const ShellMap map,
-                                                  const int* __restrict__ shell_atom, int natom,
-                                                  const double* __restrict__ Zc,
-                                                  const double* __restrict__ Cxyz,
-                                                  const double* __restrict__ Ps,
-                                                  const double* __restrict__ Ws,
-                                                  double* __restrict__ rows)
+__device__ __forceinline__ void load_pair_blocks(const ShellMap& map, int a, int b,
+                                                 const double* __restrict__ Ps,
+                                                 const double* __restrict__ Ws, double* Pb,
+                                                 double* Wb)
+{
+    constexpr int NA = cart_n(LA), NB = cart_n(LB);
+    const long long ld = map.ld[a];
+#pragma unroll
+    for (int fa = 0; fa < NA; ++fa)
+#pragma unroll
+        for (int fb = 0; fb < NB; ++fb) {
+            const long long ij = map.row[a] + fa * ld + map.col[b] + fb;
+            Pb[fa * NB + fb] = Ps[ij];
+            if (Wb) Wb[fa * NB + fb] = Ws[ij];
+        }
+}
+
+template <int LA, int LB>
+__global__ __launch_bounds__(64) void k_grad_onee_v(const int* __restrict__ sa, const int* __restrict__ sb,
+                                                    const int* __restrict__ poff, const int* __restrict__ np,
+                                                    int n, const PrimPair* __restrict__ prim,
+                                                    const double* __restrict__ xyz, const ShellMap map,
+                                                    int natom, const double* __restrict__ Zc,
+                                                    const double* __restrict__ Cxyz,
+                                                    const double* __restrict__ Ps,
+                                                    double* __restrict__ rows, double* __restrict__ part)
+{
+    constexpr int NA = cart_n(LA), NB = cart_n(LB);
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)n * natom) return;
+    const int i = (int)(t / natom), c = (int)(t % natom);
+    const int a = sa[i], b = sb[i];
+    const double A[3] = {xyz[3 * a], xyz[3 * a + 1], xyz[3 * a + 2]};
+    const double B[3] = {xyz[3 * b], xyz[3 * b + 1], xyz[3 * b + 2]};
+    double Pb[NA * NB];
+    load_pair_blocks<LA, LB>(map, a, b, Ps, nullptr, Pb, nullptr);
+    double gA[3], gB[3], gC[3];
+    one_electron_grad_v<LA, LB>(prim + poff[i], np[i], A, B, c, Zc, Cxyz, Pb, (a == b) ? 1.0 : 2.0,
+                                gA, gB, gC);
+    double* row = rows + (long long)i * 3 * natom;
+    double* pp = part + ((long long)i * natom + c) * 6;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        row[3 * c + d] = gC[d];
+        pp[d] = gA[d];
+        pp[3 + d] = gB[d];
+    }
+}
+
+template <int LA, int LB>
+__global__ __launch_bounds__(64) void k_grad_onee_st(const int* __restrict__ sa, const int* __restrict__ sb,
+                                                     const int* __restrict__ poff, const int* __restrict__ np,
+                                                     int n, const PrimPair* __restrict__ prim,
+                                                     const double* __restrict__ xyz, const ShellMap map,
+                                                     const int* __restrict__ shell_atom, int natom,
+                                                     const double* __restrict__ Ps,
+                                                     const double* __restrict__ Ws,
+                                                     double* __restrict__ rows,
+                                                     const double* __restrict__ part)
 {
     constexpr int NA = cart_n(LA), NB = cart_n(LB);
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -100,31 +154,45 @@ __global__ __launch_bounds__(64) void k_grad_onee(const int* __restrict__ sa, co
     const double A[3] = {xyz[3 * a], xyz[3 * a + 1], xyz[3 * a + 2]};
     const double B[3] = {xyz[3 * b], xyz[3 * b + 1], xyz[3 * b + 2]};
     double Pb[NA * NB], Wb[NA * NB];
-    const long long ld = map.ld[a];
+    load_pair_blocks<LA, LB>(map, a, b, Ps, Ws, Pb, Wb);
+    double gA[3] = {0.0, 0.0, 0.0}, gB[3] = {0.0, 0.0, 0.0};
+    one_electron_grad_st<LA, LB>(prim + poff[i], np[i], A, B, Pb, Wb, (a == b) ? 1.0 : 2.0, gA, gB);
+    const double* pp = part + (long long)i * natom * 6;
+    for (int c = 0; c < natom; ++c)
 #pragma unroll
-    for (int fa = 0; fa < NA; ++fa)
-#pragma unroll
-        for (int fb = 0; fb < NB; ++fb) {
-            const long long ij = map.row[a] + fa * ld + map.col[b] + fb;
-            Pb[fa * NB + fb] = Ps[ij];
-            Wb[fa * NB + fb] = Ws[ij];
+        for (int d = 0; d < 3; ++d) {
+            gA[d] += pp[6 * c + d];
+            gB[d] += pp[6 * c + 3 + d];
         }
-    one_electron_grad_pair<LA, LB>(prim + poff[i], np[i], A, B, shell_atom[a], shell_atom[b], natom,
-                                   Zc, Cxyz, Pb, Wb, (a == b) ? 1.0 : 2.0,
-                                   rows + (long long)i * 3 * natom);
+    double* row = rows + (long long)i * 3 * natom;
+    const int atomA = shell_atom[a], atomB = shell_atom[b];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        row[3 * atomA + d] += gA[d];
+        row[3 * atomB + d] += gB[d];
+    }
 }
 
 int launch_grad_one_electron(int pc, const int* sa, const int* sb, const int* poff, const int* np,
                              int n, const PrimPair* prim, const double* xyz, const ShellMap& map,
                              const int* shell_atom, int natom, const double* Z, const double* Cxyz,
-                             const double* Ps, const double* Ws, double* rows, hipStream_t s)
+                             const double* Ps, const double* Ws, double* rows, double* part,
+                             hipStream_t s)
 {
-    if (n <= 0) return kOk;
-    dim3 g((n + 63) / 64), b(64);
+    if (n <= 0 || natom <= 0) return kOk;
+    const long long nt = (long long)n * natom;
+    if ((nt + 63) / 64 > 0x7fffffffLL) {
+        set_error(kEUnsupported, "one-electron gradient: %lld (pair, nucleus) items", nt);
+        return kEUnsupported;
+    }
+    dim3 gv((unsigned)((nt + 63) / 64)), gs((n + 63) / 64), b(64);
 #define O_(id, la, lb)                                                                            \
     case id:                                                                                      \
-        hipLaunchKernelGGL((k_grad_onee<la, lb>), g, b, 0, s, sa, sb, poff, np, n, prim, xyz, map,  \
-                           shell_atom, natom, Z, Cxyz, Ps, Ws, rows);                             \
+        hipLaunchKernelGGL((k_grad_onee_v<la, lb>), gv, b, 0, s, sa, sb, poff, np, n, prim, xyz,   \
+                           map, natom, Z, Cxyz, Ps, rows, part);                                  \
+        SF_HIP(hipGetLastError());                                                                \
+        hipLaunchKernelGGL((k_grad_onee_st<la, lb>), gs, b, 0, s, sa, sb, poff, np, n, prim, xyz,  \
+                           map, shell_atom, natom, Ps, Ws, rows, (const double*)part);            \
         break;
     switch (pc) {
         O_(0, 0, 0) O_(1, 1, 0) O_(2, 1, 1) O_(3, 2, 0) O_(4, 2, 1) O_(5, 2, 2)

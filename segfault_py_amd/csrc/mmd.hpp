@@ -175,17 +175,15 @@ SF_HD SF_INLINE void one_electron_shell_pair(const PrimPair* __restrict__ prim, 
 // d/dA of a primitive: 2a (a+1| - a_k (a-1|; S and T are translation
 // invariant (d/dB = -d/dA); for the attraction to nucleus C,
 // d/dC = -(d/dA + d/dB) (the Hellmann-Feynman term).
+// Overlap and kinetic part (translation invariant): gA += d/dA, gB += d/dB.
 template <int LA, int LB>
-SF_HD SF_INLINE void one_electron_grad_pair(const PrimPair* __restrict__ prim, int np,
-                                            const double* A, const double* B, int atomA, int atomB,
-                                            int natom, const double* __restrict__ Zc,
-                                            const double* __restrict__ Cxyz,
-                                            const double* __restrict__ Pb, const double* __restrict__ Wb,
-                                            double mult, double* __restrict__ row)
+SF_HD SF_INLINE void one_electron_grad_st(const PrimPair* __restrict__ prim, int np,
+                                          const double* A, const double* B,
+                                          const double* __restrict__ Pb,
+                                          const double* __restrict__ Wb, double mult, double* gA,
+                                          double* gB)
 {
-    constexpr int NA = cart_n(LA), NB = cart_n(LB), L = LA + LB + 1;
-    double gA[3] = {0.0, 0.0, 0.0}, gB[3] = {0.0, 0.0, 0.0};
-    // overlap and kinetic: d/dA only
+    constexpr int NA = cart_n(LA), NB = cart_n(LB);
     for (int k = 0; k < np; ++k) {
         const PrimPair pp = prim[k];
         const double eb = pp.eb, ea2 = 2.0 * (pp.p - eb);
@@ -228,63 +226,104 @@ SF_HD SF_INLINE void one_electron_grad_pair(const PrimPair* __restrict__ prim, i
                 }
             }
     }
-    // nuclear attraction: d/dA, d/dB per nucleus, d/dC by invariance
-    for (int c = 0; c < natom; ++c) {
-        double gC[3] = {0.0, 0.0, 0.0};
-        for (int k = 0; k < np; ++k) {
-            const PrimPair pp = prim[k];
-            const double p = pp.p, eb = pp.eb, ea2 = 2.0 * (p - eb), eb2 = 2.0 * eb;
-            const double P[3] = {pp.Px, pp.Py, pp.Pz};
-            double E[3][LA + 2][LB + 2][LA + LB + 3];
+}
+
+// Attraction to nucleus c (charge Zc[c] at Cxyz[3c..]): gA, gB, gC (set,
+// not accumulated) = d/dA, d/dB, d/dC, with d/dC = -(d/dA + d/dB).
+template <int LA, int LB>
+SF_HD SF_INLINE void one_electron_grad_v(const PrimPair* __restrict__ prim, int np,
+                                         const double* A, const double* B, int c,
+                                         const double* __restrict__ Zc,
+                                         const double* __restrict__ Cxyz,
+                                         const double* __restrict__ Pb, double mult, double* gA,
+                                         double* gB, double* gC)
+{
+    constexpr int NA = cart_n(LA), NB = cart_n(LB), L = LA + LB + 1;
 #pragma unroll
-            for (int d = 0; d < 3; ++d) hermite_E<LA + 1, LB + 1>(0.5 * pp.ip, P[d] - A[d], P[d] - B[d], E[d]);
-            const double PC[3] = {P[0] - Cxyz[3 * c], P[1] - Cxyz[3 * c + 1], P[2] - Cxyz[3 * c + 2]};
-            double F[SF_BOYS_MTAB];
-            boys_grid(L, p * (PC[0] * PC[0] + PC[1] * PC[1] + PC[2] * PC[2]), F);
-            double R[L + 1][L + 1][L + 1];
-            coulomb_R<L>(p, PC[0], PC[1], PC[2], F, R);
-            const double pre_v = -Zc[c] * 2.0 * kPi * pp.ip * pp.K;
+    for (int d = 0; d < 3; ++d) gA[d] = gB[d] = gC[d] = 0.0;
+    for (int k = 0; k < np; ++k) {
+        const PrimPair pp = prim[k];
+        const double p = pp.p, eb = pp.eb, ea2 = 2.0 * (p - eb), eb2 = 2.0 * eb;
+        const double P[3] = {pp.Px, pp.Py, pp.Pz};
+        double E[3][LA + 2][LB + 2][LA + LB + 3];
 #pragma unroll
-            for (int fa = 0; fa < NA; ++fa)
+        for (int d = 0; d < 3; ++d) hermite_E<LA + 1, LB + 1>(0.5 * pp.ip, P[d] - A[d], P[d] - B[d], E[d]);
+        const double PC[3] = {P[0] - Cxyz[3 * c], P[1] - Cxyz[3 * c + 1], P[2] - Cxyz[3 * c + 2]};
+        double F[SF_BOYS_MTAB];
+        boys_grid(L, p * (PC[0] * PC[0] + PC[1] * PC[1] + PC[2] * PC[2]), F);
+        double R[L + 1][L + 1][L + 1];
+        coulomb_R<L>(p, PC[0], PC[1], PC[2], F, R);
+        const double pre_v = -Zc[c] * 2.0 * kPi * pp.ip * pp.K;
 #pragma unroll
-                for (int fb = 0; fb < NB; ++fb) {
-                    const double Pv = mult * pre_v * Pb[fa * NB + fb];
-                    const int a3[3] = {cart_l(LA, fa, 0), cart_l(LA, fa, 1), cart_l(LA, fa, 2)};
-                    const int b3[3] = {cart_l(LB, fb, 0), cart_l(LB, fb, 1), cart_l(LB, fb, 2)};
-                    // V(a', b') for exponent triples a', b'
-                    auto vab = [&](int ax, int ay, int az, int bx, int by, int bz) {
-                        double acc = 0.0;
-                        for (int tt = 0; tt <= ax + bx; ++tt)
-                            for (int uu = 0; uu <= ay + by; ++uu)
-                                for (int vv = 0; vv <= az + bz; ++vv)
-                                    acc += E[0][ax][bx][tt] * E[1][ay][by][uu] * E[2][az][bz][vv] *
-                                           R[tt][uu][vv];
-                        return acc;
-                    };
+        for (int fa = 0; fa < NA; ++fa)
 #pragma unroll
-                    for (int d = 0; d < 3; ++d) {
-                        int ap[3] = {a3[0], a3[1], a3[2]}, bp[3] = {b3[0], b3[1], b3[2]};
-                        ap[d] += 1;
-                        double dA = ea2 * vab(ap[0], ap[1], ap[2], b3[0], b3[1], b3[2]);
-                        if (a3[d] > 0) {
-                            ap[d] -= 2;
-                            dA -= a3[d] * vab(ap[0], ap[1], ap[2], b3[0], b3[1], b3[2]);
-                        }
-                        bp[d] += 1;
-                        double dB = eb2 * vab(a3[0], a3[1], a3[2], bp[0], bp[1], bp[2]);
-                        if (b3[d] > 0) {
-                            bp[d] -= 2;
-                            dB -= b3[d] * vab(a3[0], a3[1], a3[2], bp[0], bp[1], bp[2]);
-                        }
-                        gA[d] += Pv * dA;
-                        gB[d] += Pv * dB;
-                        gC[d] -= Pv * (dA + dB);
+            for (int fb = 0; fb < NB; ++fb) {
+                const double Pv = mult * pre_v * Pb[fa * NB + fb];
+                const int a3[3] = {cart_l(LA, fa, 0), cart_l(LA, fa, 1), cart_l(LA, fa, 2)};
+                const int b3[3] = {cart_l(LB, fb, 0), cart_l(LB, fb, 1), cart_l(LB, fb, 2)};
+                // V(a', b') for exponent triples a', b'
+                auto vab = [&](int ax, int ay, int az, int bx, int by, int bz) {
+                    double acc = 0.0;
+                    for (int tt = 0; tt <= ax + bx; ++tt)
+                        for (int uu = 0; uu <= ay + by; ++uu)
+                            for (int vv = 0; vv <= az + bz; ++vv)
+                                acc += E[0][ax][bx][tt] * E[1][ay][by][uu] * E[2][az][bz][vv] *
+                                       R[tt][uu][vv];
+                    return acc;
+                };
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    int ap[3] = {a3[0], a3[1], a3[2]}, bp[3] = {b3[0], b3[1], b3[2]};
+                    ap[d] += 1;
+                    double dA = ea2 * vab(ap[0], ap[1], ap[2], b3[0], b3[1], b3[2]);
+                    if (a3[d] > 0) {
+                        ap[d] -= 2;
+                        dA -= a3[d] * vab(ap[0], ap[1], ap[2], b3[0], b3[1], b3[2]);
                     }
+                    bp[d] += 1;
+                    double dB = eb2 * vab(a3[0], a3[1], a3[2], bp[0], bp[1], bp[2]);
+                    if (b3[d] > 0) {
+                        bp[d] -= 2;
+                        dB -= b3[d] * vab(a3[0], a3[1], a3[2], bp[0], bp[1], bp[2]);
+                    }
+                    gA[d] += Pv * dA;
+                    gB[d] += Pv * dB;
+                    gC[d] -= Pv * (dA + dB);
                 }
+            }
+    }
+}
+
+// Nuclear gradient of the one-electron energy of one shell pair:
+//   row[3c+k] = d/dR_{c,k} of  mult * sum_{ab} [P_ab (T_ab + V_ab) - W_ab S_ab]
+// for every atom c (the pair's centres and every nucleus), with P, W the
+// density and energy-weighted density blocks in the shell-normalised basis
+// (row-major NA x NB) and mult = 2 for a != b (the transposed block), 1 else.
+// d/dA of a primitive: 2a (a+1| - a_k (a-1|; S and T are translation
+// invariant (d/dB = -d/dA); for the attraction to nucleus C,
+// d/dC = -(d/dA + d/dB) (the Hellmann-Feynman term).  The GPU computes the
+// per-nucleus terms in parallel (k_grad_onee_v) and adds them in nucleus
+// order to the overlap/kinetic part (k_grad_onee_st); this is the same
+// computation on one thread, in the same order.
+template <int LA, int LB>
+SF_HD SF_INLINE void one_electron_grad_pair(const PrimPair* __restrict__ prim, int np,
+                                            const double* A, const double* B, int atomA, int atomB,
+                                            int natom, const double* __restrict__ Zc,
+                                            const double* __restrict__ Cxyz,
+                                            const double* __restrict__ Pb, const double* __restrict__ Wb,
+                                            double mult, double* __restrict__ row)
+{
+    double gA[3] = {0.0, 0.0, 0.0}, gB[3] = {0.0, 0.0, 0.0};
+    one_electron_grad_st<LA, LB>(prim, np, A, B, Pb, Wb, mult, gA, gB);
+    for (int c = 0; c < natom; ++c) {
+        double vA[3], vB[3], vC[3];
+        one_electron_grad_v<LA, LB>(prim, np, A, B, c, Zc, Cxyz, Pb, mult, vA, vB, vC);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            row[3 * c + d] = vC[d];
+            gA[d] += vA[d];
+            gB[d] += vB[d];
         }
-        row[3 * c] = gC[0];
-        row[3 * c + 1] = gC[1];
-        row[3 * c + 2] = gC[2];
     }
 #pragma unroll
     for (int d = 0; d < 3; ++d) {

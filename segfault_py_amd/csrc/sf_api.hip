@@ -201,6 +201,7 @@ struct sf_basis {
     bool grad_ready = false;
     std::vector<long long> gpart_bra, gpart_ket;  // per class offsets into d_gpart
     DevBuf d_gpart, d_gout, d_gcsr_off, d_gcsr_idx, d_grows, d_Ws, d_g2, d_gtot;
+    DevBuf d_gonee;  // one-electron gradient: per (pair, nucleus) d/dA, d/dB partials
     PairClassData pc[kNumPairClass];
     std::vector<int> pair_slot;  // one molecule: pair (a>=b) -> (pc << 24) | position
     bool have_schwarz = false, screened = false;
@@ -1738,6 +1739,7 @@ static int prepare_grad(sf_basis* b)
     long long npair = 0;
     for (auto& P : b->pc) npair += (long long)P.sa.size();
     SF_TRY(b->d_grows.ensure(std::max<long long>(1, npair) * 3 * std::max(1, natom) * sizeof(double)));
+    SF_TRY(b->d_gonee.ensure(std::max<long long>(1, npair) * 6 * std::max(1, natom) * sizeof(double)));
     SF_TRY(b->d_Ws.ensure((size_t)std::max(1LL, b->extent) * sizeof(double)));
     SF_TRY(b->d_g2.ensure(3 * std::max(1, natom) * sizeof(double)));
     SF_TRY(b->d_gtot.ensure(3 * std::max(1, natom) * sizeof(double)));
@@ -1799,7 +1801,8 @@ extern "C" int sf_grad(sf_basis* b, const double* P, const double* W, double* gr
                                         b->map(), b->d_atom.as<int>(), natom,
                                         b->d_Z.as<double>(), b->d_nuc.as<double>(),
                                         b->d_Ps.as<double>(), b->d_Ws.as<double>(),
-                                        b->d_grows.as<double>() + row * 3 * natom, s));
+                                        b->d_grows.as<double>() + row * 3 * natom,
+                                        b->d_gonee.as<double>() + row * 6 * natom, s));
         row += (long long)Pc.sa.size();
     }
     SF_TRY(launch_grad_rows(b->d_grows.as<double>(), row, 3 * natom, b->d_g2.as<double>(),

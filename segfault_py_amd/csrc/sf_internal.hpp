@@ -178,7 +178,8 @@ int launch_grad_atoms(const double* part, const long long* csr_off, const long l
 int launch_grad_one_electron(int pc, const int* sa, const int* sb, const int* poff, const int* np,
                              int n, const PrimPair* prim, const double* xyz, const ShellMap& map,
                              const int* shell_atom, int natom, const double* Z, const double* Cxyz,
-                             const double* Ps, const double* Ws, double* rows, hipStream_t s);
+                             const double* Ps, const double* Ws, double* rows, double* part,
+                             hipStream_t s);
 int launch_grad_rows(const double* rows, long long nrows, int ncol, const double* add, double* out,
                      hipStream_t s);
 void class_tile_offsets(int qcls, int off[6]);
