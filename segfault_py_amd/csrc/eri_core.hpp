@@ -119,6 +119,62 @@ struct QuartetShape {
     }
 };
 
+// One primitive quartet of a one-node class ((ss|ss): LA = 0, (ps|ss):
+// LA = 1).  The Rys node u = F1/F0 only ever appears times the weight
+// w = F0, so (ss|ss) = pref F0 and (p_k s|ss) = pref (PA_k F0 - q/(p+q)
+// PQ_k F1), with no division.
+template <int LA>
+SF_HD SF_INLINE void one_node_term(const PrimPair& pb, const PrimPair& pk, const double* A,
+                                   const NodeTable& nt, double* v)
+{
+    const double p = pb.p, q = pk.p;
+    const double pq = p + q;
+    const double rs = fast_rsqrt(pq);
+    const double ipq = rs * rs;
+    const double rho = p * q * ipq;
+    const double PQ[3] = {pb.Px - pk.Px, pb.Py - pk.Py, pb.Pz - pk.Pz};
+    const double X = rho * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]);
+    const double pref = 34.986836655249725 * pb.Kp * pk.Kp * rs;
+    double F0, F1;
+    boys01_tab(X, nt.p, nt.stride, F0, F1);
+    if constexpr (LA == 0) {
+        v[0] = pref * F0;
+    } else {
+        const double PA[3] = {pb.Px - A[0], pb.Py - A[1], pb.Pz - A[2]};
+        const double qi = q * ipq;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) v[k] = pref * (PA[k] * F0 - qi * PQ[k] * F1);
+    }
+}
+
+// The one-node classes' primitive loops (ket outer, bra inner, as in
+// eri_shell_quartet).  Two bra pairs per iteration, or prefetching the next
+// pair, measured slower: the screened loops often stop early.
+template <int LA, typename R, bool PS, typename BraPtr, typename BqPtr>
+SF_HD SF_INLINE void eri_one_node(BraPtr bra, int nbra, const PrimPair* __restrict__ ket, int nket,
+                                  const double* A, R* __restrict__ acc, const NodeTable& nt,
+                                  BqPtr bq, const double* __restrict__ kq, double tau_prim)
+{
+    constexpr int NV = (LA == 0) ? 1 : 3;
+    for (int ik = 0; ik < nket; ++ik) {
+        double qk = 0.0;
+        if constexpr (PS) {
+            qk = kq[ik];
+            if (nbra == 0 || qk * bq[0] < tau_prim) break;
+        }
+        const PrimPair pk = ket[ik];
+        for (int ib = 0; ib < nbra; ++ib) {
+            if constexpr (PS) {
+                if (bq[ib] * qk < tau_prim) break;
+            }
+            double v[NV];
+            one_node_term<LA>(load_prim(bra, ib), pk, A, nt, v);
+#pragma unroll
+            for (int k = 0; k < NV; ++k) acc[k] += (R)v[k];
+        }
+    }
+}
+
 // Accumulate (ab|cd) for all Cartesian components into acc[NINT] (not
 // zeroed here).  Bra primitive pairs bra[0..nbra), ket pairs ket[0..nket).
 // R is the type of the recurrences and of acc; X, nodes and weights are
@@ -142,6 +198,11 @@ SF_HD SF_INLINE void eri_shell_quartet(BraPtr bra, int nbra,
 {
     using S = QuartetShape<LA, LB, LC, LD>;
     constexpr int NR = S::NR, NAB = S::NAB, NCD = S::NCD;
+    if constexpr (S::L <= 1) {
+        static_assert(LB == 0 && LC == 0 && LD == 0, "the one-node classes are (ss|ss) and (ps|ss)");
+        eri_one_node<LA, R, PS>(bra, nbra, ket, nket, A, acc, nt, bq, kq, tau_prim);
+        return;
+    }
     R ABp[3][LB + 1], CDp[3][LD + 1];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
