@@ -136,7 +136,7 @@ SF_HD SF_INLINE void one_node_term(const PrimPair& pb, const PrimPair& pk, const
     const double X = rho * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]);
     const double pref = 34.986836655249725 * pb.Kp * pk.Kp * rs;
     double F0, F1;
-    boys01_tab(X, nt.p, nt.stride, F0, F1);
+    boys01_node(X, nt, F0, F1);
     if constexpr (LA == 0) {
         v[0] = pref * F0;
     } else {

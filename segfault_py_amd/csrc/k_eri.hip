@@ -94,7 +94,7 @@ __device__ __forceinline__ NodeTable stage_node_table(double* s_tab)
     if constexpr (n > 0) {
         for (int k = threadIdx.x; k < n; k += blockDim.x) s_tab[k] = node_table_elem<NR>(k);
         __syncthreads();
-        return NodeTable{s_tab, NR == 1 ? 10 : 0};
+        return NodeTable{s_tab, NR == 1 ? kBoysTaylorCols : 0, NR == 1 ? 1 : 0};
     } else {
         return global_node_table<NR>();
     }

@@ -146,6 +146,46 @@ SF_HD SF_INLINE void boys01(double X, double& F0, double& F1)
     boys01_tab(X, sf_boys_grid, SF_BOYS_MTAB, F0, F1);
 }
 
+// The one-node kernels' LDS table: rows at X_k = 0.2 k (k = 0..200, the even
+// rows of the global grid) of 12 Taylor coefficients c_m = F_m(X_k) / m!.
+// 19.3 KB instead of the 32 KB of 400 rows of F_0..F_9, so 8 blocks of 256
+// threads fit a CU's LDS instead of 5.
+constexpr int kBoysTaylorRows = 201;
+constexpr int kBoysTaylorCols = 12;
+constexpr double kBoysTaylorDX = 0.2;
+
+// 1/m! for m <= 11.
+SF_HD SF_INLINE double inv_fact(int m)
+{
+    constexpr double t[12] = {1.0,           1.0,            1.0 / 2,         1.0 / 6,
+                              1.0 / 24,      1.0 / 120,      1.0 / 720,       1.0 / 5040,
+                              1.0 / 40320,   1.0 / 362880,   1.0 / 3628800,   1.0 / 39916800};
+    return t[m];
+}
+
+// F_0 and F_1 from the Taylor rows: with d = X_k - X (|d| <= 0.1),
+// F_0(X) = sum_m c_m d^m and F_1 = -dF_0/dX = dF_0/dd = sum_m m c_m d^(m-1),
+// both from one Horner pass (truncation below 1e-19 for both).
+SF_HD SF_INLINE void boys01_taylor(double X, const double* c, double& F0, double& F1)
+{
+    if (X < 39.95) {
+        const int k = (int)(X * (1.0 / kBoysTaylorDX) + 0.5);
+        const double d = k * kBoysTaylorDX - X;
+        const double* ck = c + k * kBoysTaylorCols;
+        double p = ck[kBoysTaylorCols - 1], dp = 0.0;
+#pragma unroll
+        for (int m = kBoysTaylorCols - 2; m >= 0; --m) {
+            dp = dp * d + p;
+            p = p * d + ck[m];
+        }
+        F0 = p;
+        F1 = dp;
+    } else {
+        F0 = 0.5 * kSqrtPi * fast_rsqrt(X);
+        F1 = 0.5 * F0 * fast_rcp(X);
+    }
+}
+
 // ----------------------------------------------------------------- Rys ----
 template <int N>
 SF_HD SF_INLINE const double* rys_table()
@@ -236,7 +276,17 @@ SF_HD SF_INLINE void rys_roots1_closed(double X, double* u, double* w)
 struct NodeTable {
     const double* p;
     int stride;
+    // one node: 0 = rows of F_0..F_9 (the global grid), 1 = the Taylor rows
+    // of boys01_taylor (the kernels' LDS copy)
+    int taylor = 0;
 };
+
+// F_0 and F_1 from a one-node table of either kind.
+SF_HD SF_INLINE void boys01_node(double X, const NodeTable& t, double& F0, double& F1)
+{
+    if (t.taylor) boys01_taylor(X, t.p, F0, F1);
+    else boys01_tab(X, t.p, t.stride, F0, F1);
+}
 
 template <int N>
 SF_HD SF_INLINE NodeTable global_node_table()
@@ -246,18 +296,20 @@ SF_HD SF_INLINE NodeTable global_node_table()
 }
 
 // Doubles a kernel stages in LDS for N nodes (0: read the global table):
-// the 400 grid rows below X = 39.95 (10 columns) for N = 1, the Chebyshev
-// table for N = 2, 3.
+// the Taylor rows of boys01_taylor for N = 1, the Chebyshev table for
+// N = 2, 3.
 constexpr int node_table_lds(int n)
 {
-    return n == 1 ? 400 * 10 : n == 2 ? 48 * 4 * (SF_RYS_DEG + 1) : n == 3 ? 54 * 6 * (SF_RYS_DEG + 1) : 0;
+    return n == 1 ? kBoysTaylorRows * kBoysTaylorCols : n == 2 ? 48 * 4 * (SF_RYS_DEG + 1) : n == 3 ? 54 * 6 * (SF_RYS_DEG + 1) : 0;
 }
 
 // Element k of the LDS copy for N nodes.
 template <int N>
 SF_HD SF_INLINE double node_table_elem(int k)
 {
-    if constexpr (N == 1) return sf_boys_grid[(k / 10) * SF_BOYS_MTAB + k % 10];
+    if constexpr (N == 1)
+        return sf_boys_grid[2 * (k / kBoysTaylorCols) * SF_BOYS_MTAB + k % kBoysTaylorCols] *
+               inv_fact(k % kBoysTaylorCols);
     else return rys_table<N>()[k];
 }
 
@@ -266,7 +318,7 @@ SF_HD SF_INLINE void rys_nodes_tab(double X, const NodeTable& t, double* u, doub
 {
     if constexpr (N == 1) {
         double F0, F1;
-        boys01_tab(X, t.p, t.stride, F0, F1);
+        boys01_node(X, t, F0, F1);
         u[0] = F1 * fast_rcp(F0);
         w[0] = F0;
     } else {

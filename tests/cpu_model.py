@@ -37,6 +37,7 @@ def harness():
         H.h_rys.argtypes = [ctypes.c_int, ctypes.c_double, _f64p, _f64p]
         H.h_boys.argtypes = [ctypes.c_int, ctypes.c_double, _f64p]
         H.h_boys01.argtypes = [ctypes.c_double, _f64p]
+        H.h_boys01_kernel.argtypes = [ctypes.c_double, _f64p]
         H.h_grad.argtypes = [ctypes.c_int] * 4 + [_f64p, ctypes.c_int, _f64p, ctypes.c_int, _f64p,
                                                   _f64p, _f64p, _f64p, _f64p, _f64p]
         H.h_grad_onee.argtypes = [ctypes.c_int, ctypes.c_int, _f64p, ctypes.c_int, _f64p, _f64p,

@@ -7,6 +7,18 @@ using namespace sf;
 
 extern "C" void h_boys(int mmax, double X, double* F) { boys_grid(mmax, X, F); }
 extern "C" void h_boys01(double X, double* F) { boys01(X, F[0], F[1]); }
+// F_0, F_1 as the one-node kernels evaluate them: from the LDS Taylor rows
+// (built here from the same element function the kernels stage).
+extern "C" void h_boys01_kernel(double X, double* F)
+{
+    static double tab[node_table_lds(1)];
+    static bool init = false;
+    if (!init) {
+        for (int k = 0; k < node_table_lds(1); ++k) tab[k] = node_table_elem<1>(k);
+        init = true;
+    }
+    boys01_node(X, NodeTable{tab, kBoysTaylorCols, 1}, F[0], F[1]);
+}
 // The node evaluation the ERI kernels use (one node from F_0, F_1 of the
 // Boys grid, more from the Chebyshev tables), with the global tables.
 extern "C" int h_rys_kernel(int n, double X, double* u, double* w)

@@ -77,6 +77,26 @@ def test_boys01_closed_form_matches_mpmath():
     assert worst < 5e-15, worst
 
 
+def test_boys01_kernel_taylor_rows_match_mpmath():
+    """The one-node kernels' F_0, F_1: LDS rows at X = 0.2 k of Taylor
+    coefficients F_m / m! (m <= 11), F_1 as the derivative of the F_0
+    polynomial; against the 50-digit mpmath values, and a dense sweep over
+    [0, 45] against the 0.1-grid evaluation."""
+    H = CM.harness()
+    worst = 0.0
+    for pt in _golden("boys_mpmath.json")["points"]:
+        F = np.zeros(2)
+        H.h_boys01_kernel(float(pt["T"]), F)
+        ref = np.array([float(v) for v in pt["F"][:2]])
+        worst = max(worst, float(np.max(np.abs(F - ref) / ref)))
+    assert worst < 5e-15, worst
+    F, G = np.zeros(2), np.zeros(2)
+    for x in np.linspace(0.0, 45.0, 9001):
+        H.h_boys01_kernel(float(x), F)
+        H.h_boys01(float(x), G)
+        assert np.max(np.abs(F - G) / G) < 5e-15, x
+
+
 def _check_quartets(oracle, mol, nsample, seed, fp32=False):
     import itertools
     shells = mol.shells()
