@@ -134,9 +134,12 @@ __device__ __forceinline__ void class_quartet(const ClassParams& cp, const NodeT
     R acc[S::NINT];
 #pragma unroll
     for (int k = 0; k < S::NINT; ++k) acc[k] = (R)0;
-    // primitive quartets with Qp_b Qp_k < cp.tau_prim are skipped (the
-    // segment's threshold; 0 keeps every one)
+    // primitive quartets with Qp_b Qp_k < tau are skipped (the segment's
+    // threshold; 0 keeps every one); in a launch over both mixed segments,
+    // the bra's kets past its n64 FP64 ones take the FP32 segment's threshold
     constexpr bool PS = true;
+    double tau = cp.tau_prim;
+    if (cp.n64 && j - cp.ket0[i] >= cp.n64[i]) tau = cp.tau_prim_lo;
     int bpo = cp.bra.poff[i], bnp = cp.bra.np[i];
     if constexpr (UNI) {
         // the bra's primitive pairs are read-only for the whole launch: read
@@ -147,11 +150,11 @@ __device__ __forceinline__ void class_quartet(const ClassParams& cp, const NodeT
         using CDbl = const __attribute__((address_space(4))) double*;
         eri_shell_quartet<LA, LB, LC, LD, R, PS, CPrim, CDbl>(
             (CPrim)(cp.prim + bpo), bnp, cp.prim + cp.ket.poff[j], cp.ket.np[j], A, B, C, D, acc,
-            nt, (CDbl)(cp.primq + bpo), cp.primq + cp.ket.poff[j], cp.tau_prim);
+            nt, (CDbl)(cp.primq + bpo), cp.primq + cp.ket.poff[j], tau);
     } else {
         eri_shell_quartet<LA, LB, LC, LD, R, PS>(cp.prim + bpo, bnp, cp.prim + cp.ket.poff[j],
                                                  cp.ket.np[j], A, B, C, D, acc, nt, cp.primq + bpo,
-                                                 cp.primq + cp.ket.poff[j], cp.tau_prim);
+                                                 cp.primq + cp.ket.poff[j], tau);
     }
     if constexpr (MODE == kModeRaw) {
 #pragma unroll

@@ -1274,6 +1274,8 @@ static ClassParams class_params(sf_basis* b, int c, const double* Ps)
     cp.P = Ps;
     cp.primq = b->d_prim_q.as<double>();
     cp.tau_prim = b->tau_prim64;  // set per segment
+    cp.n64 = nullptr;
+    cp.tau_prim_lo = 0.0;
     cp.out = nullptr;
     cp.raw = Q.d_raw.as<double>();
     cp.pos = Q.d_pos.as<unsigned>();
@@ -1440,6 +1442,19 @@ static int run_jk(sf_basis* b, const double* P, double* J, double* K, int precis
         QuartetClassData& Q = b->qc[c];
         if (Q.nq_all == 0) continue;
         ClassParams cp = class_params(b, c, b->d_Ps.as<double>());
+        if (precision == 1 && b->engine[c] == 0 && !class_split(c) && b->tune[c][2].arith64) {
+            // both segments in FP64 arithmetic: one launch over every quartet
+            // of the class, each bra's FP32-segment kets with that segment's
+            // primitive threshold (the same integrals as two launches)
+            set_segment(b, c, 0, 0, cp, b->d_zero.as<int>());
+            cp.tau_prim = b->tau_prim64;
+            cp.n64 = Q.d_k032.as<int>();
+            cp.tau_prim_lo = b->tau_prim;
+            SF_TRY(launch_class_engine(b, c, 0, kModeJK, cp, b->tune[c][1], s));
+            SF_TRY(mark());
+            slot.push_back(2 * c);
+            continue;
+        }
         for (int seg = 0; seg <= precision; ++seg) {
             set_segment(b, c, precision, seg, cp, b->d_zero.as<int>());
             SF_TRY(launch_class_engine(b, c, seg, kModeJK, cp, b->tune[c][tune_slot(precision, seg)], s));
@@ -1569,61 +1584,105 @@ extern "C" int sf_autotune(sf_basis* b, const double* P, int precision, void* st
     SF_HIP(hipEventCreate(&e0));
     SF_HIP(hipEventCreate(&e1));
     int rc = kOk;
+    const int blocks[3] = {64, 128, 256};
+    const int mults[3] = {0, 2, 8};
+    // fastest launch shape for launches of (segment seg of) class c with
+    // parameters tcp in arithmetic ar; candidates are timed on a prefix of
+    // the quartets: the ranking of launch shapes does not need every quartet,
+    // and this bounds the tuning time for the largest classes
+    auto time_shapes = [&](int c, int seg, ClassParams tcp, int ar, Tune& best, float& best_ms) -> int {
+        tcp.nq = std::min(tcp.nq, kTuneQuartets);
+        // work items of one launch: quartets, or waves x component slices
+        const long long items =
+            class_split(c) ? ((tcp.nq + 63) / 64) * 64 * class_nslice(c) : tcp.nq;
+        best_ms = 1e30f;
+        Tune warm;
+        warm.arith64 = ar;
+        SF_TRY(launch_class_engine(b, c, seg, kModeJK, tcp, warm, s));  // untimed warm-up
+        for (int bi = 0; bi < 3; ++bi)
+            for (int mi = 0; mi < 3; ++mi) {
+                Tune t;
+                t.block = blocks[bi];
+                t.grid = mults[mi] * ncu;
+                t.arith64 = ar;
+                const long long need = (items + t.block - 1) / t.block;
+                if (t.grid > 0 && t.grid >= need) continue;  // same as grid 0
+                if (hipEventRecord(e0, s) != hipSuccess) {
+                    set_error(kEHip, "sf_autotune: event record failed");
+                    return kEHip;
+                }
+                SF_TRY(launch_class_engine(b, c, seg, kModeJK, tcp, t, s));
+                float el = 0.f;
+                if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                    hipEventElapsedTime(&el, e0, e1) != hipSuccess) {
+                    set_error(kEHip, "sf_autotune: event timing failed");
+                    return kEHip;
+                }
+                if (el < best_ms) {
+                    best_ms = el;
+                    best = t;
+                }
+            }
+        return kOk;
+    };
+    // time of a whole launch over n quartets from its prefix timing
+    auto full = [](float t, long long n) {
+        return (double)t * (double)n / (double)std::max(1LL, std::min(n, kTuneQuartets));
+    };
     for (int c = 0; c < kNumQuartetClass && rc == kOk; ++c) {
         QuartetClassData& Q = b->qc[c];
         ClassParams cp = class_params(b, c, b->d_Ps.as<double>());
+        float seg_ms[2] = {0.f, 0.f};
+        float ms32 = 0.f;  // FP32 segment in FP32 arithmetic
+        Tune best32;
         for (int seg = 0; seg <= precision && rc == kOk; ++seg) {
             set_segment(b, c, precision, seg, cp, b->d_zero.as<int>());
-            if (ms) ms[3 * c + tune_slot(precision, seg)] = 0.0;
-            if (Q.nq_all == 0 || cp.nq == 0) continue;
-            // candidates are timed on a prefix of the segment: the ranking of
-            // launch shapes does not need every quartet, and this bounds the
-            // tuning time for the largest classes
-            cp.nq = std::min(cp.nq, kTuneQuartets);
-            // work items of one launch: quartets, or waves x component slices
-            const long long items = class_split(c) ? ((cp.nq + 63) / 64) * 64 * class_nslice(c) : cp.nq;
             const int slot = tune_slot(precision, seg);
+            if (ms) ms[3 * c + slot] = 0.0;
+            if (Q.nq_all == 0 || cp.nq == 0) continue;
             Tune best = b->tune[c][slot];
             float best_ms = 1e30f;
-            const int blocks[3] = {64, 128, 256};
-            const int mults[3] = {0, 2, 8};
+            rc = time_shapes(c, seg, cp, 0, best, best_ms);
+            ms32 = best_ms;
+            best32 = best;
             // the FP32 segment also tries FP64 arithmetic (never less accurate)
-            const int narith = (seg == 1 && b->engine[c] == 0) ? 2 : 1;
-            for (int ar = 0; ar < narith && rc == kOk; ++ar) {
-                Tune warm;
-                warm.arith64 = ar;
-                rc = launch_class_engine(b, c, seg, kModeJK, cp, warm, s);  // untimed warm-up
-                for (int bi = 0; bi < 3 && rc == kOk; ++bi)
-                    for (int mi = 0; mi < 3 && rc == kOk; ++mi) {
-                        Tune t;
-                        t.block = blocks[bi];
-                        t.grid = mults[mi] * ncu;
-                        t.arith64 = ar;
-                        const long long need = (items + t.block - 1) / t.block;
-                        if (t.grid > 0 && t.grid >= need) continue;  // same as grid 0
-                        if (hipEventRecord(e0, s) != hipSuccess) {
-                            set_error(kEHip, "sf_autotune: event record failed");
-                            rc = kEHip;
-                            break;
-                        }
-                        rc = launch_class_engine(b, c, seg, kModeJK, cp, t, s);
-                        if (rc) break;
-                        float el = 0.f;
-                        if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-                            hipEventElapsedTime(&el, e0, e1) != hipSuccess) {
-                            set_error(kEHip, "sf_autotune: event timing failed");
-                            rc = kEHip;
-                            break;
-                        }
-                        if (el < best_ms) {
-                            best_ms = el;
-                            best = t;
-                        }
-                    }
+            if (!rc && seg == 1 && b->engine[c] == 0) {
+                Tune b64;
+                float m64 = 1e30f;
+                rc = time_shapes(c, seg, cp, 1, b64, m64);
+                if (!rc && m64 < best_ms) {
+                    best_ms = m64;
+                    best = b64;
+                }
             }
             if (rc) break;
             b->tune[c][slot] = best;
+            seg_ms[seg] = best_ms;
             if (ms) ms[3 * c + slot] = best_ms;
+        }
+        // classes with at most 81 integrals per quartet: FP64 arithmetic in
+        // the FP32 segment means one launch over both segments (run_jk), so
+        // the choice is between that launch and two with FP32 arithmetic
+        if (!rc && precision == 1 && b->engine[c] == 0 && !class_split(c) && Q.nq64 > 0 &&
+            Q.nq32 > 0) {
+            ClassParams mcp = class_params(b, c, b->d_Ps.as<double>());
+            set_segment(b, c, 0, 0, mcp, b->d_zero.as<int>());
+            mcp.tau_prim = b->tau_prim64;
+            mcp.n64 = Q.d_k032.as<int>();
+            mcp.tau_prim_lo = b->tau_prim;
+            Tune bm;
+            float mm = 1e30f;
+            rc = time_shapes(c, 0, mcp, 0, bm, mm);
+            if (rc) break;
+            const double two = full(seg_ms[0], Q.nq64) + full(ms32, Q.nq32);
+            if (full(mm, Q.nq_all) < two) {
+                b->tune[c][1] = bm;
+                b->tune[c][2].arith64 = 1;
+                if (ms) ms[3 * c + 1] = mm, ms[3 * c + 2] = 0.0;
+            } else {
+                b->tune[c][2] = best32;
+                if (ms) ms[3 * c + 2] = ms32;
+            }
         }
     }
     (void)hipEventDestroy(e0);

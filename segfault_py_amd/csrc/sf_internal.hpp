@@ -67,7 +67,9 @@ struct Tune {
     int block = 256;
     int grid = 0;
     // FP32 segment of a mixed build only: 1 = run its quartets with FP64
-    // arithmetic (same primitive threshold), where that is the faster one
+    // arithmetic (same primitive threshold), where that is the faster one;
+    // classes with at most 81 integrals per quartet then run both segments
+    // as one launch (ClassParams::n64)
     int arith64 = 0;
 };
 
@@ -87,6 +89,8 @@ struct ClassParams {
     const double* P;           // density in the shell-normalised basis
     const double* primq;       // Schwarz factor of each primitive pair (same index as prim)
     double tau_prim;           // skip primitive quartets with Qp_b Qp_k < tau_prim (per segment)
+    const int* n64;            // one launch for both mixed segments: kets at index >= n64[i]
+    double tau_prim_lo;        //   of bra i use tau_prim_lo (nullptr: tau_prim for all)
     double* out;               // kModeRaw: SoA planes out[k * qstride + q]
     double* raw;               // split classes: raw integral planes (NINT * qstride)
     const unsigned* pos;       // kModeJK: pos[t * qstride + q] = slot of tile t in `tiles`

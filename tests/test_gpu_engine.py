@@ -211,6 +211,39 @@ def test_fp64_primitive_screening(gpu, oracle):
     assert np.max(np.abs(bw.eri_dense() - ref)) < 1e-11
 
 
+def test_mixed_one_launch_equals_fp64_with_equal_thresholds(gpu):
+    """Mixed build with every FP32 segment in FP64 arithmetic (classes with
+    at most 81 integrals per quartet then run both segments as one launch,
+    with a threshold per quartet) and both primitive thresholds equal: the
+    same integrals as the FP64 build, so J and K are equal bit for bit,
+    eagerly and from a replayed graph."""
+    from segfault_py_amd import engine, workloads as W
+    mol = W.config_molecule(3)
+    b = engine.Basis(mol).screen(1e-12, 1e-5)
+    _, P = _density(mol.nbf, 8, "cuda")
+    b.set_prim_tau_fp64(1e-15)
+    b.set_prim_tau(1e-15)
+    sh = np.tile(np.array([256, 0, 0], dtype=np.int32), 63).reshape(21, 3, 3)
+    b.set_tuning(sh)
+    J64, K64 = b.jk(P, 0)
+    Jf, Kf = b.jk(P, 1)                    # FP32 arithmetic: close, not equal
+    assert float((Jf - J64).abs().max()) < 1e-9
+    sh[:, 2, 2] = 1
+    b.set_tuning(sh)
+    Jm, Km = b.jk(P, 1)
+    assert gpu.equal(Jm, J64) and gpu.equal(Km, K64)
+    Jg, Kg = b._zeros(), b._zeros()
+    for _ in range(2):
+        b.jk_graph(P, (Jg, Kg), 1)
+        gpu.cuda.synchronize()
+        assert gpu.equal(Jg, J64) and gpu.equal(Kg, K64)
+    # thresholds differ again: the one launch still uses each segment's own
+    b.set_prim_tau(-1.0)
+    Jd, Kd = b.jk(P, 1)
+    assert not gpu.equal(Jd, J64)
+    assert float((Jd - J64).abs().max()) < 1e-9
+
+
 def test_prim_quartet_counts(gpu):
     """The evaluated-primitive-quartet counter: with primitive-quartet
     screening off it equals the static count of the quartet lists (per
