@@ -337,21 +337,19 @@ __global__ __launch_bounds__(256) void k_digest_split(const ClassParams cp,
     static_switch<0, 6>(t, [&](auto tc) {
         constexpr int T = decltype(tc)::value;
         using G = TileGeom<LA, LB, LC, LD, T>;
-        double Pm[G::M1][G::M2];
-        load_tile_density<LA, LB, LC, LD, T>(cp.P, qm, Pm);
         double* dst = cp.tiles + cp.pos[T * cp.qstride + qa];
-        double row[G::N2];
         if (rows) {
+            double Pm[G::M1][G::M2];
+            load_tile_density<LA, LB, LC, LD, T>(cp.P, qm, Pm);
+            double row[G::N2];
             digest_tile_row<LA, LB, LC, LD, T>(raw + qa, cp.qstride, fac, Pm, r, row);
 #pragma unroll
             for (int e = 0; e < G::N2; ++e) dst[r * G::N2 + e] = row[e];
         } else {
+            double tile[G::N1 * G::N2];
+            digest_tile<LA, LB, LC, LD, T>(raw + qa, cp.qstride, fac, cp.P, qm, tile);
 #pragma unroll
-            for (int rr = 0; rr < G::N1; ++rr) {
-                digest_tile_row<LA, LB, LC, LD, T>(raw + qa, cp.qstride, fac, Pm, rr, row);
-#pragma unroll
-                for (int e = 0; e < G::N2; ++e) dst[rr * G::N2 + e] = row[e];
-            }
+            for (int e = 0; e < G::N1 * G::N2; ++e) dst[e] = tile[e];
         }
     });
     }

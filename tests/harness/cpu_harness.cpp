@@ -137,6 +137,32 @@ static void run_tiles(const double* v, double fac, const double* P, int nbf, con
     digest_tile<LA, LB, LC, LD, 5>(v, 1, fac, P, qm, t + S::T_BD);
 }
 
+// The same six tiles built row by row (digest_tile_row, the split-class
+// digestion's row mode).
+template <int LA, int LB, int LC, int LD, int T>
+static void tile_by_rows(const double* v, double fac, const double* P, const QuartetMap& qm,
+                         double* t)
+{
+    using G = TileGeom<LA, LB, LC, LD, T>;
+    double Pm[G::M1][G::M2];
+    load_tile_density<LA, LB, LC, LD, T>(P, qm, Pm);
+    for (int r = 0; r < G::N1; ++r) digest_tile_row<LA, LB, LC, LD, T>(v, 1, fac, Pm, r, t + r * G::N2);
+}
+
+template <int LA, int LB, int LC, int LD>
+static void run_tile_rows(const double* v, double fac, const double* P, int nbf, const int* f0,
+                          double* t)
+{
+    using S = QuartetShape<LA, LB, LC, LD>;
+    const QuartetMap qm = dense_map(nbf, f0);
+    tile_by_rows<LA, LB, LC, LD, 0>(v, fac, P, qm, t + S::T_AB);
+    tile_by_rows<LA, LB, LC, LD, 1>(v, fac, P, qm, t + S::T_CD);
+    tile_by_rows<LA, LB, LC, LD, 2>(v, fac, P, qm, t + S::T_AC);
+    tile_by_rows<LA, LB, LC, LD, 3>(v, fac, P, qm, t + S::T_AD);
+    tile_by_rows<LA, LB, LC, LD, 4>(v, fac, P, qm, t + S::T_BC);
+    tile_by_rows<LA, LB, LC, LD, 5>(v, fac, P, qm, t + S::T_BD);
+}
+
 template <int LA, int LB, int LC, int LD>
 static void run_digest(const double* v, const double* P, int nbf, const int* f0, double* t)
 {
@@ -209,6 +235,19 @@ extern "C" int h_digest_tiles(int la, int lb, int lc, int ld, const double* v, d
 #define X(a, b, c, d)                                      \
     if (la == a && lb == b && lc == c && ld == d) {        \
         run_tiles<a, b, c, d>(v, fac, P, nbf, f0, t);      \
+        return 0;                                          \
+    }
+    SF_CLASSES(X)
+#undef X
+    return 1;
+}
+
+extern "C" int h_digest_tile_rows(int la, int lb, int lc, int ld, const double* v, double fac,
+                                  const double* P, int nbf, const int* f0, double* t)
+{
+#define X(a, b, c, d)                                      \
+    if (la == a && lb == b && lc == c && ld == d) {        \
+        run_tile_rows<a, b, c, d>(v, fac, P, nbf, f0, t);  \
         return 0;                                          \
     }
     SF_CLASSES(X)

@@ -226,6 +226,10 @@ def test_digest_tiles_equal_digest_quartet():
         assert H.h_digest(*ls, np.ascontiguousarray(v * 0.25), P, nbf, f0, t1) == 0
         assert H.h_digest_tiles(*ls, v, 0.25, P, nbf, f0, t2) == 0
         assert np.max(np.abs(t1 - t2)) < 1e-12
+        # the split-class digestion's row mode: same sums, same order
+        t3 = np.zeros(nt)
+        assert H.h_digest_tile_rows(*ls, v, 0.25, P, nbf, f0, t3) == 0
+        assert np.array_equal(t2, t3)
 
 
 @pytest.mark.parametrize("fixture", ["water_sp", "water_ccpvdz"])
