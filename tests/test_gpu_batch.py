@@ -64,6 +64,29 @@ def test_batch_jk_bitwise_equal_single(batch, precision):
         assert float(J[k, n:, :].abs().sum()) == 0.0 and float(K[k, :, n:].abs().sum()) == 0.0
 
 
+def test_batch_mixed_one_launch_bitwise_equal_single(batch):
+    """Mixed builds with every FP32 segment in FP64 arithmetic (one launch
+    over both segments for the classes with at most 81 integrals per
+    quartet, thresholds chosen per quartet from each bra's n64 within its
+    molecule): the batch equals each molecule's own handle bit for bit."""
+    mols, bb, singles, P, Ps = batch
+    sh = np.tile(np.array([256, 0, 0], dtype=np.int32), 63).reshape(21, 3, 3)
+    sh[:, 2, 2] = 1
+    default = np.tile(np.array([256, 0, 0], dtype=np.int32), 63)
+    bb.set_tuning(sh)
+    J, K = bb.jk(P, 1)
+    bb.set_tuning(default)
+    Jd, Kd = bb.jk(P, 1)
+    assert float((J - Jd).abs().max()) < 1e-9 and float((K - Kd).abs().max()) < 1e-9
+    for k, b in enumerate(singles):
+        n = b.nbf
+        b.set_tuning(sh)
+        Js, Ks = b.jk(Ps[k], 1)
+        b.set_tuning(default)
+        assert bool((J[k, :n, :n] == Js).all()), k
+        assert bool((K[k, :n, :n] == Ks).all()), k
+
+
 def test_batch_one_electron_bitwise_equal_single(batch):
     mols, bb, singles, P, Ps = batch
     S, T, V = bb.one_electron()
