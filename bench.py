@@ -144,7 +144,8 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only "
-                                   "(see DESIGN.md), --gpus must be 1"}))
+                                   "(partition of each class's quartet ranges across GPUs, then an all-reduce of "
+                                   "X and Y), not run; --gpus must be 1"}))
         sys.exit(2)
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
