@@ -708,7 +708,8 @@ SF_HD SF_INLINE void grad_bra_component(const PrimPair* __restrict__ bra, int nb
                                         const PrimPair* __restrict__ ket, int nket,
                                         const double* A, const double* B, const double* C,
                                         const double* D, const double* __restrict__ Dfac,
-                                        double* __restrict__ g)
+                                        double* __restrict__ g,
+                                        const NodeTable& nt = global_node_table<(QuartetShape<LA, LB, LC, LD>::L + 1) / 2 + 1>())
 {
     using S = QuartetShape<LA, LB, LC, LD>;
     constexpr int NR = (S::L + 1) / 2 + 1;
@@ -742,7 +743,7 @@ SF_HD SF_INLINE void grad_bra_component(const PrimPair* __restrict__ bra, int nb
             const double X = rho * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]);
             const double pref = 34.986836655249725 * pb.Kp * pk.Kp * rs;
             double u[NR], w[NR];
-            rys_nodes_tab<NR>(X, global_node_table<NR>(), u, w);
+            rys_nodes_tab<NR>(X, nt, u, w);
 #pragma unroll
             for (int r = 0; r < NR; ++r) {
                 const double t2 = u[r];

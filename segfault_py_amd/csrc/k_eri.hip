@@ -365,6 +365,10 @@ __global__ __launch_bounds__(256) void k_grad_split(const ClassParams cp, double
 {
     using S = QuartetShape<LA, LB, LC, LD>;
     constexpr int NS = S::NA * S::NB, NK = S::NC * S::ND;
+    // derivative integrals need one node more than the integrals (L + 1)
+    constexpr int NRG = (S::L + 1) / 2 + 1;
+    __shared__ double s_tab[node_table_lds(NRG) > 0 ? node_table_lds(NRG) : 1];
+    const NodeTable nt = stage_node_table<NRG>(s_tab);  // every thread, before any exit
     const long long gw = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int s = __builtin_amdgcn_readfirstlane((int)(gw % NS));
     const long long q = (gw / NS) * 64 + (threadIdx.x & 63);
@@ -402,7 +406,7 @@ __global__ __launch_bounds__(256) void k_grad_split(const ClassParams cp, double
                            Pa[qm.col[3] + fd] * Pb[qm.col[2] + fc]);
         grad_bra_component<LA, LB, LC, LD, FA, FB, R>(cp.prim + cp.bra.poff[i], cp.bra.np[i],
                                                       cp.prim + cp.ket.poff[j], cp.ket.np[j], A,
-                                                      B, C, D, Df, g);
+                                                      B, C, D, Df, g, nt);
     });
 #pragma unroll
     for (int k = 0; k < 9; ++k) gout[(long long)(s * 9 + k) * cp.qstride + qa] = g[k];
