@@ -53,6 +53,8 @@ def parse():
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the 256-molecule batched SCF measurement")
     ap.add_argument("--batch-size", type=int, default=256)
+    ap.add_argument("--gradient-only", action="store_true",
+                    help="only the (H2O)10 SCF + three FP64 nuclear gradients (for kernel tracing)")
     ap.add_argument("--builds-only", action="store_true",
                     help="autotune, then only W + K FP64 J/K builds (for kernel tracing)")
     ap.add_argument("--only-batch", action="store_true",
@@ -152,6 +154,9 @@ def main():
         return
     if args.only_batch:
         print(json.dumps({"batch": batch_workload(args.batch_size)}))
+        return
+    if args.gradient_only:
+        print(json.dumps({"gradient_calls": gradient_calls(3)}))
         return
     import torch
     from segfault_py_amd import engine, metrics, workloads as W
@@ -355,6 +360,25 @@ def gradient_workload() -> dict:
     out["max_dgrad_mixed_vs_fp64"] = float(np.max(np.abs(g["mixed"] - g["fp64"])))
     out["max_abs_gradient"] = float(np.max(np.abs(g["fp64"])))
     return out
+
+
+def gradient_calls(ncall: int) -> dict:
+    """(H2O)10 FP64 SCF, then ncall FP64 nuclear gradients (the first one
+    includes its one-time setup); wall seconds of each call."""
+    import torch
+    from segfault_py_amd import engine, grad as G, hf, workloads as W
+
+    mol = W.config_molecule(3)
+    b = engine.Basis(mol).screen(1e-12, 1e-5)
+    r = hf.scf_diis(mol, basis=b, precision=0, tol=1e-10)
+    times = []
+    for _ in range(ncall):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        G.rhf_gradient(mol, r, b, 0)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t)
+    return {"workload": f"(H2O)10, 30 atoms, N={mol.nbf}, s/p + d(O)", "gradient_s": times}
 
 
 def batch_workload(nmol: int) -> dict:
