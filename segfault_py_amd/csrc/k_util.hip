@@ -344,20 +344,32 @@ int launch_reduce(int n1, int n2, const DestList& dl, const double* tiles, const
 // ------------------------------------------- gradient reductions ---------
 // Bra side: one wavefront per bra pair i of a class sums d/dA and d/dB over
 // its kets (the contiguous quartets [base[i], base[i+1])) and all bra
-// component slices, in a fixed order.
+// component slices, in a fixed order.  It also leaves each quartet's ket
+// derivatives summed over the slices, kc[6q..6q+5] = (d/dC, d/dD = -(A+B+C)),
+// so the ket side reads 6 contiguous doubles per quartet.
 __global__ __launch_bounds__(256) void k_grad_reduce_bra(const double* __restrict__ gout, int ns,
                                                          long long qstride,
                                                          const long long* __restrict__ base,
-                                                         int nbra, double* __restrict__ part)
+                                                         int nbra, double* __restrict__ part,
+                                                         double* __restrict__ kc)
 {
     const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (i >= nbra) return;
     const int lane = threadIdx.x & 63;
     double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    for (long long q = base[i] + lane; q < base[i + 1]; q += 64)
+    for (long long q = base[i] + lane; q < base[i + 1]; q += 64) {
+        double gs[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         for (int s = 0; s < ns; ++s)
 #pragma unroll
-            for (int k = 0; k < 6; ++k) acc[k] += gout[(long long)(s * 9 + k) * qstride + q];
+            for (int k = 0; k < 9; ++k) gs[k] += gout[(long long)(s * 9 + k) * qstride + q];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) acc[k] += gs[k];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            kc[6 * q + k] = gs[6 + k];
+            kc[6 * q + 3 + k] = -(gs[k] + gs[3 + k] + gs[6 + k]);
+        }
+    }
 #pragma unroll
     for (int k = 0; k < 6; ++k)
 #pragma unroll
@@ -371,8 +383,7 @@ __global__ __launch_bounds__(256) void k_grad_reduce_bra(const double* __restric
 // i in [i0, i1) with n_i = base[i+1] - base[i] > j (i0 = j when bra and ket
 // lists coincide, else 0); n_i > j holds on a prefix of that range because
 // the lists are sorted by Schwarz factor.  Sums d/dC and d/dD = -(A+B+C).
-__global__ __launch_bounds__(256) void k_grad_reduce_ket(const double* __restrict__ gout, int ns,
-                                                         long long qstride,
+__global__ __launch_bounds__(256) void k_grad_reduce_ket(const double* __restrict__ kc,
                                                          const long long* __restrict__ base,
                                                          int nbra, int nket, int same,
                                                          double* __restrict__ part)
@@ -389,17 +400,9 @@ __global__ __launch_bounds__(256) void k_grad_reduce_ket(const double* __restric
     }
     double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     for (int i = i0 + lane; i < lo; i += 64) {
-        const long long q = base[i] + j;
-        for (int s = 0; s < ns; ++s) {
-            double g[9];
+        const double* v = kc + 6 * (base[i] + j);
 #pragma unroll
-            for (int k = 0; k < 9; ++k) g[k] = gout[(long long)(s * 9 + k) * qstride + q];
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                acc[k] += g[6 + k];
-                acc[3 + k] -= g[k] + g[3 + k] + g[6 + k];
-            }
-        }
+        for (int k = 0; k < 6; ++k) acc[k] += v[k];
     }
 #pragma unroll
     for (int k = 0; k < 6; ++k)
@@ -436,21 +439,21 @@ __global__ __launch_bounds__(64) void k_grad_atoms(const double* __restrict__ pa
 }
 
 int launch_grad_reduce_bra(const double* gout, int nslice, long long qstride, const long long* base,
-                           int nbra, double* part, hipStream_t s)
+                           int nbra, double* part, double* kc, hipStream_t s)
 {
     if (nbra <= 0) return kOk;
     hipLaunchKernelGGL(k_grad_reduce_bra, dim3((nbra + 3) / 4), dim3(256), 0, s, gout, nslice,
-                       qstride, base, nbra, part);
+                       qstride, base, nbra, part, kc);
     SF_HIP(hipGetLastError());
     return kOk;
 }
 
-int launch_grad_reduce_ket(const double* gout, int nslice, long long qstride, const long long* base,
-                           int nbra, int nket, int same, double* part, hipStream_t s)
+int launch_grad_reduce_ket(const double* kc, const long long* base, int nbra, int nket, int same,
+                           double* part, hipStream_t s)
 {
     if (nket <= 0) return kOk;
-    hipLaunchKernelGGL(k_grad_reduce_ket, dim3((nket + 3) / 4), dim3(256), 0, s, gout, nslice,
-                       qstride, base, nbra, nket, same, part);
+    hipLaunchKernelGGL(k_grad_reduce_ket, dim3((nket + 3) / 4), dim3(256), 0, s, kc, base, nbra,
+                       nket, same, part);
     SF_HIP(hipGetLastError());
     return kOk;
 }
