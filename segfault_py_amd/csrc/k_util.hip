@@ -342,34 +342,41 @@ int launch_reduce(int n1, int n2, const DestList& dl, const double* tiles, const
 }
 
 // ------------------------------------------- gradient reductions ---------
+// Per quartet q of a class (one thread each, flat over the class): the
+// derivatives summed over the bra component slices, as two records of six
+// doubles, ab[6q..] = (d/dA, d/dB) and kc[6q..] = (d/dC, d/dD = -(A+B+C)).
+__global__ __launch_bounds__(256) void k_grad_slices(const double* __restrict__ gout, int ns,
+                                                     long long qstride, double* __restrict__ ab,
+                                                     double* __restrict__ kc)
+{
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= qstride) return;
+    double gs[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int s = 0; s < ns; ++s)
+#pragma unroll
+        for (int k = 0; k < 9; ++k) gs[k] += gout[(long long)(s * 9 + k) * qstride + q];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) ab[6 * q + k] = gs[k];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        kc[6 * q + k] = gs[6 + k];
+        kc[6 * q + 3 + k] = -(gs[k] + gs[3 + k] + gs[6 + k]);
+    }
+}
+
 // Bra side: one wavefront per bra pair i of a class sums d/dA and d/dB over
-// its kets (the contiguous quartets [base[i], base[i+1])) and all bra
-// component slices, in a fixed order.  It also leaves each quartet's ket
-// derivatives summed over the slices, kc[6q..6q+5] = (d/dC, d/dD = -(A+B+C)),
-// so the ket side reads 6 contiguous doubles per quartet.
-__global__ __launch_bounds__(256) void k_grad_reduce_bra(const double* __restrict__ gout, int ns,
-                                                         long long qstride,
+// its kets (the contiguous quartets [base[i], base[i+1])), in a fixed order.
+__global__ __launch_bounds__(256) void k_grad_reduce_bra(const double* __restrict__ ab,
                                                          const long long* __restrict__ base,
-                                                         int nbra, double* __restrict__ part,
-                                                         double* __restrict__ kc)
+                                                         int nbra, double* __restrict__ part)
 {
     const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (i >= nbra) return;
     const int lane = threadIdx.x & 63;
     double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    for (long long q = base[i] + lane; q < base[i + 1]; q += 64) {
-        double gs[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-        for (int s = 0; s < ns; ++s)
+    for (long long q = base[i] + lane; q < base[i + 1]; q += 64)
 #pragma unroll
-            for (int k = 0; k < 9; ++k) gs[k] += gout[(long long)(s * 9 + k) * qstride + q];
-#pragma unroll
-        for (int k = 0; k < 6; ++k) acc[k] += gs[k];
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            kc[6 * q + k] = gs[6 + k];
-            kc[6 * q + 3 + k] = -(gs[k] + gs[3 + k] + gs[6 + k]);
-        }
-    }
+        for (int k = 0; k < 6; ++k) acc[k] += ab[6 * q + k];
 #pragma unroll
     for (int k = 0; k < 6; ++k)
 #pragma unroll
@@ -439,11 +446,18 @@ __global__ __launch_bounds__(64) void k_grad_atoms(const double* __restrict__ pa
 }
 
 int launch_grad_reduce_bra(const double* gout, int nslice, long long qstride, const long long* base,
-                           int nbra, double* part, double* kc, hipStream_t s)
+                           int nbra, double* part, double* ab, double* kc, hipStream_t s)
 {
-    if (nbra <= 0) return kOk;
-    hipLaunchKernelGGL(k_grad_reduce_bra, dim3((nbra + 3) / 4), dim3(256), 0, s, gout, nslice,
-                       qstride, base, nbra, part, kc);
+    if (nbra <= 0 || qstride <= 0) return kOk;
+    if ((qstride + 255) / 256 > 0x7fffffffLL) {
+        set_error(kEUnsupported, "gradient reduction: %lld quartets", qstride);
+        return kEUnsupported;
+    }
+    hipLaunchKernelGGL(k_grad_slices, dim3((unsigned)((qstride + 255) / 256)), dim3(256), 0, s, gout,
+                       nslice, qstride, ab, kc);
+    SF_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_grad_reduce_bra, dim3((nbra + 3) / 4), dim3(256), 0, s,
+                       (const double*)ab, base, nbra, part);
     SF_HIP(hipGetLastError());
     return kOk;
 }

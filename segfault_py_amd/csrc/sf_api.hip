@@ -202,7 +202,8 @@ struct sf_basis {
     std::vector<long long> gpart_bra, gpart_ket;  // per class offsets into d_gpart
     DevBuf d_gpart, d_gout, d_gcsr_off, d_gcsr_idx, d_grows, d_Ws, d_g2, d_gtot;
     DevBuf d_gonee;  // one-electron gradient: per (pair, nucleus) d/dA, d/dB partials
-    DevBuf d_gkc;    // per quartet of one class: ket derivatives summed over slices (6)
+    DevBuf d_gkc;    // per quartet of one class: ket, then bra derivatives summed over slices
+    long long gkc_q = 1;  // quartets d_gkc holds per half
     PairClassData pc[kNumPairClass];
     std::vector<int> pair_slot;  // one molecule: pair (a>=b) -> (pc << 24) | position
     bool have_schwarz = false, screened = false;
@@ -1802,7 +1803,8 @@ static int prepare_grad(sf_basis* b)
     SF_TRY(b->d_gonee.ensure(std::max<long long>(1, npair) * 6 * std::max(1, natom) * sizeof(double)));
     long long maxq = 1;
     for (int c = 0; c < kNumQuartetClass; ++c) maxq = std::max(maxq, b->qc[c].nq_all);
-    SF_TRY(b->d_gkc.ensure((size_t)maxq * 6 * sizeof(double)));
+    SF_TRY(b->d_gkc.ensure((size_t)maxq * 12 * sizeof(double)));
+    b->gkc_q = maxq;
     SF_TRY(b->d_Ws.ensure((size_t)std::max(1LL, b->extent) * sizeof(double)));
     SF_TRY(b->d_g2.ensure(3 * std::max(1, natom) * sizeof(double)));
     SF_TRY(b->d_gtot.ensure(3 * std::max(1, natom) * sizeof(double)));
@@ -1848,6 +1850,7 @@ extern "C" int sf_grad(sf_basis* b, const double* P, const double* W, double* gr
         SF_TRY(launch_grad_reduce_bra(b->d_gout.as<double>(), class_nslice(c), Q.nq_all,
                                       Q.d_base_all.as<long long>(), Q.nbra,
                                       b->d_gpart.as<double>() + b->gpart_bra[c],
+                                      b->d_gkc.as<double>() + 6 * b->gkc_q,
                                       b->d_gkc.as<double>(), s));
         SF_TRY(launch_grad_reduce_ket(b->d_gkc.as<double>(), Q.d_base_all.as<long long>(), Q.nbra,
                                       (int)b->pc[Q.pck].s_sa.size(), Q.same,

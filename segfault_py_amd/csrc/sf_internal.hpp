@@ -173,8 +173,8 @@ int launch_grad_class(int qcls, int fp32, const ClassParams& cp, double* gout, h
 // per-block counts of the primitive quartets the Rys kernels evaluate in a segment
 int launch_count_prims(const ClassParams& cp, int nblock, unsigned long long* part, hipStream_t s);
 int launch_grad_reduce_bra(const double* gout, int nslice, long long qstride,
-                           const long long* base, int nbra, double* part, double* kc,
-                           hipStream_t s);
+                           const long long* base, int nbra, double* part, double* ab,
+                           double* kc, hipStream_t s);
 int launch_grad_reduce_ket(const double* kc, const long long* base, int nbra, int nket, int same,
                            double* part, hipStream_t s);
 int launch_grad_atoms(const double* part, const long long* csr_off, const long long* csr_idx,
