@@ -30,6 +30,16 @@ SF_HD SF_INLINE void static_switch(int s, F&& f)
     }
 }
 
+// Calls f(std::integral_constant<int, i>) for i = I..N-1 in order.
+template <int I, int N, typename F>
+SF_HD SF_INLINE void static_for(F&& f)
+{
+    if constexpr (I < N) {
+        f(std::integral_constant<int, I>{});
+        static_for<I + 1, N>(f);
+    }
+}
+
 // Cartesian component i of angular momentum L in the reference order
 // (molecule.py:138-165): lx descending, then ly descending.
 constexpr int cart_n(int L) { return (L + 1) * (L + 2) / 2; }
@@ -111,6 +121,11 @@ struct QuartetShape {
     // (eri_bra_row), the others one per (64 quartets, component pair)
     static constexpr bool SPLIT_ROWS = SPLIT && NB * NC * ND <= 54;
     static constexpr int NSLICE_ERI = SPLIT_ROWS ? NA : NA * NB;
+    // nuclear gradient: classes with an s shell B (and at most 36 density
+    // factors per quartet) run every component of A in one thread
+    // (grad_all_a), the others one wavefront per bra component pair
+    static constexpr bool GRAD_ALL_A = LB == 0 && NA * NC * ND <= 36;
+    static constexpr int NSLICE_GRAD = GRAD_ALL_A ? 1 : NA * NB;
     static constexpr int tile_n1(int T) { return (T == 0 || T == 2 || T == 3) ? NA : (T == 1) ? NC : NB; }
     static constexpr int tile_n2(int T) { return (T == 0) ? NB : (T == 2 || T == 4) ? NC : ND; }
     static constexpr int tile_off(int T)
@@ -793,6 +808,105 @@ SF_HD SF_INLINE void grad_bra_component(const PrimPair* __restrict__ bra, int nb
                         g[7] += (double)(f * x0 * dcy * z0);
                         g[8] += (double)(f * x0 * y0 * dcz);
                     }
+            }
+        }
+    }
+}
+
+// The same derivatives as grad_bra_component for every component FA of A
+// at once, for an s shell B (LB = 0): the nodes and weights of a primitive
+// quartet are evaluated once instead of once per FA.  Dfac[FA*NC*ND + ...]
+// holds each component's density factors; g[0..8] += all of them.
+template <int LA, int LB, int LC, int LD, typename R>
+SF_HD SF_INLINE void grad_all_a(const PrimPair* __restrict__ bra, int nbra,
+                                const PrimPair* __restrict__ ket, int nket, const double* A,
+                                const double* B, const double* C, const double* D,
+                                const double* __restrict__ Dfac, double* __restrict__ g,
+                                const NodeTable& nt)
+{
+    static_assert(LB == 0, "grad_all_a: shell B must be an s shell");
+    using S = QuartetShape<LA, LB, LC, LD>;
+    constexpr int NR = (S::L + 1) / 2 + 1;
+    constexpr int NK = S::NC * S::ND;
+    R CDp[3][LD + 1];
+    R AB[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const R cd = (R)(C[k] - D[k]);
+        AB[k] = (R)(A[k] - B[k]);
+        CDp[k][0] = (R)1;
+#pragma unroll
+        for (int e = 1; e <= LD; ++e) CDp[k][e] = CDp[k][e - 1] * cd;
+    }
+    for (int ik = 0; ik < nket; ++ik) {
+        const PrimPair pk = ket[ik];
+        const double q = pk.p;
+        const R tc = (R)(2.0 * (q - pk.eb));
+        const double QC[3] = {pk.Px - C[0], pk.Py - C[1], pk.Pz - C[2]};
+        for (int ib = 0; ib < nbra; ++ib) {
+            const PrimPair pb = bra[ib];
+            const double p = pb.p;
+            const R ta = (R)(2.0 * (p - pb.eb)), tb = (R)(2.0 * pb.eb);
+            const double PA[3] = {pb.Px - A[0], pb.Py - A[1], pb.Pz - A[2]};
+            const double pq = p + q;
+            const double rs = fast_rsqrt(pq);
+            const double ipq = rs * rs;
+            const double rho = p * q * ipq;
+            const double PQ[3] = {pb.Px - pk.Px, pb.Py - pk.Py, pb.Pz - pk.Pz};
+            const double X = rho * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]);
+            const double pref = 34.986836655249725 * pb.Kp * pk.Kp * rs;
+            double u[NR], w[NR];
+            rys_nodes_tab<NR>(X, nt, u, w);
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                const double t2 = u[r];
+                const R b00 = (R)(0.5 * ipq * t2);
+                const R b10 = (R)(0.5 * pb.ip * (1.0 - q * ipq * t2));
+                const R b01 = (R)(0.5 * pk.ip * (1.0 - p * ipq * t2));
+                R c00[3], c00p[3];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    c00[k] = (R)(PA[k] - q * ipq * PQ[k] * t2);
+                    c00p[k] = (R)(QC[k] + p * ipq * PQ[k] * t2);
+                }
+                const R g00x = (R)(pref * w[r]);
+                static_for<0, S::NA>([&](auto fac) {
+                    constexpr int FA = decltype(fac)::value;
+                    constexpr int ax = cart_l(LA, FA, 0), ay = cart_l(LA, FA, 1), az = cart_l(LA, FA, 2);
+                    DirDeriv<ax, 0, LC, LD, R> Dx;
+                    DirDeriv<ay, 0, LC, LD, R> Dy;
+                    DirDeriv<az, 0, LC, LD, R> Dz;
+                    dir_deriv<ax, 0, LC, LD, R>(g00x, c00[0], c00p[0], b10, b01, b00, AB[0], CDp[0], Dx);
+                    dir_deriv<ay, 0, LC, LD, R>((R)1, c00[1], c00p[1], b10, b01, b00, AB[1], CDp[1], Dy);
+                    dir_deriv<az, 0, LC, LD, R>((R)1, c00[2], c00p[2], b10, b01, b00, AB[2], CDp[2], Dz);
+#pragma unroll
+                    for (int fc = 0; fc < S::NC; ++fc)
+#pragma unroll
+                        for (int fd = 0; fd < S::ND; ++fd) {
+                            const int cx = cart_l(LC, fc, 0), cy = cart_l(LC, fc, 1), cz = cart_l(LC, fc, 2);
+                            const int dx = cart_l(LD, fd, 0), dy = cart_l(LD, fd, 1), dz = cart_l(LD, fd, 2);
+                            const R f = (R)Dfac[FA * NK + fc * S::ND + fd];
+                            const R x0 = Dx.I0[cx][dx], y0 = Dy.I0[cy][dy], z0 = Dz.I0[cz][dz];
+                            const R dax = ta * Dx.IAp[cx][dx] - (R)ax * Dx.IAm[cx][dx];
+                            const R day = ta * Dy.IAp[cy][dy] - (R)ay * Dy.IAm[cy][dy];
+                            const R daz = ta * Dz.IAp[cz][dz] - (R)az * Dz.IAm[cz][dz];
+                            const R dbx = tb * Dx.IBp[cx][dx];
+                            const R dby = tb * Dy.IBp[cy][dy];
+                            const R dbz = tb * Dz.IBp[cz][dz];
+                            const R dcx = tc * Dx.I0[cx + 1][dx] - (cx > 0 ? (R)cx * Dx.I0[cx > 0 ? cx - 1 : 0][dx] : (R)0);
+                            const R dcy = tc * Dy.I0[cy + 1][dy] - (cy > 0 ? (R)cy * Dy.I0[cy > 0 ? cy - 1 : 0][dy] : (R)0);
+                            const R dcz = tc * Dz.I0[cz + 1][dz] - (cz > 0 ? (R)cz * Dz.I0[cz > 0 ? cz - 1 : 0][dz] : (R)0);
+                            g[0] += (double)(f * dax * y0 * z0);
+                            g[1] += (double)(f * x0 * day * z0);
+                            g[2] += (double)(f * x0 * y0 * daz);
+                            g[3] += (double)(f * dbx * y0 * z0);
+                            g[4] += (double)(f * x0 * dby * z0);
+                            g[5] += (double)(f * x0 * y0 * dbz);
+                            g[6] += (double)(f * dcx * y0 * z0);
+                            g[7] += (double)(f * x0 * dcy * z0);
+                            g[8] += (double)(f * x0 * y0 * dcz);
+                        }
+                });
             }
         }
     }

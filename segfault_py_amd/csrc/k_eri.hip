@@ -364,7 +364,7 @@ template <int LA, int LB, int LC, int LD, typename R>
 __global__ __launch_bounds__(256) void k_grad_split(const ClassParams cp, double* __restrict__ gout)
 {
     using S = QuartetShape<LA, LB, LC, LD>;
-    constexpr int NS = S::NA * S::NB, NK = S::NC * S::ND;
+    constexpr int NS = S::NSLICE_GRAD, NK = S::NC * S::ND;
     // derivative integrals need one node more than the integrals (L + 1)
     constexpr int NRG = (S::L + 1) / 2 + 1;
     __shared__ double s_tab[node_table_lds(NRG) > 0 ? node_table_lds(NRG) : 1];
@@ -390,6 +390,26 @@ __global__ __launch_bounds__(256) void k_grad_split(const ClassParams cp, double
     double g[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) g[k] = 0.0;
+    if constexpr (S::GRAD_ALL_A) {
+        // every component of A in this thread (B is an s shell): one slice
+        double Df[S::NA * NK];
+        const double* Pb = P + qm.row[1];   // row 0 (the s function) of shell B
+#pragma unroll
+        for (int fa = 0; fa < S::NA; ++fa) {
+            const double* Pa = P + qm.row[0] + fa * n;
+            const double pab = Pa[qm.col[1]];
+#pragma unroll
+            for (int fc = 0; fc < S::NC; ++fc)
+#pragma unroll
+                for (int fd = 0; fd < S::ND; ++fd)
+                    Df[fa * NK + fc * S::ND + fd] =
+                        fac * (4.0 * pab * P[qm.row[2] + fc * n + qm.col[3] + fd] -
+                               Pa[qm.col[2] + fc] * Pb[qm.col[3] + fd] -
+                               Pa[qm.col[3] + fd] * Pb[qm.col[2] + fc]);
+        }
+        grad_all_a<LA, LB, LC, LD, R>(cp.prim + cp.bra.poff[i], cp.bra.np[i], cp.prim + cp.ket.poff[j],
+                                      cp.ket.np[j], A, B, C, D, Df, g, nt);
+    } else {
     static_switch<0, NS>(s, [&](auto sc) {
         constexpr int FA = decltype(sc)::value / S::NB, FB = decltype(sc)::value % S::NB;
         const double* Pa = P + qm.row[0] + FA * n;   // row fa of P
@@ -408,6 +428,7 @@ __global__ __launch_bounds__(256) void k_grad_split(const ClassParams cp, double
                                                       cp.prim + cp.ket.poff[j], cp.ket.np[j], A,
                                                       B, C, D, Df, g, nt);
     });
+    }
 #pragma unroll
     for (int k = 0; k < 9; ++k) gout[(long long)(s * 9 + k) * cp.qstride + qa] = g[k];
 }
@@ -418,7 +439,7 @@ static int launch_grad_t(int fp32, const ClassParams& cp, double* gout, hipStrea
     using S = QuartetShape<LA, LB, LC, LD>;
     if (cp.nq <= 0) return kOk;
     const long long waves = (cp.nq + 63) / 64;
-    const long long nb = (waves * S::NA * S::NB + 3) / 4;
+    const long long nb = (waves * S::NSLICE_GRAD + 3) / 4;
     if (nb > 0x7fffffffLL) {
         set_error(kEUnsupported, "quartet segment too large (%lld)", cp.nq);
         return kEUnsupported;
@@ -830,7 +851,7 @@ int class_nslice(int qcls)
 {
     switch (qcls) {
 #define X(id, a, b, c, d) \
-    case id: return QuartetShape<a, b, c, d>::NA * QuartetShape<a, b, c, d>::NB;
+    case id: return QuartetShape<a, b, c, d>::NSLICE_GRAD;
         SF_FOR_EACH_CLASS(X)
 #undef X
     }

@@ -41,6 +41,7 @@ def harness():
         H.h_boys01_kernel.argtypes = [ctypes.c_double, _f64p]
         H.h_grad.argtypes = [ctypes.c_int] * 4 + [_f64p, ctypes.c_int, _f64p, ctypes.c_int, _f64p,
                                                   _f64p, _f64p, _f64p, _f64p, _f64p]
+        H.h_grad_slices.argtypes = H.h_grad.argtypes
         H.h_grad_onee.argtypes = [ctypes.c_int, ctypes.c_int, _f64p, ctypes.c_int, _f64p, _f64p,
                                   ctypes.c_int, ctypes.c_int, ctypes.c_int, _f64p, _f64p, _f64p,
                                   _f64p, ctypes.c_double, _f64p]
@@ -207,11 +208,13 @@ def one_electron_model(mol):
     return S, T, V
 
 
-def quartet_grad(shells, a, b, c, d, Dfac):
-    """sum_{ijkl} Dfac * d(ij|kl)/d(A,B,C) from the kernels' derivative code."""
+def quartet_grad(shells, a, b, c, d, Dfac, slices=False):
+    """sum_{ijkl} Dfac * d(ij|kl)/d(A,B,C) from the kernels' derivative code
+    (their decomposition, or with slices one bra component pair at a time)."""
     S = [shells[x] for x in (a, b, c, d)]
     g = np.zeros(9)
-    assert harness().h_grad(S[0].L, S[1].L, S[2].L, S[3].L, prim_pairs(S[0], S[1]),
+    fn = harness().h_grad_slices if slices else harness().h_grad
+    assert fn(S[0].L, S[1].L, S[2].L, S[3].L, prim_pairs(S[0], S[1]),
                             S[0].nprim * S[1].nprim, prim_pairs(S[2], S[3]), S[2].nprim * S[3].nprim,
                             S[0].center, S[1].center, S[2].center, S[3].center,
                             np.ascontiguousarray(Dfac, dtype=np.float64), g) == 0

@@ -270,13 +270,24 @@ extern "C" int h_one_electron(int la, int lb, const double* prim, int np, const 
     return 1;
 }
 
+// The kernels' decomposition: every component of A at once (grad_all_a)
+// where the class allows it, else per bra component pair; slices = 1
+// forces the per-pair path.
 template <int LA, int LB, int LC, int LD>
 static void run_grad(const double* bra, int nb, const double* ket, int nk, const double* A,
                      const double* B, const double* C, const double* D, const double* Dfac,
-                     double* g)
+                     double* g, int slices = 0)
 {
     using S = QuartetShape<LA, LB, LC, LD>;
     for (int k = 0; k < 9; ++k) g[k] = 0.0;
+    if constexpr (S::GRAD_ALL_A) {
+        if (!slices) {
+            grad_all_a<LA, LB, LC, LD, double>((const PrimPair*)bra, nb, (const PrimPair*)ket, nk, A, B,
+                                               C, D, Dfac, g,
+                                               global_node_table<(S::L + 1) / 2 + 1>());
+            return;
+        }
+    }
     for (int s = 0; s < S::NA * S::NB; ++s) {
         static_switch<0, S::NA * S::NB>(s, [&](auto sc) {
             constexpr int FA = decltype(sc)::value / S::NB, FB = decltype(sc)::value % S::NB;
@@ -294,6 +305,20 @@ extern "C" int h_grad(int la, int lb, int lc, int ld, const double* bra, int nb,
 #define X(a, b, c, d)                                                          \
     if (la == a && lb == b && lc == c && ld == d) {                            \
         run_grad<a, b, c, d>(bra, nb, ket, nk, A, B, C, D, Dfac, g);           \
+        return 0;                                                              \
+    }
+    SF_CLASSES(X)
+#undef X
+    return 1;
+}
+
+extern "C" int h_grad_slices(int la, int lb, int lc, int ld, const double* bra, int nb,
+                             const double* ket, int nk, const double* A, const double* B,
+                             const double* C, const double* D, const double* Dfac, double* g)
+{
+#define X(a, b, c, d)                                                          \
+    if (la == a && lb == b && lc == c && ld == d) {                            \
+        run_grad<a, b, c, d>(bra, nb, ket, nk, A, B, C, D, Dfac, g, 1);        \
         return 0;                                                              \
     }
     SF_CLASSES(X)

@@ -294,6 +294,27 @@ def _basis_text(mol):
     return W.even_tempered_text(W.BASIS_SP)
 
 
+def test_gradient_all_components_of_a_equals_slices(water_ccpvdz):
+    """grad_all_a (every component of A in one thread, B an s shell) against
+    the per-bra-component-pair path, for every class that uses it."""
+    shells = water_ccpvdz.shells()
+    last = {}
+    for k, sh in enumerate(shells):
+        last[sh.L] = k
+    rng = np.random.default_rng(21)
+    n = 0
+    # the canonical classes with B an s shell and at most 36 density factors
+    for la, lc, ld in [(0, 0, 0), (1, 0, 0), (1, 1, 0), (2, 0, 0), (2, 1, 0), (2, 2, 0)]:
+        q = (last[la], last[0], last[lc], last[ld])
+        nA, nC, nD = CM.ncart(la), CM.ncart(lc), CM.ncart(ld)
+        Dfac = rng.standard_normal(nA * nC * nD)
+        g1 = CM.quartet_grad(shells, *q, Dfac)
+        g2 = CM.quartet_grad(shells, *q, Dfac, slices=True)
+        assert np.max(np.abs(g1 - g2)) <= 1e-13 * max(1.0, np.max(np.abs(g2))), (la, lc, ld)
+        n += 1
+    assert n == 6
+
+
 def test_one_and_two_electron_gradient_models_match_fd(oracle, water_sp):
     """Per-pair one-electron derivative routine and per-quartet derivative
     ERIs, assembled on the CPU, vs central differences (h = 1e-4 bohr) of
