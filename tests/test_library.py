@@ -90,3 +90,46 @@ def test_batch_and_graph_argument_errors(lib):
     assert lib.sf_jk_graph(None, None, None, None, 0, None) == 1
     assert lib.sf_autotune(None, None, 0, None, shapes, np.zeros(63)) == 1
     assert lib.sf_basis_set_layout(None, np.zeros(1, dtype=np.int64), np.zeros(1, dtype=np.int32)) == 1
+
+
+def test_product_never_touches_the_oracle():
+    """The package, the C-ABI header and bench's timed legs do not import,
+    link or load anything from oracle/ (only tests, smoke() and bench's
+    cpu_baseline leg use it, as the checker)."""
+    pkg = os.path.join(ROOT, "segfault_py_amd")
+    pat = re.compile(r"\boracle\b|liboracle", re.I)
+    hits = []
+    for d, _, files in os.walk(pkg):
+        if "_build" in d or "__pycache__" in d:
+            continue
+        for f in files:
+            if f.endswith((".py", ".hip", ".hpp", ".h", ".cpp", ".inc")):
+                p = os.path.join(d, f)
+                for n, line in enumerate(open(p, errors="replace"), 1):
+                    if pat.search(line):
+                        hits.append(f"{p}:{n}: {line.strip()}")
+    hits += [f"include/sfeng.h: {l.strip()}" for l in open(os.path.join(ROOT, "include", "sfeng.h"))
+             if "liboracle" in l or "#include" in l and "oracle" in l]
+    assert not hits, "\n".join(hits)
+    # bench.py imports the oracle only inside its cpu_baseline function
+    src = open(os.path.join(ROOT, "bench.py")).read()
+    funcs = re.split(r"\ndef ", src)
+    users = [f.split("(")[0] for f in funcs[1:] if "import oracle" in f]
+    assert users == ["cpu_baseline"], users
+    assert "import oracle" not in funcs[0]
+
+
+def test_no_cpu_fallback(monkeypatch):
+    """Without the library, or without a GPU, the product path raises
+    EngineError instead of computing anything on the CPU."""
+    import torch
+    from segfault_py_amd import engine
+    monkeypatch.setattr(engine, "_LIB", None)
+    monkeypatch.setattr(engine, "LIB_PATH", os.path.join(ROOT, "no_such_dir", "libsfeng.so"))
+    with pytest.raises(engine.EngineError, match="not built"):
+        engine.lib()
+    monkeypatch.undo()
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is visible: the no-GPU error path cannot be exercised here")
+    with pytest.raises(engine.EngineError, match="no GPU"):
+        engine.boys(torch.zeros(4, dtype=torch.float64), 2)
