@@ -23,16 +23,13 @@ from .molecule import BOHR_ANGSTROM, Molecule
 
 def nuclear_repulsion_gradient(mol) -> np.ndarray:
     """d E_nuc / dR_A = -sum_B Z_A Z_B (R_A - R_B) / |R_A - R_B|^3 (natom x 3)."""
-    R = np.asarray(mol.xyz__, dtype=np.float64)
+    R = np.asarray(mol.xyz__, dtype=np.float64).reshape(-1, 3)
     Z = np.asarray(mol.numbers, dtype=np.float64)
-    g = np.zeros_like(R)
-    for a in range(len(Z)):
-        for b in range(len(Z)):
-            if a == b:
-                continue
-            d = R[a] - R[b]
-            g[a] -= Z[a] * Z[b] * d / np.linalg.norm(d) ** 3
-    return g
+    d = R[:, None, :] - R[None, :, :]
+    r = np.sqrt(np.einsum("abk,abk->ab", d, d))
+    np.fill_diagonal(r, np.inf)
+    w = np.outer(Z, Z) / r ** 3
+    return -np.einsum("ab,abk->ak", w, d)
 
 
 def rhf_gradient(mol, result: hf.SCFResult, basis: engine.Basis, precision: int = 0) -> np.ndarray:
