@@ -58,7 +58,9 @@ int sf_eri_components(int nf, const int *npgf, const int *poff, const double *co
 /* ---- Shell basis handle -------------------------------------------------
  * nshell contracted shells (L <= 2) in AO order.  Shell s: angular momentum
  * L[s], nprim[s] primitives (stored back to back in expns/coefs), centre
- * center[3s..3s+2] (bohr), atom shell_atom[s].  coefs carry the contraction
+ * center[3s..3s+2] (bohr), atom shell_atom[s] in [0, natom) (status 1
+ * otherwise; in a batch, one of its own molecule's nuclei).  shell_atom may
+ * be NULL: then sf_grad refuses the handle (status 4).  coefs carry the contraction
  * coefficient times sqrt(2^(2L+1.5) a^(L+1.5) / pi^1.5), i.e. the
  * reference's normalize_factors*coefs of the x^L component times
  * sqrt((2L-1)!!); the library applies 1/sqrt((2lx-1)!!(2ly-1)!!(2lz-1)!!)

@@ -48,8 +48,10 @@ def lib():
         L.orc_eri_3d_rys.argtypes = [_i32p, _f64p, _f64p]
         L.orc_eri_3d_rys.restype = ctypes.c_double
         basis = [ctypes.c_int, _i32p, _i32p, _f64p, _f64p, _f64p, _i32p, _f64p]
-        L.orc_eri_tensor.argtypes = basis + [_f64p, ctypes.c_int]
-        L.orc_eri_list.argtypes = basis + [ctypes.c_long, _i32p, _f64p, ctypes.c_int]
+        L.orc_eri_tensor.argtypes = basis + [_f64p, ctypes.c_int, ctypes.c_int]
+        L.orc_eri_list.argtypes = basis + [ctypes.c_long, _i32p, _f64p, ctypes.c_int, ctypes.c_int]
+        L.orc_rys5_fast_init.argtypes = []
+        L.orc_rys5_fast.argtypes = [ctypes.c_double, _f64p, _f64p]
         L.orc_one_electron.argtypes = basis + [ctypes.c_int, _f64p, _f64p, _f64p, _f64p, _f64p,
                                                ctypes.c_int]
         _LIB = L
@@ -96,19 +98,30 @@ def _basis_args(mol):
             np.ascontiguousarray(norms), shell, center)
 
 
-def eri_tensor(mol, nthreads: int = 0) -> np.ndarray:
-    """Dense N^4 ERI tensor, like build_eri_parallel (hf.py:33-61)."""
+def rys5_fast(X: float):
+    """Five nodes and weights from the fitted tables (orc_rys5_fast)."""
+    lib().orc_rys5_fast_init()
+    u = np.zeros(5)
+    w = np.zeros(5)
+    lib().orc_rys5_fast(float(X), u, w)
+    return u, w
+
+
+def eri_tensor(mol, nthreads: int = 0, fast_roots: bool = False) -> np.ndarray:
+    """Dense N^4 ERI tensor, like build_eri_parallel (hf.py:33-61).
+    fast_roots: nodes from the fitted tables (orc_rys5_fast) instead of
+    the exact quadrature."""
     args = _basis_args(mol)
     n = args[0]
     out = np.zeros(n ** 4)
-    lib().orc_eri_tensor(*args, out, nthreads)
+    lib().orc_eri_tensor(*args, out, nthreads, int(fast_roots))
     return out.reshape(n, n, n, n)
 
 
-def eri_list(mol, quartets: np.ndarray, nthreads: int = 0) -> np.ndarray:
+def eri_list(mol, quartets: np.ndarray, nthreads: int = 0, fast_roots: bool = False) -> np.ndarray:
     q = np.ascontiguousarray(quartets, dtype=np.int32).reshape(-1, 4)
     out = np.zeros(len(q))
-    lib().orc_eri_list(*_basis_args(mol), len(q), q.reshape(-1), out, nthreads)
+    lib().orc_eri_list(*_basis_args(mol), len(q), q.reshape(-1), out, nthreads, int(fast_roots))
     return out
 
 

@@ -29,7 +29,7 @@
 /*   F_{m-1} = (2T F_m + e^{-T}) / (2m-1);                                  */
 /* for T >= 30, F_0 = sqrt(pi/T) erf(sqrt T) / 2 and the upward recursion    */
 /*   F_{m+1} = ((2m+1) F_m - e^{-T}) / (2T).                                */
-/* Pinned by tests/test_oracle_boys.py: mpmath hyp1f1 at 40 digits,          */
+/* Pinned by tests/test_oracle.py (test_boys_*): mpmath hyp1f1 at 40 digits,*/
 /* F_m(0) = 1/(2m+1), the F_0 erf closed form and the large-T asymptote.    */
 /* ------------------------------------------------------------------------ */
 void orc_boys(int mmax, double T, double *F)
@@ -68,7 +68,7 @@ void orc_boys(int mmax, double T, double *F)
 /*     coefficients alpha_j, beta_j of the monic orthogonal polynomials in u; */
 /*  3. Golub-Welsch: eigen-decomposition of the Jacobi matrix by cyclic      */
 /*     Jacobi rotations; node = eigenvalue, weight = beta_0 * v_0^2.          */
-/* Pinned by tests/test_oracle_rys.py: mpmath Golub-Welsch at 50 digits and  */
+/* Pinned by tests/test_oracle.py (test_rys_*): mpmath Golub-Welsch, 50 dig.,*/
 /* the moment identities sum w u^k = F_k(X), k < 2n.                         */
 /* ------------------------------------------------------------------------ */
 #define ORC_NGL 96
@@ -208,6 +208,91 @@ int orc_rys_roots(int nroots, double X, double *u, double *w)
 }
 
 /* ------------------------------------------------------------------------ */
+/* Five-node Rys quadrature at the cost of a fitted evaluator.               */
+/* Reference: rys.root5 (core.f90:232-812) evaluates piecewise polynomial    */
+/* fits of degree <= 15 on X intervals and an asymptotic form for large X.   */
+/* Method (own, with the same cost class): on X in [0, 64) cut into 128      */
+/* intervals of width 0.5, each of the 5 nodes and 5 weights is a 16-term    */
+/* Chebyshev interpolant of the exact quadrature above (orc_rys_roots,       */
+/* sampled at the 16 Chebyshev points of the interval when the tables are    */
+/* built); for X >= 64 the quadrature is that of the half-range Gaussian     */
+/* exp(-X t^2) on [0, inf): u_r = h_r / X, w_r = g_r / sqrt(X) with          */
+/* (h_r, g_r) = (X0 u_r(X0), sqrt(X0) w_r(X0)) from the exact solver at      */
+/* X0 = 1e6 (the [1, inf) tail changes the moments by ~e^-X).               */
+/* Used by orc_eri_* with root_method 1 (the CPU-baseline timing and large   */
+/* test tensors); pinned by tests/test_oracle.py (test_rys5_fast_*) against */
+/* the exact solver and the 50-digit mpmath vectors.                         */
+/* ------------------------------------------------------------------------ */
+#define F5_NI 128
+#define F5_W 0.5
+#define F5_XMAX (F5_NI * F5_W)
+#define F5_M 16
+static double f5_c[F5_NI][10][F5_M];
+static double f5_h[5], f5_g[5];
+static int f5_ready = 0;
+
+void orc_rys5_fast_init(void)
+{
+    if (f5_ready) return;
+    double u[5], w[5];
+    for (int k = 0; k < F5_NI; ++k) {
+        const double a = k * F5_W, b = a + F5_W;
+        double f[F5_M][10];
+        for (int j = 0; j < F5_M; ++j) {
+            const double x = cos(ORC_PI * (j + 0.5) / F5_M);
+            orc_rys_roots(5, 0.5 * (a + b) + 0.5 * (b - a) * x, u, w);
+            for (int r = 0; r < 5; ++r) {
+                f[j][r] = u[r];
+                f[j][5 + r] = w[r];
+            }
+        }
+        for (int v = 0; v < 10; ++v)
+            for (int n = 0; n < F5_M; ++n) {
+                double c = 0.0;
+                for (int j = 0; j < F5_M; ++j) c += f[j][v] * cos(ORC_PI * n * (j + 0.5) / F5_M);
+                f5_c[k][v][n] = 2.0 * c / F5_M;
+            }
+    }
+    const double X0 = 1e6;
+    orc_rys_roots(5, X0, u, w);
+    for (int r = 0; r < 5; ++r) {
+        f5_h[r] = u[r] * X0;
+        f5_g[r] = w[r] * sqrt(X0);
+    }
+    f5_ready = 1;
+}
+
+/* 5 nodes (t^2, ascending) and weights; orc_rys5_fast_init() must have run. */
+void orc_rys5_fast(double X, double *u, double *w)
+{
+    if (X < 0.0) X = 0.0;
+    if (X >= F5_XMAX) {
+        const double ix = 1.0 / X, isx = sqrt(ix);
+        for (int r = 0; r < 5; ++r) {
+            u[r] = f5_h[r] * ix;
+            w[r] = f5_g[r] * isx;
+        }
+        return;
+    }
+    int k = (int)(X / F5_W);
+    if (k >= F5_NI) k = F5_NI - 1;
+    const double a = k * F5_W;
+    const double s = (2.0 * (X - a) - F5_W) / F5_W;
+    for (int v = 0; v < 10; ++v) {
+        const double *c = f5_c[k][v];
+        double b1 = 0.0, b2 = 0.0;
+        for (int n = F5_M - 1; n >= 1; --n) {
+            const double t = 2.0 * s * b1 - b2 + c[n];
+            b2 = b1;
+            b1 = t;
+        }
+        const double y = s * b1 - b2 + 0.5 * c[0];
+        if (v < 5) u[v] = y;
+        else w[v - 5] = y;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
 /* Reference Rys ERI path, restated.                                        */
 /* ------------------------------------------------------------------------ */
 
@@ -278,8 +363,11 @@ static double orc_ix(int l1, int l2, int l3, int l4, double x1, double x2, doubl
     return orc_shift(G, l3 + l4 + 1, l1, l2, l3, l4, x1 - x2, x3 - x4);
 }
 
-/* rys.eri_3d_rys, core.f90:917-964: primitive quartet, always 5 nodes. */
-double orc_eri_3d_rys(const int l[12], const double xyz[12], const double ex[4])
+/* rys.eri_3d_rys, core.f90:917-964: primitive quartet, always 5 nodes;
+ * root_method 0 = the exact quadrature (orc_rys_roots), 1 = the fitted
+ * evaluator (orc_rys5_fast, tables built). */
+static double eri_3d_rys_m(const int l[12], const double xyz[12], const double ex[4],
+                           int root_method)
 {
     const double a = ex[0], b = ex[1], c = ex[2], d = ex[3];
     const double a_ = a + b, b_ = c + d, rho = a_ * b_ / (a_ + b_);
@@ -290,7 +378,8 @@ double orc_eri_3d_rys(const int l[12], const double xyz[12], const double ex[4])
         rpq2 += (P - Q) * (P - Q);
     }
     double u[5], w[5];
-    orc_rys_roots(5, rpq2 * rho, u, w);
+    if (root_method == 1) orc_rys5_fast(rpq2 * rho, u, w);
+    else orc_rys_roots(5, rpq2 * rho, u, w);
     double res = 0.0;
     for (int r = 0; r < 5; ++r) {
         double ix = orc_ix(l[0], l[3], l[6], l[9], xyz[0], xyz[3], xyz[6], xyz[9], a, b, c, d, u[r]);
@@ -301,9 +390,15 @@ double orc_eri_3d_rys(const int l[12], const double xyz[12], const double ex[4])
     return res * 2.0 * sqrt(rho / ORC_PI);
 }
 
+double orc_eri_3d_rys(const int l[12], const double xyz[12], const double ex[4])
+{
+    return eri_3d_rys_m(l, xyz, ex, 0);
+}
+
 /* rys.eri_rys, core.f90:967-1002: contraction over the 4 primitive sets. */
-double orc_eri_rys(const int npgf[4], const double *coefs, const double *expns,
-                   const double *norms, const int shells[12], const double centers[12])
+static double eri_rys_m(const int npgf[4], const double *coefs, const double *expns,
+                        const double *norms, const int shells[12], const double centers[12],
+                        int root_method)
 {
     const int o1 = 0, o2 = npgf[0], o3 = o2 + npgf[1], o4 = o3 + npgf[2];
     int l[12];
@@ -319,11 +414,17 @@ double orc_eri_rys(const int npgf[4], const double *coefs, const double *expns,
             for (int i3 = 0; i3 < npgf[2]; ++i3)
                 for (int i4 = 0; i4 < npgf[3]; ++i4) {
                     double ex[4] = {expns[o1 + i1], expns[o2 + i2], expns[o3 + i3], expns[o4 + i4]};
-                    double tmp = orc_eri_3d_rys(l, xyz, ex);
+                    double tmp = eri_3d_rys_m(l, xyz, ex, root_method);
                     res += norms[o1 + i1] * norms[o2 + i2] * norms[o3 + i3] * norms[o4 + i4] *
                            coefs[o1 + i1] * coefs[o2 + i2] * coefs[o3 + i3] * coefs[o4 + i4] * tmp;
                 }
     return res;
+}
+
+double orc_eri_rys(const int npgf[4], const double *coefs, const double *expns,
+                   const double *norms, const int shells[12], const double centers[12])
+{
+    return eri_rys_m(npgf, coefs, expns, norms, shells, centers, 0);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -430,7 +531,7 @@ double orc_eri_mmd(const int npgf[4], const double *coefs, const double *expns,
 /* ------------------------------------------------------------------------ */
 static double quartet_rys(int i, int j, int k, int l, const int *npgf, const int *poff,
                           const double *coefs, const double *expns, const double *norms,
-                          const int *shell, const double *center)
+                          const int *shell, const double *center, int root_method)
 {
     const int f[4] = {i, j, k, l};
     int np[4], sh[12];
@@ -450,16 +551,17 @@ static double quartet_rys(int i, int j, int k, int l, const int *npgf, const int
             ce[3 * a + k3] = center[3 * f[a] + k3];
         }
     }
-    return orc_eri_rys(np, cf, ex, nr, sh, ce);
+    return eri_rys_m(np, cf, ex, nr, sh, ce, root_method);
 }
 
 /* build_eri_parallel, src/hf/hf.py:33-61: unique quartets i>=j, k>=l,
  * ij >= kl with ij = i(i+1)/2 + j, each scattered to its 8 images. */
 void orc_eri_tensor(int N, const int *npgf, const int *poff, const double *coefs,
                     const double *expns, const double *norms, const int *shell,
-                    const double *center, double *eri, int nthreads)
+                    const double *center, double *eri, int nthreads, int root_method)
 {
     const long n = N;
+    if (root_method == 1) orc_rys5_fast_init();
 #ifdef _OPENMP
     if (nthreads > 0) omp_set_num_threads(nthreads);
 #pragma omp parallel for schedule(dynamic)
@@ -469,7 +571,8 @@ void orc_eri_tensor(int N, const int *npgf, const int *poff, const double *coefs
             for (int k = 0; k < N; ++k)
                 for (int l = 0; l <= k; ++l) {
                     if (i * (i + 1) / 2 + j < k * (k + 1) / 2 + l) continue;
-                    double v = quartet_rys(i, j, k, l, npgf, poff, coefs, expns, norms, shell, center);
+                    double v = quartet_rys(i, j, k, l, npgf, poff, coefs, expns, norms, shell, center,
+                                           root_method);
                     const int P[8][4] = {{i, j, k, l}, {j, i, k, l}, {i, j, l, k}, {j, i, l, k},
                                          {k, l, i, j}, {l, k, i, j}, {k, l, j, i}, {l, k, j, i}};
                     for (int m = 0; m < 8; ++m)
@@ -479,16 +582,19 @@ void orc_eri_tensor(int N, const int *npgf, const int *poff, const double *coefs
 
 void orc_eri_list(int N, const int *npgf, const int *poff, const double *coefs,
                   const double *expns, const double *norms, const int *shell,
-                  const double *center, long nq, const int *quartets, double *out, int nthreads)
+                  const double *center, long nq, const int *quartets, double *out, int nthreads,
+                  int root_method)
 {
     (void)N;
+    if (root_method == 1) orc_rys5_fast_init();
 #ifdef _OPENMP
     if (nthreads > 0) omp_set_num_threads(nthreads);
 #pragma omp parallel for schedule(dynamic, 64)
 #endif
     for (long q = 0; q < nq; ++q)
         out[q] = quartet_rys(quartets[4 * q], quartets[4 * q + 1], quartets[4 * q + 2],
-                             quartets[4 * q + 3], npgf, poff, coefs, expns, norms, shell, center);
+                             quartets[4 * q + 3], npgf, poff, coefs, expns, norms, shell, center,
+                             root_method);
 }
 
 /* One-electron integrals, mmd2.py:14-84 (overlap_3d, kinetic_3d,

@@ -35,6 +35,14 @@ void orc_boys(int mmax, double T, double *F);
  * sum_r w_r u_r^k = F_k(X) for k = 0..2*nroots-1.  Returns 0 on success. */
 int orc_rys_roots(int nroots, double X, double *u, double *w);
 
+/* Five-node quadrature from fitted tables (the cost class of rys.root5,
+ * core.f90:232-812): 16-term Chebyshev interpolants of orc_rys_roots on 128
+ * X intervals of width 0.5, the half-range Gauss-Hermite limit for X >= 64.
+ * orc_rys5_fast_init() builds the tables (not thread-safe; the whole-basis
+ * functions below call it before their parallel loops). */
+void orc_rys5_fast_init(void);
+void orc_rys5_fast(double X, double *u, double *w);
+
 /* One contracted Cartesian-component quartet, argument meaning of
  * rys.eri_rys (core.f90:967-1002): for each of the 4 functions, npgf[k]
  * primitives; the flat arrays hold the 4 functions' primitives back to back.
@@ -52,14 +60,16 @@ double orc_coulomb_r(int t, int u, int v, int n, double p,
                      double pcx, double pcy, double pcz, double rpc);
 double orc_eri_3d_rys(const int l[12], const double xyz[12], const double ex[4]);
 
-/* Whole-basis operations over N functions (layout described above). */
+/* Whole-basis operations over N functions (layout described above).
+ * root_method: 0 = exact quadrature (orc_rys_roots), 1 = fitted tables
+ * (orc_rys5_fast). */
 void orc_eri_tensor(int N, const int *npgf, const int *poff, const double *coefs,
                     const double *expns, const double *norms, const int *shell,
-                    const double *center, double *eri, int nthreads);
+                    const double *center, double *eri, int nthreads, int root_method);
 void orc_eri_list(int N, const int *npgf, const int *poff, const double *coefs,
                   const double *expns, const double *norms, const int *shell,
                   const double *center, long nq, const int *quartets,
-                  double *out, int nthreads);
+                  double *out, int nthreads, int root_method);
 void orc_one_electron(int N, const int *npgf, const int *poff, const double *coefs,
                       const double *expns, const double *norms, const int *shell,
                       const double *center, int natom, const double *Z,

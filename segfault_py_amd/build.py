@@ -44,16 +44,32 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found")
 
 
-def build(verbose: bool = False, extra_flags: list[str] | None = None) -> str:
+def _newest_header() -> float:
+    """Latest modification time of the headers every unit includes."""
+    inc = os.path.join(os.path.dirname(HERE), "include")
+    paths = [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith((".hpp", ".inc", ".h"))]
+    paths += [os.path.join(inc, f) for f in os.listdir(inc) if f.endswith(".h")]
+    return max(os.path.getmtime(p) for p in paths)
+
+
+def build(verbose: bool = False, extra_flags: list[str] | None = None,
+          incremental: bool = False) -> str:
+    """Compile every unit and link libsfeng.so.  incremental=True skips a
+    unit whose object is newer than its source and every header."""
     bdir = os.path.join(HERE, "_build")
     os.makedirs(bdir, exist_ok=True)
     cc = hipcc()
     flags = HIPCC_FLAGS + list(extra_flags or [])
     objs = []
     cmds = []
+    hdr = _newest_header() if incremental else 0.0
     for src, defs, oname in UNITS:
         obj = os.path.join(bdir, oname)
         objs.append(obj)
+        if incremental and os.path.exists(obj):
+            t = os.path.getmtime(obj)
+            if t > hdr and t > os.path.getmtime(os.path.join(CSRC, src)):
+                continue
         cmds.append((oname, [cc, *flags, *defs, "-c", os.path.join(CSRC, src), "-o", obj]))
     jobs = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", os.cpu_count() or 4))))
     failed = []
@@ -101,4 +117,4 @@ def build(verbose: bool = False, extra_flags: list[str] | None = None) -> str:
 
 
 if __name__ == "__main__":
-    print(build(verbose="-v" in sys.argv))
+    print(build(verbose="-v" in sys.argv, incremental="--incremental" in sys.argv))

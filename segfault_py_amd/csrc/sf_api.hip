@@ -181,6 +181,7 @@ struct JKGraph {
 
 struct sf_basis {
     int nshell = 0, nbf = 0, natom = 0, nmol = 1;
+    bool have_shell_atom = false;          // shell_atom given at creation (sf_grad needs it)
     std::vector<int> L, nprim, pstart, foff, atom;
     std::vector<int> sh_mol, sh_loc;       // molecule of each shell, index inside it
     std::vector<int> mol_shell, mol_atom;  // nmol+1 ranges of shells and of nuclei
@@ -439,6 +440,7 @@ static int create_basis(int nmol, const int* mol_shell, int nshell, const int* L
     b->nshell = nshell;
     b->natom = natom;
     b->nmol = nmol;
+    b->have_shell_atom = shell_atom != nullptr;
     b->mol_shell.assign(mol_shell, mol_shell + nmol + 1);
     if (mol_atom) b->mol_atom.assign(mol_atom, mol_atom + nmol + 1);
     else b->mol_atom = {0, natom};
@@ -453,6 +455,15 @@ static int create_basis(int nmol, const int* mol_shell, int nshell, const int* L
             }
             if (nprim[s] <= 0 || nprim[s] > 64) {
                 set_error(kEInval, "shell %d: nprim = %d", s, nprim[s]);
+                delete b;
+                return kEInval;
+            }
+            // the atom of a shell must be one of its own molecule's nuclei:
+            // the gradient sums per atom through it (host lists and device
+            // rows), so an index outside the range would address past them
+            if (shell_atom && (shell_atom[s] < b->mol_atom[m] || shell_atom[s] >= b->mol_atom[m + 1])) {
+                set_error(kEInval, "shell %d: atom %d outside molecule %d's atoms [%d, %d)", s,
+                          shell_atom[s], m, b->mol_atom[m], b->mol_atom[m + 1]);
                 delete b;
                 return kEInval;
             }
@@ -1829,6 +1840,10 @@ extern "C" int sf_grad(sf_basis* b, const double* P, const double* W, double* gr
     }
     if (b->nnuc != b->natom || b->natom <= 0) {
         set_error(kEState, "sf_grad: nuclei (sf_basis_set_nuclei) must match the %d atoms", b->natom);
+        return kEState;
+    }
+    if (!b->have_shell_atom) {
+        set_error(kEState, "sf_grad: the basis was created without shell_atom (no shell-to-atom map)");
         return kEState;
     }
     hipStream_t s = (hipStream_t)stream;

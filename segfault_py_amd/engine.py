@@ -126,10 +126,17 @@ def _stream_ptr(torch):
 
 
 # --------------------------------------------------------- batch kernels --
+def _check_vector(x, name):
+    torch = _torch()
+    if not isinstance(x, torch.Tensor) or x.dtype != torch.float64 or not x.is_cuda:
+        raise EngineError(1, f"{name} must be a float64 CUDA tensor")
+    return x.contiguous()
+
+
 def boys(T, mmax: int):
     """F_m(T) for m = 0..mmax on the GPU; T is a float64 CUDA tensor."""
     torch = _torch()
-    T = T.contiguous()
+    T = _check_vector(T, "T")
     F = torch.empty((T.numel(), mmax + 1), dtype=torch.float64, device=T.device)
     check(lib().sf_boys(T.numel(), mmax, ctypes.c_void_p(T.data_ptr()),
                         ctypes.c_void_p(F.data_ptr()), _stream_ptr(torch)))
@@ -139,7 +146,7 @@ def boys(T, mmax: int):
 def rys_roots(X, nroots: int):
     """Rys nodes (as t^2) and weights on the GPU; X is a float64 CUDA tensor."""
     torch = _torch()
-    X = X.contiguous()
+    X = _check_vector(X, "X")
     u = torch.empty((X.numel(), nroots), dtype=torch.float64, device=X.device)
     w = torch.empty_like(u)
     check(lib().sf_rys_roots(X.numel(), nroots, ctypes.c_void_p(X.data_ptr()),
@@ -259,9 +266,21 @@ class Basis:
 
     def _check_matrix(self, P, name="P"):
         torch = _torch()
-        if P.dtype != torch.float64 or tuple(P.shape) != tuple(self.shape) or not P.is_cuda:
+        if (not isinstance(P, torch.Tensor) or P.dtype != torch.float64
+                or tuple(P.shape) != tuple(self.shape) or not P.is_cuda):
             raise EngineError(1, f"{name} must be a float64 CUDA tensor of shape {self.shape}")
         return P.contiguous()
+
+    def _check_out(self, out):
+        """Output pair (J, K): float64 CUDA tensors of ``self.shape``, each
+        contiguous (the library writes them in place, row-major)."""
+        if not isinstance(out, (tuple, list)) or len(out) != 2:
+            raise EngineError(1, "out must be a pair (J, K)")
+        for name, M in zip(("J", "K"), out):
+            self._check_matrix(M, name)
+            if not M.is_contiguous():
+                raise EngineError(1, f"{name} must be contiguous (written in place)")
+        return out
 
     def one_electron(self):
         """S, T, V (float64 CUDA tensors of shape ``self.shape``)."""
@@ -279,8 +298,8 @@ class Basis:
         energy-weighted density W (float64 CUDA tensors); precision 1 runs
         the derivative quartets below tau_mixed in FP32."""
         torch = _torch()
-        P = P.contiguous()
-        W = W.contiguous()
+        P = self._check_matrix(P, "P")
+        W = self._check_matrix(W, "W")
         g = np.zeros(3 * self.mol.natom)
         check(lib().sf_grad(self._h, ctypes.c_void_p(P.data_ptr()), ctypes.c_void_p(W.data_ptr()), g,
                             int(precision), _stream_ptr(torch)))
@@ -348,7 +367,7 @@ class Basis:
         """J, K for the density P (float64 CUDA tensor of shape ``self.shape``)."""
         torch = _torch()
         P = self._check_matrix(P)
-        J, K = (self._zeros(), self._zeros()) if out is None else out
+        J, K = (self._zeros(), self._zeros()) if out is None else self._check_out(out)
         check(lib().sf_jk(self._h, ctypes.c_void_p(P.data_ptr()), ctypes.c_void_p(J.data_ptr()),
                           ctypes.c_void_p(K.data_ptr()), int(precision), _stream_ptr(torch)))
         return J, K
@@ -361,7 +380,7 @@ class Basis:
         if not P.is_contiguous():
             raise EngineError(1, "P must be contiguous (the graph keeps its address)")
         self._check_matrix(P)
-        J, K = out
+        J, K = self._check_out(out)
         check(lib().sf_jk_graph(self._h, ctypes.c_void_p(P.data_ptr()),
                                 ctypes.c_void_p(J.data_ptr()), ctypes.c_void_p(K.data_ptr()),
                                 int(precision), _stream_ptr(torch)))
@@ -407,7 +426,7 @@ class Basis:
         returns J, K, ms[44] (see sf_jk_profile in include/sfeng.h)."""
         torch = _torch()
         P = self._check_matrix(P)
-        J, K = (self._zeros(), self._zeros()) if out is None else out
+        J, K = (self._zeros(), self._zeros()) if out is None else self._check_out(out)
         ms = np.zeros(44)
         check(lib().sf_jk_profile(self._h, ctypes.c_void_p(P.data_ptr()),
                                   ctypes.c_void_p(J.data_ptr()), ctypes.c_void_p(K.data_ptr()),

@@ -11,7 +11,10 @@ angular-momentum class, one-electron derivative kernels, fixed-order
 reductions); the nuclear-repulsion part is O(natom^2) on the host.
 
 ``RHFEnergy.apply(coords_bohr)`` is a ``torch.autograd.Function``: forward
-runs the SCF at the given geometry, backward returns the analytic gradient.
+runs the SCF at the given geometry (and raises if it does not converge: the
+analytic gradient assumes a stationary SCF), backward runs the derivative
+kernels and returns the analytic gradient, so a forward without a backward
+pays for no gradient.
 """
 from __future__ import annotations
 
@@ -66,17 +69,23 @@ class RHFEnergy:
             @staticmethod
             def forward(ctx, coords, template):
                 mol = molecule_at(template, coords.detach().cpu().numpy())
-                o = RHFEnergy.options
+                o = dict(RHFEnergy.options)
                 basis = engine.Basis(mol).screen(o["tau"], o["tau_mixed"])
                 res = hf.scf_diis(mol, tol=o["tol"], basis=basis, precision=o["precision"])
-                g = rhf_gradient(mol, res, basis, o["precision"])
-                ctx.save_for_backward(torch.tensor(g, dtype=coords.dtype, device=coords.device))
+                if not res.converged:
+                    raise RuntimeError(f"RHFEnergy: SCF did not converge in {res.iterations} "
+                                       "iterations; the analytic gradient needs a converged SCF")
+                # the gradient is computed in backward, from what forward leaves here
+                ctx.sf_state = (mol, res, basis, o["precision"])
                 return torch.tensor(res.energy, dtype=coords.dtype, device=coords.device)
 
             @staticmethod
             def backward(ctx, grad_out):
-                (g,) = ctx.saved_tensors
-                return grad_out * g, None
+                mol, res, basis, precision = ctx.sf_state
+                g = rhf_gradient(mol, res, basis, precision)
+                del ctx.sf_state
+                gt = torch.tensor(g, dtype=grad_out.dtype, device=grad_out.device)
+                return grad_out * gt, None
 
         return _F
 

@@ -93,7 +93,7 @@ def _run(mol, mode: str, tol: float, maxiter: int, tau: float, precision: int, t
         etot_old = etot
         if counter > maxiter:
             if verbose:
-                print(f"SCF iteration did not coverge in {maxiter} steps!")
+                print(f"SCF stopped after {maxiter} iterations without converging")
             break
         if mode != "diis" or counter <= 2:
             continue
@@ -116,7 +116,7 @@ def _run(mol, mode: str, tol: float, maxiter: int, tau: float, precision: int, t
             coef = torch.linalg.solve(Bm, rhs)
         except torch.linalg.LinAlgError:
             if verbose:
-                print("DIIS solve Pulay equation failed, last step Fock matrix is resoted")
+                print("singular Pulay system: keeping this iteration's Fock matrix")
             continue
         if not bool(torch.isfinite(coef).all()):
             continue

@@ -1,0 +1,185 @@
+"""Parity with the oracle on every workload family the engine is measured on.
+
+* water, even-tempered s/p (the CPU reference case): GPU DIIS SCF against
+  the oracle's SCF (tau = 0) within 1e-10 Eh, mixed within 1e-6 Eh;
+* C16H34's basis (C 4s3p1d, H 3s1p) on CH4 (N = 43, every quartet class):
+  FP64 J/K against the oracle einsum within 1e-9, FP64 SCF against the
+  oracle SCF within 1e-8 Eh, mixed J/K and SCF against the oracle within
+  1e-6 Eh; C16H34 at full size (N = 508): mixed within 1e-6 Eh of FP64 and
+  bitwise repeatable builds;
+* the 20-water basis on three waters (s/p/d, N = 66): mixed J/K and energy
+  against the oracle;
+* the 256-molecule batch: a batch of its smallest distinct molecules
+  against the oracle's SCF of each molecule within 1e-9 Eh, and the full
+  batch: all converge, sampled molecules within 1e-8 Eh of their own
+  single-molecule SCF, mixed within 1e-6 Eh of FP64.
+
+Large oracle tensors use the oracle's fitted five-node quadrature
+(``fast_roots``), which tests/test_oracle.py pins to the exact quadrature.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TOL_SCF = 1e-10
+
+
+def _e2(P, J, K):
+    return float(0.5 * np.sum(P * (J - 0.5 * K)))
+
+
+def _density(n, seed):
+    rng = np.random.default_rng(seed)
+    C = rng.standard_normal((n, n // 3 + 1))
+    return C @ C.T / n
+
+
+def _oracle_scf(oracle, mol, eri, tol=TOL_SCF):
+    S, T, V = oracle.one_electron(mol, 8)
+    E, _, conv, it = oracle.scf_diis(S, T + V, int(mol.numbers.sum()) // 2,
+                                     lambda P: oracle.jk_dense(eri, P), mol.e_nuc, tol=tol)
+    assert conv
+    return E
+
+
+@pytest.fixture(scope="module")
+def ch4_alkane(oracle):
+    from segfault_py_amd import workloads as W
+    from segfault_py_amd.molecule import Molecule
+    s, x = W.alkane(1, seed=3)
+    mol = Molecule.from_geometry(s, x, W.even_tempered_text(W.BASIS_ALKANE))
+    return mol, oracle.eri_tensor(mol, 16, fast_roots=True)
+
+
+@pytest.fixture(scope="module")
+def water3_spd(oracle):
+    from segfault_py_amd import workloads as W
+    from segfault_py_amd.molecule import Molecule
+    s, x = W.water_cluster(3, seed=7, box=5.0)
+    mol = Molecule.from_geometry(s, x, W.even_tempered_text(W.BASIS_SPD))
+    return mol, oracle.eri_tensor(mol, 16, fast_roots=True)
+
+
+def test_water_sp_scf_matches_oracle_scf(gpu, oracle, water_sp):
+    from segfault_py_amd import hf
+    eri = oracle.eri_tensor(water_sp, 8)
+    E_ref = _oracle_scf(oracle, water_sp, eri)
+    r = hf.scf_diis(water_sp, tol=TOL_SCF, tau=0.0)
+    assert r.converged and abs(r.energy - E_ref) < 1e-10, (r.energy, E_ref)
+    rm = hf.scf_diis(water_sp, tol=TOL_SCF, tau=0.0, precision=1)
+    assert rm.converged and abs(rm.energy - E_ref) < 1e-6
+
+
+def test_alkane_basis_jk_matches_oracle(gpu, oracle, ch4_alkane):
+    """Every quartet class of the C16H34 basis (C d shells, H p shells)."""
+    from segfault_py_amd import engine
+    mol, eri = ch4_alkane
+    Pn = _density(mol.nbf, 31)
+    P = gpu.tensor(Pn, dtype=gpu.float64, device="cuda")
+    Jr, Kr = oracle.jk_dense(eri, Pn)
+    b = engine.Basis(mol).screen(1e-12, 1e-5)
+    counts = b.counts()[0]
+    assert (counts[:, 0] > 0).all(), "CH4 in this basis covers all 21 classes"
+    assert counts[:, 2].sum() > 0, "some quartets fall below tau_mixed"
+    J, K = b.jk(P, 0)
+    assert np.max(np.abs(J.cpu().numpy() - Jr)) < 1e-9
+    assert np.max(np.abs(K.cpu().numpy() - Kr)) < 1e-9
+    Jm, Km = b.jk(P, 1)
+    Jm, Km = Jm.cpu().numpy(), Km.cpu().numpy()
+    assert np.max(np.abs(Jm - Jr)) < 1e-6 and np.max(np.abs(Km - Kr)) < 1e-6
+    assert abs(_e2(Pn, Jm, Km) - _e2(Pn, Jr, Kr)) < 1e-6
+
+
+def test_alkane_basis_scf_matches_oracle(gpu, oracle, ch4_alkane):
+    from segfault_py_amd import hf
+    mol, eri = ch4_alkane
+    E_ref = _oracle_scf(oracle, mol, eri)
+    r = hf.scf_diis(mol, tol=TOL_SCF)
+    assert r.converged and abs(r.energy - E_ref) < 1e-8, (r.energy, E_ref)
+    rm = hf.scf_diis(mol, tol=TOL_SCF, precision=1)
+    assert rm.converged and abs(rm.energy - E_ref) < 1e-6, (rm.energy, E_ref)
+
+
+def test_c16h34_mixed_close_to_fp64_and_repeatable(gpu):
+    """The full C16H34 workload (N = 508): mixed SCF within 1e-6 Eh of FP64;
+    FP64 and mixed builds repeat bit for bit."""
+    from segfault_py_amd import engine, hf, workloads as W
+    mol = W.config_molecule(2)
+    b = engine.Basis(mol).screen(1e-12, 1e-5)
+    P = gpu.tensor(_density(mol.nbf, 32), dtype=gpu.float64, device="cuda")
+    for prec in (0, 1):
+        J1, K1 = b.jk(P, prec)
+        J2, K2 = b.jk(P, prec)
+        assert gpu.equal(J1, J2) and gpu.equal(K1, K2), prec
+    r64 = hf.scf_diis(mol, basis=b, tol=1e-9)
+    rmx = hf.scf_diis(mol, basis=b, tol=1e-9, precision=1)
+    assert r64.converged and rmx.converged
+    assert abs(rmx.energy - r64.energy) < 1e-6
+
+
+def test_mixed_cluster_jk_and_energy_match_oracle(gpu, oracle, water3_spd):
+    """Mixed precision against the oracle (not against the FP64 GPU build)
+    on the 20-water basis: J/K, the two-electron energy and the SCF energy."""
+    from segfault_py_amd import engine, hf
+    mol, eri = water3_spd
+    Pn = _density(mol.nbf, 33)
+    P = gpu.tensor(Pn, dtype=gpu.float64, device="cuda")
+    Jr, Kr = oracle.jk_dense(eri, Pn)
+    b = engine.Basis(mol).screen(1e-12, 1e-5)
+    assert b.counts()[0][:, 2].sum() > 0
+    Jm, Km = b.jk(P, 1)
+    Jm, Km = Jm.cpu().numpy(), Km.cpu().numpy()
+    assert np.max(np.abs(Jm - Jr)) < 1e-6 and np.max(np.abs(Km - Kr)) < 1e-6
+    assert abs(_e2(Pn, Jm, Km) - _e2(Pn, Jr, Kr)) < 1e-6
+    E_ref = _oracle_scf(oracle, mol, eri)
+    rm = hf.scf_diis(mol, basis=b, tol=TOL_SCF, precision=1)
+    assert rm.converged and abs(rm.energy - E_ref) < 1e-6, (rm.energy, E_ref)
+    r64 = hf.scf_diis(mol, basis=b, tol=TOL_SCF)
+    assert r64.converged and abs(r64.energy - E_ref) < 1e-8, (r64.energy, E_ref)
+
+
+def _small_batch_members(limit_nbf=48, count=8):
+    """The smallest distinct molecules of the 256-molecule batch (one per
+    function count, up to ``count``)."""
+    from segfault_py_amd import workloads as W
+    mols = W.batch(256, seed=5)
+    seen, out = set(), []
+    for m in sorted(mols, key=lambda m: m.nbf):
+        if m.nbf <= limit_nbf and m.nbf not in seen:
+            seen.add(m.nbf)
+            out.append(m)
+    # fill up with further molecules of the smallest sizes
+    for m in mols:
+        if len(out) >= count:
+            break
+        if m.nbf <= 32 and all(m is not o for o in out):
+            out.append(m)
+    return out[:count]
+
+
+def test_batch_scf_matches_oracle_per_molecule(gpu, oracle):
+    from segfault_py_amd import hf
+    mols = _small_batch_members()
+    assert len(mols) == 8 and len({m.nbf for m in mols}) >= 4
+    res = hf.scf_batch(mols, tol=TOL_SCF)
+    assert all(res.converged)
+    for m, E in zip(mols, res.energies):
+        E_ref = _oracle_scf(oracle, m, oracle.eri_tensor(m, 16, fast_roots=True))
+        assert abs(E - E_ref) < 1e-9, (m.nbf, E, E_ref)
+    resm = hf.scf_batch(mols, tol=TOL_SCF, precision=1)
+    assert all(resm.converged)
+    assert max(abs(a - b) for a, b in zip(resm.energies, res.energies)) < 1e-6
+
+
+def test_full_256_molecule_batch(gpu):
+    from segfault_py_amd import hf, workloads as W
+    mols = W.batch(256, seed=5)
+    r64 = hf.scf_batch(mols)
+    assert all(r64.converged)
+    rmx = hf.scf_batch(mols, precision=1)
+    assert all(rmx.converged)
+    assert max(abs(a - b) for a, b in zip(rmx.energies, r64.energies)) < 1e-6
+    for k in (0, 85, 170, 255):
+        r = hf.scf_diis(mols[k], tol=1e-8)
+        assert r.converged and abs(r.energy - r64.energies[k]) < 1e-8, k

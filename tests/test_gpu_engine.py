@@ -3,8 +3,8 @@
 Bars: Boys / Rys nodes against 50-digit mpmath vectors; ERIs within 1e-11
 absolute of the oracle; J/K within 1e-10 of the oracle's dense einsum;
 screening lists bit-identical to the CPU builder on the same Q; SCF energies
-within 1e-7 Eh of the reference's published values; mixed precision within
-1e-6 Eh of FP64.
+within 1e-9 Eh of the reference's published values (which the oracle
+reproduces to ~1e-13); mixed precision within 1e-6 Eh of FP64.
 """
 import json
 import os
@@ -130,7 +130,7 @@ def test_jk_cluster_matches_einsum(gpu, oracle):
     from segfault_py_amd.molecule import Molecule
     s, x = W.water_cluster(3, seed=7, box=5.0)
     mol = Molecule.from_geometry(s, x, W.even_tempered_text(W.BASIS_SPD))
-    eri = oracle.eri_tensor(mol, 16)
+    eri = oracle.eri_tensor(mol, 16, fast_roots=True)
     Pn, P = _density(mol.nbf, 4, "cuda")
     Jr, Kr = oracle.jk_dense(eri, Pn)
     b = engine.Basis(mol).screen(1e-12, 0.0)
@@ -286,7 +286,7 @@ def test_scf_energy_with_gpu_jk(gpu, oracle, name, group):
         return J.cpu().numpy(), K.cpu().numpy()
 
     E, _, conv, _ = oracle.scf_diis(S, T + V, int(mol.numbers.sum()) // 2, jk, mol.e_nuc)
-    assert conv and abs(E - ref["etot"]) < 1e-7, (E, ref["etot"])
+    assert conv and abs(E - ref["etot"]) < 1e-9, (E, ref["etot"])
 
 
 def test_one_electron_kernel_matches_oracle(gpu, oracle, water_ccpvdz):
@@ -309,7 +309,7 @@ def test_gpu_scf_diis_matches_reference_energy(gpu, name, group):
     mol = Molecule(os.path.join(GOLDEN, "molecules", f"{name}.xyz"),
                    os.path.join(GOLDEN, "cc-pvdz-HCNO.dat"))
     r = hf.scf_diis(mol)
-    assert r.converged and abs(r.energy - ref["etot"]) < 1e-7, (r.energy, ref["etot"])
+    assert r.converged and abs(r.energy - ref["etot"]) < 1e-9, (r.energy, ref["etot"])
     # mixed precision stays within 1e-6 Eh of FP64
     rm = hf.scf_diis(mol, precision=1)
     assert rm.converged and abs(rm.energy - r.energy) < 1e-6

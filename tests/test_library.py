@@ -64,6 +64,58 @@ def test_argument_errors_are_status_codes(lib):
         engine.check(rc)
 
 
+def test_shell_atom_must_be_in_its_molecule(lib):
+    """A shell's atom index outside its own molecule's nuclei is rejected
+    (status 1, no handle) before any device work: the gradient addresses
+    per-atom rows and lists through it."""
+    h = ctypes.c_void_p()
+    L = np.zeros(2, dtype=np.int32)
+    npr = np.ones(2, dtype=np.int32)
+    for atoms in ([0, 2], [-1, 0], [0, 5]):
+        rc = lib.sf_basis_create(2, L, npr, np.zeros(6), np.ones(2), np.ones(2), 2,
+                                 np.array(atoms, dtype=np.int32), ctypes.byref(h))
+        assert rc == 1 and not h.value, atoms
+        assert b"outside molecule" in lib.sf_last_error()
+    # batch: molecule 1 owns nuclei [1, 2); its shell may not name nucleus 0
+    rc = lib.sf_basis_create_batch(2, np.array([0, 1, 2], dtype=np.int32), 2, L, npr, np.zeros(6),
+                                   np.ones(2), np.ones(2), 2, np.array([0, 1, 2], dtype=np.int32),
+                                   np.array([0, 0], dtype=np.int32), ctypes.byref(h))
+    assert rc == 1 and not h.value
+
+
+def test_wrappers_check_tensors_before_the_library(monkeypatch):
+    """The Python wrappers reject CPU, float32, wrongly shaped and (for
+    outputs) non-contiguous tensors with EngineError before any library
+    call: a bad pointer reaching a kernel would fault the GPU."""
+    import torch
+    from segfault_py_amd import engine
+
+    class NoLib:
+        def __getattr__(self, name):
+            raise AssertionError(f"library called: {name}")
+
+    monkeypatch.setattr(engine, "_torch", lambda: torch)
+    monkeypatch.setattr(engine, "lib", lambda: NoLib())
+    b = engine.Basis.__new__(engine.Basis)
+    b.shape = (4, 4)
+    b._h = None
+    b.mol = type("M", (), {"natom": 1})()
+    good = torch.zeros(4, 4, dtype=torch.float64)   # CPU: still rejected (not CUDA)
+    for bad in (good, good.float(), torch.zeros(3, 3, dtype=torch.float64), None):
+        with pytest.raises(engine.EngineError):
+            b.jk(bad)
+        with pytest.raises(engine.EngineError):
+            b.gradient(good, bad)
+        with pytest.raises(engine.EngineError):
+            b._check_out((bad, bad))
+    with pytest.raises(engine.EngineError):
+        b._check_out(good)
+    with pytest.raises(engine.EngineError):
+        engine.boys(torch.zeros(4, dtype=torch.float32), 2)
+    with pytest.raises(engine.EngineError):
+        engine.rys_roots(torch.zeros(4, dtype=torch.float64), 2)   # CPU tensor
+
+
 def test_batch_and_graph_argument_errors(lib):
     """Bad molecule ranges, tuning shapes and null handles are rejected
     before any device work."""

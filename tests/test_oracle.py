@@ -77,6 +77,48 @@ def test_rys_moment_identities(oracle, n):
             assert abs(np.sum(w * u ** k) - F[k]) <= 3e-13 * F[k], (n, X, k)
 
 
+def test_rys5_fast_matches_exact_and_mpmath(oracle):
+    """The fitted five-node evaluator (orc_rys5_fast: Chebyshev interpolants
+    of the exact quadrature on X < 64, the half-range Gauss-Hermite limit
+    above) against the exact quadrature on a dense X sample crossing every
+    interval edge and the switch at 64, and against the 50-digit mpmath
+    vectors."""
+    rng = np.random.default_rng(5)
+    Xs = np.concatenate([rng.uniform(0, 90, 3000), np.arange(0, 70.01, 0.25),
+                         [63.999999, 64.0, 64.000001, 1e-12, 500.0, 1e4]])
+    worst_u = worst_w = 0.0
+    for X in Xs:
+        u, w = oracle.rys_roots(5, X)
+        uf, wf = oracle.rys5_fast(X)
+        worst_u = max(worst_u, float(np.max(np.abs(uf - u) / u)))
+        worst_w = max(worst_w, float(np.max(np.abs(wf - w) / w)))
+    assert worst_u < 5e-14 and worst_w < 2e-13, (worst_u, worst_w)
+    for pt in _golden("rys_mpmath.json")["points"]:
+        if pt["n"] != 5:
+            continue
+        uf, wf = oracle.rys5_fast(float(pt["X"]))
+        ur = np.array([float(v) for v in pt["u"]])
+        wr = np.array([float(v) for v in pt["w"]])
+        assert np.max(np.abs(uf - ur) / ur) < 2e-13 and np.max(np.abs(wf - wr) / wr) < 2e-13, pt["X"]
+
+
+def test_rys5_fast_moment_identities(oracle):
+    for X in (0.0, 0.3, 7.7, 31.4, 63.9, 64.0, 150.0):
+        u, w = oracle.rys5_fast(X)
+        F = oracle.boys(9, X)
+        for k in range(10):
+            assert abs(np.sum(w * u ** k) - F[k]) <= 1e-12 * F[k], (X, k)
+
+
+def test_eri_fast_roots_equal_exact(oracle, water_sp):
+    """The ERI path with the fitted nodes equals the exact-node path."""
+    rng = np.random.default_rng(6)
+    q = rng.integers(0, water_sp.nbf, (600, 4)).astype(np.int32)
+    a = oracle.eri_list(water_sp, q, 4)
+    b = oracle.eri_list(water_sp, q, 4, fast_roots=True)
+    assert np.max(np.abs(a - b)) < 1e-13
+
+
 # ---------------------------------------------------------- primitives ----
 def _rand_prim(rng, ls):
     xyz = rng.uniform(-1.5, 1.5, 12)
