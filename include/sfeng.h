@@ -55,6 +55,20 @@ int sf_eri_components(int nf, const int *npgf, const int *poff, const double *co
                       const double *center, long long nq, const int *quartets,
                       double *out);
 
+/* sf_eri_components without any device allocation: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments, checks and
+ * results as sf_eri_components (bit for bit); the function table, the
+ * quartet list and the results are staged in a static 32 MiB device array
+ * per GPU (held until the process ends), so one call holds at most ~1.3e6
+ * quartets (each section 256-byte aligned: 4 nf + 4 nf + 24 nprim + 12 nf +
+ * 24 nf + 24 nq bytes); larger calls return status 5.  Calls on one GPU
+ * share the array: calls that overlap on one GPU race. */
+int sf_eri_components_noalloc(int nf, const int *npgf, const int *poff, const double *coefs,
+                              const double *expns, const double *norms, const int *shell,
+                              const double *center, long long nq, const int *quartets,
+                              double *out);
+
 /* ---- Shell basis handle -------------------------------------------------
  * nshell contracted shells (L <= 2) in AO order.  Shell s: angular momentum
  * L[s], nprim[s] primitives (stored back to back in expns/coefs), centre

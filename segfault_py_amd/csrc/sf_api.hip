@@ -15,6 +15,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <thread>
@@ -291,10 +292,12 @@ extern "C" int sf_rys_roots(long long n, int nroots, const double* X, double* u,
     return launch_rys(n, nroots, X, u, w, (hipStream_t)stream);
 }
 
-extern "C" int sf_eri_components(int nf, const int* npgf, const int* poff, const double* coefs,
+// Argument checks shared by sf_eri_components and its allocation-free
+// variant; *nprim_out = primitives the function table spans.
+static int check_components_args(int nf, const int* npgf, const int* poff, const double* coefs,
                                  const double* expns, const double* norms, const int* shell,
                                  const double* center, long long nq, const int* quartets,
-                                 double* out)
+                                 double* out, long long* nprim_out)
 {
     if (nf <= 0 || nq < 0 || !npgf || !poff || !coefs || !expns || !norms || !shell || !center ||
         (nq > 0 && (!quartets || !out))) {
@@ -329,6 +332,18 @@ extern "C" int sf_eri_components(int nf, const int* npgf, const int* poff, const
             return kEUnsupported;
         }
     }
+    *nprim_out = nprim;
+    return kOk;
+}
+
+extern "C" int sf_eri_components(int nf, const int* npgf, const int* poff, const double* coefs,
+                                 const double* expns, const double* norms, const int* shell,
+                                 const double* center, long long nq, const int* quartets,
+                                 double* out)
+{
+    long long nprim = 0;
+    SF_TRY(check_components_args(nf, npgf, poff, coefs, expns, norms, shell, center, nq, quartets,
+                                 out, &nprim));
     if (nq == 0) return kOk;
     DevBuf a, b, c, d, e, f, g, h, qb, ob;
     SF_TRY(a.ensure(nf * sizeof(int)));
@@ -352,6 +367,91 @@ extern "C" int sf_eri_components(int nf, const int* npgf, const int* poff, const
                                  e.as<double>(), g.as<int>(), h.as<double>(), nq, qb.as<int>(),
                                  ob.as<double>(), 0));
     SF_HIP(hipMemcpy(out, ob.p, nq * sizeof(double), hipMemcpyDeviceToHost));
+    return kOk;
+}
+
+// ---------------------------------------------- allocation-free staging ----
+// sf_eri_components_noalloc stages the function table, the quartet list
+// and the results in this array instead of allocating device memory per
+// call.  In host code `scratch` names a host-side stand-in, not the device
+// address: the address is fetched once per device with hipGetSymbolAddress
+// (scratch_ptr) and only that pointer is handed to copies and kernels.
+// Every device holds its own copy until the process ends; calls on one
+// device share it, so calls that overlap on one device race.
+static constexpr size_t kScratchBytes = 32u << 20;  // 32 MiB per device
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+static constexpr int kMaxScratchDevices = 64;
+static void* g_scratch_dev[kMaxScratchDevices] = {nullptr};
+static std::mutex g_scratch_mu;
+
+// Device address of `scratch` on the current device (looked up on the
+// first call made with that device current).
+static int scratch_ptr(unsigned char** out)
+{
+    int dev = 0;
+    SF_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxScratchDevices) {
+        set_error(kEUnsupported, "device %d: beyond the %d devices with staging scratch", dev,
+                  kMaxScratchDevices);
+        return kEUnsupported;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        SF_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) {
+            set_error(kEHip, "hipGetSymbolAddress(scratch) returned a null address");
+            return kEHip;
+        }
+        g_scratch_dev[dev] = p;
+    }
+    *out = static_cast<unsigned char*>(g_scratch_dev[dev]);
+    return kOk;
+}
+
+static size_t align256(size_t n) { return (n + 255) & ~size_t(255); }
+
+extern "C" int sf_eri_components_noalloc(int nf, const int* npgf, const int* poff,
+                                         const double* coefs, const double* expns,
+                                         const double* norms, const int* shell,
+                                         const double* center, long long nq, const int* quartets,
+                                         double* out)
+{
+    long long nprim = 0;
+    SF_TRY(check_components_args(nf, npgf, poff, coefs, expns, norms, shell, center, nq, quartets,
+                                 out, &nprim));
+    if (nq == 0) return kOk;
+    // sections of the staging array, each 256-byte aligned
+    const size_t sz[9] = {nf * sizeof(int),          nf * sizeof(int),
+                          nprim * sizeof(double),    nprim * sizeof(double),
+                          nprim * sizeof(double),    3 * (size_t)nf * sizeof(int),
+                          3 * (size_t)nf * sizeof(double), 4 * (size_t)nq * sizeof(int),
+                          (size_t)nq * sizeof(double)};
+    size_t off[9], total = 0;
+    for (int k = 0; k < 9; ++k) {
+        off[k] = total;
+        total += align256(sz[k]);
+    }
+    if (total > kScratchBytes) {
+        set_error(kEUnsupported,
+                  "sf_eri_components_noalloc: %zu bytes of tables and results exceed the %zu-byte "
+                  "staging array (use sf_eri_components, or fewer quartets per call)",
+                  total, kScratchBytes);
+        return kEUnsupported;
+    }
+    unsigned char* base = nullptr;
+    SF_TRY(scratch_ptr(&base));
+    const void* src[8] = {npgf, poff, coefs, expns, norms, shell, center, quartets};
+    for (int k = 0; k < 8; ++k)
+        SF_HIP(hipMemcpy(base + off[k], src[k], sz[k], hipMemcpyHostToDevice));
+    SF_TRY(launch_eri_components(
+        nf, reinterpret_cast<const int*>(base + off[0]), reinterpret_cast<const int*>(base + off[1]),
+        reinterpret_cast<const double*>(base + off[2]), reinterpret_cast<const double*>(base + off[3]),
+        reinterpret_cast<const double*>(base + off[4]), reinterpret_cast<const int*>(base + off[5]),
+        reinterpret_cast<const double*>(base + off[6]), nq,
+        reinterpret_cast<const int*>(base + off[7]), reinterpret_cast<double*>(base + off[8]), 0));
+    SF_HIP(hipMemcpy(out, base + off[8], sz[8], hipMemcpyDeviceToHost));
     return kOk;
 }
 

@@ -64,6 +64,7 @@ def lib():
     L.sf_eri_rys.argtypes = [_i32p, _f64p, _f64p, _f64p, _i32p, _f64p, _f64p]
     L.sf_eri_components.argtypes = [ctypes.c_int, _i32p, _i32p, _f64p, _f64p, _f64p, _i32p, _f64p,
                                     ctypes.c_longlong, _i32p, _f64p]
+    L.sf_eri_components_noalloc.argtypes = L.sf_eri_components.argtypes
     L.sf_basis_create.argtypes = [ctypes.c_int, _i32p, _i32p, _f64p, _f64p, _f64p, ctypes.c_int,
                                   _i32p, ctypes.POINTER(vp)]
     L.sf_basis_destroy.argtypes = [vp]
@@ -105,7 +106,7 @@ def check(code: int):
 def exported_symbols() -> list[str]:
     """Names declared in include/sfeng.h (for the load/export test)."""
     return ["sf_last_error", "sf_version", "sf_boys", "sf_rys_roots", "sf_eri_rys",
-            "sf_eri_components", "sf_basis_create", "sf_basis_destroy", "sf_basis_nbf",
+            "sf_eri_components", "sf_eri_components_noalloc", "sf_basis_create", "sf_basis_destroy", "sf_basis_nbf",
             "sf_schwarz", "sf_screen", "sf_quartet_counts", "sf_pair_list", "sf_quartet_list",
             "sf_eri_dense", "sf_jk", "sf_jk_profile", "sf_basis_set_nuclei", "sf_one_electron",
             "sf_grad", "sf_basis_create_batch", "sf_basis_set_layout", "sf_basis_layout",
@@ -183,14 +184,16 @@ class _Rys:
 rys = _Rys()
 
 
-def eri_components(mol, quartets) -> np.ndarray:
-    """Contracted component quartets (function indices) of a Molecule."""
+def eri_components(mol, quartets, noalloc: bool = False) -> np.ndarray:
+    """Contracted component quartets (function indices) of a Molecule;
+    noalloc stages through the library's static device array
+    (sf_eri_components_noalloc) instead of allocating per call."""
     npgf, poff, coefs, expns, norms, shell, center = mol.cgf_arrays()
     q = np.ascontiguousarray(quartets, dtype=np.int32).reshape(-1, 4)
     out = np.zeros(len(q))
-    check(lib().sf_eri_components(len(npgf), npgf, poff, np.ascontiguousarray(coefs),
-                                  np.ascontiguousarray(expns), np.ascontiguousarray(norms), shell,
-                                  center, len(q), q.reshape(-1), out))
+    fn = lib().sf_eri_components_noalloc if noalloc else lib().sf_eri_components
+    check(fn(len(npgf), npgf, poff, np.ascontiguousarray(coefs), np.ascontiguousarray(expns),
+             np.ascontiguousarray(norms), shell, center, len(q), q.reshape(-1), out))
     return out
 
 
