@@ -52,6 +52,8 @@ def lib():
         L.orc_eri_list.argtypes = basis + [ctypes.c_long, _i32p, _f64p, ctypes.c_int, ctypes.c_int]
         L.orc_rys5_fast_init.argtypes = []
         L.orc_rys5_fast.argtypes = [ctypes.c_double, _f64p, _f64p]
+        L.orc_time_rys5_fast.argtypes = [ctypes.c_long, ctypes.c_double]
+        L.orc_time_rys5_fast.restype = ctypes.c_double
         L.orc_one_electron.argtypes = basis + [ctypes.c_int, _f64p, _f64p, _f64p, _f64p, _f64p,
                                                ctypes.c_int]
         _LIB = L

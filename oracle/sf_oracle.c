@@ -8,9 +8,11 @@
  * the reference repository root).  Parity of each function is pinned by the
  * tests named in its comment.
  */
+#define _POSIX_C_SOURCE 199309L /* clock_gettime */
 #include "sf_oracle.h"
 
 #include <math.h>
+#include <time.h>
 #include <stdlib.h>
 #include <string.h>
 #ifdef _OPENMP
@@ -290,6 +292,24 @@ void orc_rys5_fast(double X, double *u, double *w)
         if (v < 5) u[v] = y;
         else w[v - 5] = y;
     }
+}
+
+/* Seconds one thread spends on n fitted five-node evaluations at X spread
+ * over [0, xmax) (the root share of the CPU-baseline cost split). */
+double orc_time_rys5_fast(long n, double xmax)
+{
+    orc_rys5_fast_init();
+    double u[5], w[5], sink = 0.0;
+    struct timespec t0, t1;
+    clock_gettime(CLOCK_MONOTONIC, &t0);
+    for (long i = 0; i < n; ++i) {
+        orc_rys5_fast(xmax * (double)((i * 7919) % n) / (double)n, u, w);
+        sink += u[4] + w[0];
+    }
+    clock_gettime(CLOCK_MONOTONIC, &t1);
+    volatile double keep = sink;
+    (void)keep;
+    return (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
 }
 
 /* ------------------------------------------------------------------------ */

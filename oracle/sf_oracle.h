@@ -42,6 +42,8 @@ int orc_rys_roots(int nroots, double X, double *u, double *w);
  * functions below call it before their parallel loops). */
 void orc_rys5_fast_init(void);
 void orc_rys5_fast(double X, double *u, double *w);
+/* Seconds one thread needs for n orc_rys5_fast calls, X spread over [0, xmax). */
+double orc_time_rys5_fast(long n, double xmax);
 
 /* One contracted Cartesian-component quartet, argument meaning of
  * rys.eri_rys (core.f90:967-1002): for each of the 4 functions, npgf[k]

@@ -197,6 +197,103 @@ def eri_components(mol, quartets, noalloc: bool = False) -> np.ndarray:
     return out
 
 
+# ------------------------------------------- the reference's CGF list --
+# Cartesian component order inside a shell (src/hf/molecule.py:138-165).
+_CART = {L: [(lx, ly, L - lx - ly) for lx in range(L, -1, -1) for ly in range(L - lx, -1, -1)]
+         for L in range(4)}
+
+
+def _dfact2(n: int) -> float:
+    r = 1.0
+    while n > 1:
+        r *= n
+        n -= 2
+    return r
+
+
+def shells_from_cgfs(cgfs, rtol: float = 1e-12):
+    """Group the reference's per-component function list into shells.
+
+    ``cgfs`` is the list ``scf_diis`` builds (src/hf/hf.py:254-259): one
+    ``ContractedGaussianFunction`` per Cartesian component (attributes
+    ``npgf, expns, coefs, shell, center, normalize_factors``,
+    src/hf/molecule.py:173-226), in the AO order of molecule.py:138-165.
+    A shell is a run of consecutive components in that order with the same
+    centre and exponents; every component's ``coefs * normalize_factors *
+    sqrt((2lx-1)!!(2ly-1)!!(2lz-1)!!)`` must agree (to ``rtol``: the
+    reference rescales one coefficient list shared by a shell's components
+    once per component) and is the shell's coefficient.  Atoms are the
+    distinct centres in order of first appearance.
+
+    Returns (L, nprim, center[3*nshell], expns, coefs, shell_atom,
+    atom_xyz[natom, 3]); raises ValueError when the list is not made of
+    whole shells.
+    """
+    L_, nprim, center, expns, coefs, sh_atom = [], [], [], [], [], []
+    atoms = []
+    k = 0
+    n = len(cgfs)
+    while k < n:
+        c0 = cgfs[k]
+        comp = tuple(int(v) for v in c0.shell)
+        L = sum(comp)
+        if L not in _CART or comp != _CART[L][0]:
+            raise ValueError(f"function {k}: component {comp} does not start a shell "
+                             f"(expected {(L, 0, 0)})")
+        nc = len(_CART[L])
+        if k + nc > n:
+            raise ValueError(f"function {k}: shell of L={L} is cut off by the end of the list")
+        ex = np.asarray(c0.expns, dtype=np.float64)
+        cen = np.asarray(c0.center, dtype=np.float64).reshape(3)
+        if int(c0.npgf) != len(ex):
+            raise ValueError(f"function {k}: npgf {c0.npgf} != {len(ex)} exponents")
+        ref = None
+        for m, want in enumerate(_CART[L]):
+            c = cgfs[k + m]
+            got = tuple(int(v) for v in c.shell)
+            if got != want:
+                raise ValueError(f"function {k + m}: component {got}, expected {want}")
+            if not np.array_equal(np.asarray(c.expns, dtype=np.float64), ex):
+                raise ValueError(f"function {k + m}: exponents differ from its shell's")
+            if not np.array_equal(np.asarray(c.center, dtype=np.float64).reshape(3), cen):
+                raise ValueError(f"function {k + m}: centre differs from its shell's")
+            f2 = _dfact2(2 * got[0] - 1) * _dfact2(2 * got[1] - 1) * _dfact2(2 * got[2] - 1)
+            eff = (np.asarray(c.coefs, dtype=np.float64)
+                   * np.asarray(c.normalize_factors, dtype=np.float64) * np.sqrt(f2))
+            if ref is None:
+                ref = eff
+            elif not np.allclose(eff, ref, rtol=rtol, atol=0.0):
+                raise ValueError(f"function {k + m}: normalised coefficients differ from "
+                                 "its shell's first component")
+        for ia, a in enumerate(atoms):
+            if np.array_equal(a, cen):
+                break
+        else:
+            ia = len(atoms)
+            atoms.append(cen)
+        L_.append(L)
+        nprim.append(len(ex))
+        center.append(cen)
+        expns.append(ex)
+        coefs.append(ref)
+        sh_atom.append(ia)
+        k += nc
+    return (np.array(L_, dtype=np.int32), np.array(nprim, dtype=np.int32),
+            np.ascontiguousarray(np.concatenate(center)), np.ascontiguousarray(np.concatenate(expns)),
+            np.ascontiguousarray(np.concatenate(coefs)), np.array(sh_atom, dtype=np.int32),
+            np.array(atoms, dtype=np.float64).reshape(-1, 3))
+
+
+def build_eri_parallel(cgfs, ncgf: int) -> np.ndarray:
+    """Drop-in for the reference's ``build_eri_parallel(cgfs, ncgf)``
+    (src/hf/hf.py:33-61, called at :264): the dense (ncgf,)*4 tensor in
+    chemists' notation for the reference's own function list, every unique
+    quartet computed (no screening) on the GPU."""
+    if ncgf != len(cgfs):
+        raise ValueError(f"ncgf = {ncgf} but the list holds {len(cgfs)} functions")
+    return Basis.from_cgfs(cgfs).screen(0.0, 0.0).eri_dense()
+
+
 # ------------------------------------------------------------ the basis --
 def _shell_arrays(mols):
     """Concatenated shell arrays of a list of molecules (AO order kept)."""
@@ -222,6 +319,20 @@ def _shell_arrays(mols):
             np.array(mol_shell, dtype=np.int32), np.array(mol_atom, dtype=np.int32))
 
 
+class _CgfMolecule:
+    """What a Basis built from a function list knows of its molecule: the
+    atom centres (bohr) and, if given, their nuclear charges."""
+
+    def __init__(self, xyz, numbers):
+        self.xyz__ = np.asarray(xyz, dtype=np.float64).reshape(-1, 3)
+        self.natom = len(self.xyz__)
+        if numbers is not None:
+            numbers = np.asarray(numbers, dtype=np.float64).reshape(-1)
+            if len(numbers) != self.natom:
+                raise ValueError(f"{len(numbers)} nuclear charges for {self.natom} centres")
+        self.numbers = numbers
+
+
 class Basis:
     """A shell basis on the GPU: Schwarz screening, quartet lists, J/K.
 
@@ -240,14 +351,36 @@ class Basis:
         self._h = h
         self._finish([mol])
 
+    @classmethod
+    def from_cgfs(cls, cgfs, numbers=None) -> "Basis":
+        """A basis from the reference's per-component function list
+        (``cgfs`` of src/hf/hf.py:254-259; see shells_from_cgfs).  Atoms are
+        the distinct function centres in order of first appearance;
+        ``numbers`` (their nuclear charges, e.g. ``[a.number for a in
+        mol.atoms]`` of the reference's Molecule) places point charges there
+        for the one-electron integrals and gradients; without it only the
+        two-electron entry points have nuclei to use (none)."""
+        L, nprim, center, expns, coefs, atom, axyz = shells_from_cgfs(cgfs)
+        self = cls.__new__(cls)
+        self.mol = _CgfMolecule(axyz, numbers)
+        self.mols = [self.mol]
+        self.nshell = len(L)
+        h = ctypes.c_void_p()
+        check(lib().sf_basis_create(self.nshell, L, nprim, center, expns, coefs, len(axyz), atom,
+                                    ctypes.byref(h)))
+        self._h = h
+        self._finish([self.mol])
+        return self
+
     def _finish(self, mols):
         n = ctypes.c_int()
         check(lib().sf_basis_nbf(self._h, ctypes.byref(n)))
         self.nbf = n.value
-        Z = np.ascontiguousarray(np.concatenate([m.numbers for m in mols]), dtype=np.float64)
-        xyz = np.ascontiguousarray(np.concatenate([np.asarray(m.xyz__, dtype=np.float64).reshape(-1, 3)
-                                                   for m in mols]).reshape(-1))
-        check(lib().sf_basis_set_nuclei(self._h, len(Z), Z, xyz))
+        if all(m.numbers is not None for m in mols):
+            Z = np.ascontiguousarray(np.concatenate([m.numbers for m in mols]), dtype=np.float64)
+            xyz = np.ascontiguousarray(np.concatenate([np.asarray(m.xyz__, dtype=np.float64).reshape(-1, 3)
+                                                       for m in mols]).reshape(-1))
+            check(lib().sf_basis_set_nuclei(self._h, len(Z), Z, xyz))
         self.shape = (self.nbf, self.nbf)
         self.tau = None
         self.tau_mixed = None

@@ -315,6 +315,25 @@ def test_gpu_scf_diis_matches_reference_energy(gpu, name, group):
     assert rm.converged and abs(rm.energy - r.energy) < 1e-6
 
 
+def test_gpu_scf_reproduces_every_published_energy(gpu):
+    """All molecules of the reference's benchmark (molecule_test.json,
+    cc-pVDZ, Cartesian): the GPU DIIS SCF with the reference's tolerance
+    (1e-8) reproduces each published energy within 1e-9 Eh."""
+    from segfault_py_amd import hf
+    from segfault_py_amd.molecule import Molecule
+    worst = []
+    for group, mols in _golden("molecule_test.json").items():
+        for name, ref in mols.items():
+            mol = Molecule(os.path.join(GOLDEN, "molecules", f"{name}.xyz"),
+                           os.path.join(GOLDEN, "cc-pvdz-HCNO.dat"))
+            assert mol.nbf == ref["ncgf"], name
+            r = hf.scf_diis(mol)
+            assert r.converged, name
+            worst.append((abs(r.energy - ref["etot"]), name))
+    assert len(worst) >= 30
+    assert max(worst)[0] < 1e-9, sorted(worst)[-3:]
+
+
 def test_gpu_scf_variants_agree(gpu, water_sp):
     from segfault_py_amd import hf
     e = [f(water_sp, tol=1e-10).energy for f in (hf.scf, hf.scf_damping, hf.scf_diis)]
