@@ -574,8 +574,22 @@ static int create_basis(int nmol, const int* mol_shell, int nshell, const int* L
             b->atom.push_back(shell_atom ? shell_atom[s] : 0);
             b->sh_mol.push_back(m);
             b->sh_loc.push_back(s - mol_shell[m]);
-            for (int k = 0; k < 3; ++k) b->xyz.push_back(center[3 * s + k]);
+            for (int k = 0; k < 3; ++k) {
+                if (!std::isfinite(center[3 * s + k])) {
+                    set_error(kEInval, "shell %d: centre is not finite", s);
+                    delete b;
+                    return kEInval;
+                }
+                b->xyz.push_back(center[3 * s + k]);
+            }
             for (int i = 0; i < nprim[s]; ++i) {
+                if (!(expns[pst + i] > 0.0) || !std::isfinite(expns[pst + i]) ||
+                    !std::isfinite(coefs[pst + i])) {
+                    set_error(kEInval, "shell %d: primitive %d has exponent %g, coefficient %g", s,
+                              i, expns[pst + i], coefs[pst + i]);
+                    delete b;
+                    return kEInval;
+                }
                 b->expn.push_back(expns[pst + i]);
                 b->coef.push_back(coefs[pst + i]);
             }
@@ -1839,6 +1853,14 @@ extern "C" int sf_basis_set_nuclei(sf_basis* b, int natom, const double* Z, cons
     if (!b || natom < 0 || (natom > 0 && (!Z || !xyz))) {
         set_error(kEInval, "sf_basis_set_nuclei: bad arguments");
         return kEInval;
+    }
+    for (int a = 0; a < natom; ++a) {
+        if (!(Z[a] >= 0.0) || !std::isfinite(Z[a]) || !std::isfinite(xyz[3 * a]) ||
+            !std::isfinite(xyz[3 * a + 1]) || !std::isfinite(xyz[3 * a + 2])) {
+            set_error(kEInval, "sf_basis_set_nuclei: nucleus %d has charge %g at (%g, %g, %g)", a,
+                      Z[a], xyz[3 * a], xyz[3 * a + 1], xyz[3 * a + 2]);
+            return kEInval;
+        }
     }
     b->nnuc = natom;
     SF_TRY(b->d_Z.upload(std::vector<double>(Z, Z + natom)));

@@ -47,8 +47,29 @@ class SCFResult:
         return 2.0 * (Co * self.eps[:self.nocc]) @ Co.T
 
 
+def _diis_extrapolate(Fs, res):
+    """Pulay extrapolation over the stored Fock matrices (hf.py:324-346):
+    B_ij = <r_i, r_j> from one GEMM over the stacked residuals, bordered by
+    -1, solved with solve_ex (no exception, so no host synchronisation);
+    returns (F_diis, ok) with ok false where the reference's solve raises
+    (exactly singular system) or the coefficients are not finite."""
+    import torch
+
+    n = len(Fs)
+    R = torch.stack(res).reshape(n, -1)
+    Bm = torch.full((n + 1, n + 1), -1.0, dtype=R.dtype, device=R.device)
+    Bm[:n, :n] = R @ R.T
+    Bm[n, n] = 0.0
+    rhs = torch.zeros(n + 1, dtype=R.dtype, device=R.device)
+    rhs[n] = -1.0
+    coef, info = torch.linalg.solve_ex(Bm, rhs)
+    ok = (info == 0) & torch.isfinite(coef).all()
+    F = torch.tensordot(coef[:n], torch.stack(Fs), dims=1)
+    return F, ok
+
+
 def _run(mol, mode: str, tol: float, maxiter: int, tau: float, precision: int, tau_mixed: float,
-         basis=None, verbose: bool = False) -> SCFResult:
+         basis=None, verbose: bool = False, graph: bool = True) -> SCFResult:
     import torch
 
     t0 = time.perf_counter()
@@ -56,15 +77,26 @@ def _run(mol, mode: str, tol: float, maxiter: int, tau: float, precision: int, t
     S, T, V = b.one_electron()
     H = T + V
     vals, vecs = torch.linalg.eigh(S)
+    lam_min, lam_max = float(vals[0]), float(vals[-1])
+    if not lam_min > 1e-12 * lam_max:
+        # S^-1/2 of the reference (hf.py:278-279) would amplify rounding by
+        # 1/sqrt(lam_min): refuse instead of iterating on noise
+        raise ValueError(f"overlap matrix is not positive definite to working precision "
+                         f"(eigenvalues {lam_min:.3e} .. {lam_max:.3e}): linearly dependent "
+                         "basis functions")
     X = vecs @ torch.diag(1.0 / torch.sqrt(vals)) @ vecs.T
     nocc = int(sum(int(z) for z in mol.numbers)) // 2
+    on_gpu = S.is_cuda
     F = H.clone()
     Fs, res = [], []
     P_old = torch.zeros_like(S)
+    # the J/K build reads P from, and writes J and K to, fixed buffers, so
+    # every iteration replays one captured graph (sf_jk_graph)
+    P_buf, J_buf, K_buf = torch.zeros_like(S), torch.zeros_like(S), torch.zeros_like(S)
     etot_old = float("inf")
     counter = 0
     converged = False
-    jk_s = 0.0
+    events = []
     history = []
     while True:
         eps, Cp = torch.linalg.eigh(X @ F @ X)
@@ -76,12 +108,33 @@ def _run(mol, mode: str, tol: float, maxiter: int, tau: float, precision: int, t
             P_old = P_new
         else:
             P = P_new
-        tj = time.perf_counter()
-        J, K = b.jk(P, precision)
-        torch.cuda.synchronize()
-        jk_s += time.perf_counter() - tj
-        F = H + J - 0.5 * K
-        etot = float(0.5 * torch.sum(P * (H + F)))
+        P_buf.copy_(P)
+        if on_gpu:
+            ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+            ev[0].record()
+        if graph:
+            b.jk_graph(P_buf, (J_buf, K_buf), precision)
+        else:
+            b.jk(P_buf, precision, out=(J_buf, K_buf))
+        if on_gpu:
+            ev[1].record()
+            events.append(ev)
+        F = H + J_buf - 0.5 * K_buf
+        e_dev = 0.5 * torch.sum(P * (H + F))
+        # the DIIS step of this iteration (it runs unless this iteration
+        # converges) is queued before the one host read of the iteration
+        diis = mode == "diis" and counter + 1 > 2 and counter + 1 <= maxiter
+        if diis:
+            if len(Fs) > 6:
+                Fs.pop(0)
+                res.pop(0)
+            res.append(X @ (F @ P @ S - S @ P @ F) @ X)
+            Fs.append(F)
+            F_diis, ok = _diis_extrapolate(Fs, res)
+            host = torch.stack([e_dev, ok.to(e_dev.dtype)]).cpu()
+        else:
+            host = e_dev.reshape(1).cpu()
+        etot = float(host[0])
         history.append(etot + mol.e_nuc)
         if verbose:
             print(f"Iter {counter:>4}\tenergy difference {etot - etot_old:.10f}\t\t"
@@ -95,32 +148,15 @@ def _run(mol, mode: str, tol: float, maxiter: int, tau: float, precision: int, t
             if verbose:
                 print(f"SCF stopped after {maxiter} iterations without converging")
             break
-        if mode != "diis" or counter <= 2:
+        if not diis:
             continue
-        if len(Fs) > 6:
-            Fs.pop(0)
-            res.pop(0)
-        res.append(X @ (F @ P @ S - S @ P @ F) @ X)
-        Fs.append(F)
-        n = len(Fs)
-        Bm = torch.zeros((n + 1, n + 1), dtype=torch.float64, device=S.device)
-        Bm[-1, :] = -1.0
-        Bm[:, -1] = -1.0
-        Bm[-1, -1] = 0.0
-        for i in range(n):
-            for j in range(n):
-                Bm[i, j] += torch.sum(res[i] * res[j])
-        rhs = torch.zeros(n + 1, dtype=torch.float64, device=S.device)
-        rhs[-1] = -1.0
-        try:
-            coef = torch.linalg.solve(Bm, rhs)
-        except torch.linalg.LinAlgError:
-            if verbose:
-                print("singular Pulay system: keeping this iteration's Fock matrix")
-            continue
-        if not bool(torch.isfinite(coef).all()):
-            continue
-        F = sum(Fs[i] * coef[i] for i in range(n))
+        if bool(host[1]):
+            F = F_diis
+        elif verbose:
+            print("singular Pulay system: keeping this iteration's Fock matrix")
+    if on_gpu:
+        torch.cuda.synchronize()
+    jk_s = 1e-3 * sum(a.elapsed_time(z) for a, z in events)
     if verbose:
         print(f"SCF energy {etot + mol.e_nuc}")
     return SCFResult(energy=etot + mol.e_nuc, e_elec=etot, e_nuc=mol.e_nuc, converged=converged,
@@ -129,21 +165,21 @@ def _run(mol, mode: str, tol: float, maxiter: int, tau: float, precision: int, t
 
 
 def scf(mol, tol=1e-8, maxiter=30, *, tau=1e-12, precision=0, tau_mixed=1e-5, basis=None,
-        verbose=False) -> SCFResult:
+        verbose=False, graph=True) -> SCFResult:
     """Plain SCF iterations (hf.py:65-160)."""
-    return _run(mol, "plain", tol, maxiter, tau, precision, tau_mixed, basis, verbose)
+    return _run(mol, "plain", tol, maxiter, tau, precision, tau_mixed, basis, verbose, graph)
 
 
 def scf_damping(mol, tol=1e-8, maxiter=100, *, tau=1e-12, precision=0, tau_mixed=1e-5, basis=None,
-                verbose=False) -> SCFResult:
+                verbose=False, graph=True) -> SCFResult:
     """SCF with density damping 0.1 (hf.py:164-232)."""
-    return _run(mol, "damping", tol, maxiter, tau, precision, tau_mixed, basis, verbose)
+    return _run(mol, "damping", tol, maxiter, tau, precision, tau_mixed, basis, verbose, graph)
 
 
 def scf_diis(mol, tol=1e-8, maxiter=100, *, tau=1e-12, precision=0, tau_mixed=1e-5, basis=None,
-             verbose=False) -> SCFResult:
+             verbose=False, graph=True) -> SCFResult:
     """SCF with Pulay DIIS (hf.py:236-353)."""
-    return _run(mol, "diis", tol, maxiter, tau, precision, tau_mixed, basis, verbose)
+    return _run(mol, "diis", tol, maxiter, tau, precision, tau_mixed, basis, verbose, graph)
 
 
 @dataclass
@@ -192,6 +228,10 @@ def scf_batch(mols, tol=1e-8, maxiter=100, *, tau=1e-12, precision=0, tau_mixed=
     # S^-1/2 per molecule: identity on the padding keeps S invertible, the
     # mask removes it again
     lam, U = torch.linalg.eigh(S + torch.diag_embed(pad))
+    bad = lam[:, 0] <= 1e-12 * lam[:, -1]
+    if bool(bad.any()):
+        raise ValueError(f"molecules {torch.nonzero(bad).flatten().tolist()}: overlap matrix not "
+                         "positive definite to working precision (linearly dependent functions)")
     X = (U * (1.0 / torch.sqrt(lam))[:, None, :]) @ U.transpose(1, 2)
     X = (X * mask2).contiguous()
     shift = torch.diag_embed(pad * 1e6)                          # padded orbitals last
@@ -247,18 +287,16 @@ def scf_batch(mols, tol=1e-8, maxiter=100, *, tau=1e-12, precision=0, tau_mixed=
         errs.append(X @ (F @ P @ S - S @ P @ F) @ X)
         Fs.append(F)
         k = len(Fs)
-        Bm = torch.zeros((B, k + 1, k + 1), dtype=torch.float64, device=dev)
-        Bm[:, -1, :] = -1.0
-        Bm[:, :, -1] = -1.0
-        Bm[:, -1, -1] = 0.0
-        for i in range(k):
-            for j in range(k):
-                Bm[:, i, j] += torch.sum(errs[i] * errs[j], dim=(1, 2))
+        # B_ij = <e_i, e_j> per molecule: one batched GEMM over the stacked residuals
+        R = torch.stack(errs, dim=1).reshape(B, k, n * n)
+        Bm = torch.full((B, k + 1, k + 1), -1.0, dtype=torch.float64, device=dev)
+        Bm[:, :k, :k] = torch.bmm(R, R.transpose(1, 2).contiguous())
+        Bm[:, k, k] = 0.0
         rhs = torch.zeros((B, k + 1), dtype=torch.float64, device=dev)
         rhs[:, -1] = -1.0
         coef, info = torch.linalg.solve_ex(Bm, rhs)
         ok = (info == 0) & torch.isfinite(coef).all(dim=1) & (~done)
-        F_diis = sum(Fs[i] * coef[:, i, None, None] for i in range(k))
+        F_diis = torch.einsum("bk,bkij->bij", coef[:, :k], torch.stack(Fs, dim=1))
         F = torch.where(ok[:, None, None], F_diis, F)
     torch.cuda.synchronize()
     t2 = time.perf_counter()
