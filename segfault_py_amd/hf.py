@@ -47,25 +47,66 @@ class SCFResult:
         return 2.0 * (Co * self.eps[:self.nocc]) @ Co.T
 
 
-def _diis_extrapolate(Fs, res):
-    """Pulay extrapolation over the stored Fock matrices (hf.py:324-346):
-    B_ij = <r_i, r_j> from one GEMM over the stacked residuals, bordered by
-    -1, solved with solve_ex (no exception, so no host synchronisation);
-    returns (F_diis, ok) with ok false where the reference's solve raises
-    (exactly singular system) or the coefficients are not finite."""
-    import torch
+class _Pulay:
+    """The DIIS store of hf.py:324-346 -- at most 7 Fock matrices and their
+    commutator residuals, oldest first -- with the residuals' Gram matrix
+    B_ij = <r_i, r_j> kept up to date: a new residual adds one row, the
+    dots of every stored residual with it (one broadcast multiply and one
+    row sum; rocBLAS takes ~30 ms for the equivalent (7 x n^2)(n^2 x 7)
+    GEMM at n = 440).  Works on one (n, n) system or a batch (b, n, n)."""
 
-    n = len(Fs)
-    R = torch.stack(res).reshape(n, -1)
-    Bm = torch.full((n + 1, n + 1), -1.0, dtype=R.dtype, device=R.device)
-    Bm[:n, :n] = R @ R.T
-    Bm[n, n] = 0.0
-    rhs = torch.zeros(n + 1, dtype=R.dtype, device=R.device)
-    rhs[n] = -1.0
-    coef, info = torch.linalg.solve_ex(Bm, rhs)
-    ok = (info == 0) & torch.isfinite(coef).all()
-    F = torch.tensordot(coef[:n], torch.stack(Fs), dims=1)
-    return F, ok
+    def __init__(self, nmax: int = 7):
+        self.nmax = nmax
+        self.Fs, self.res = [], []
+        self.B = None
+
+    def push(self, F, r):
+        import torch
+
+        if len(self.Fs) >= self.nmax:
+            self.Fs.pop(0)
+            self.res.pop(0)
+            self.B = self.B[..., 1:, 1:]
+        self.Fs.append(F)
+        self.res.append(r)
+        k = len(self.res)
+        lead = r.shape[:-2]
+        R = torch.stack(self.res, dim=len(lead)).reshape(*lead, k, -1)
+        d = (R * r.reshape(*lead, 1, -1)).sum(dim=-1)               # (..., k)
+        B = torch.empty((*lead, k, k), dtype=r.dtype, device=r.device)
+        if k > 1:
+            B[..., :k - 1, :k - 1] = self.B
+        B[..., k - 1, :] = d
+        B[..., :, k - 1] = d
+        self.B = B
+
+    def extrapolate(self):
+        """(F_diis, ok): the Pulay combination from the bordered system,
+        solved with solve_ex (no exception, so no host synchronisation); ok
+        is false where the reference's solve raises (exactly singular) or
+        the coefficients are not finite."""
+        import torch
+
+        k = len(self.Fs)
+        lead = self.B.shape[:-2]
+        Bm = torch.full((*lead, k + 1, k + 1), -1.0, dtype=self.B.dtype, device=self.B.device)
+        Bm[..., :k, :k] = self.B
+        Bm[..., k, k] = 0.0
+        rhs = torch.zeros((*lead, k + 1), dtype=self.B.dtype, device=self.B.device)
+        rhs[..., k] = -1.0
+        coef, info = torch.linalg.solve_ex(Bm, rhs)
+        ok = (info == 0) & torch.isfinite(coef).all(dim=-1)
+        Fst = torch.stack(self.Fs, dim=len(lead))                     # (..., k, n, n)
+        F = (coef[..., :k, None, None] * Fst).sum(dim=len(lead))
+        return F, ok
+
+
+def _diis_extrapolate(Fs, res):
+    """Pulay extrapolation of the stored matrices (oldest first); see _Pulay."""
+    d = _Pulay(len(Fs))
+    for F, r in zip(Fs, res):
+        d.push(F, r)
+    return d.extrapolate()
 
 
 def _run(mol, mode: str, tol: float, maxiter: int, tau: float, precision: int, tau_mixed: float,
@@ -88,7 +129,7 @@ def _run(mol, mode: str, tol: float, maxiter: int, tau: float, precision: int, t
     nocc = int(sum(int(z) for z in mol.numbers)) // 2
     on_gpu = S.is_cuda
     F = H.clone()
-    Fs, res = [], []
+    pulay = _Pulay(7)
     P_old = torch.zeros_like(S)
     # the J/K build reads P from, and writes J and K to, fixed buffers, so
     # every iteration replays one captured graph (sf_jk_graph)
@@ -125,12 +166,8 @@ def _run(mol, mode: str, tol: float, maxiter: int, tau: float, precision: int, t
         # converges) is queued before the one host read of the iteration
         diis = mode == "diis" and counter + 1 > 2 and counter + 1 <= maxiter
         if diis:
-            if len(Fs) > 6:
-                Fs.pop(0)
-                res.pop(0)
-            res.append(X @ (F @ P @ S - S @ P @ F) @ X)
-            Fs.append(F)
-            F_diis, ok = _diis_extrapolate(Fs, res)
+            pulay.push(F, X @ (F @ P @ S - S @ P @ F) @ X)
+            F_diis, ok = pulay.extrapolate()
             host = torch.stack([e_dev, ok.to(e_dev.dtype)]).cpu()
         else:
             host = e_dev.reshape(1).cpu()
@@ -247,7 +284,7 @@ def scf_batch(mols, tol=1e-8, maxiter=100, *, tau=1e-12, precision=0, tau_mixed=
     E_old = torch.full((B,), float("inf"), dtype=torch.float64, device=dev)
     done = torch.zeros(B, dtype=torch.bool, device=dev)
     done_iter = torch.full((B,), -1, dtype=torch.long, device=dev)
-    Fs, errs = [], []
+    pulay = _Pulay(7)
     jk_s = 0.0
     counter = 0
     torch.cuda.synchronize()
@@ -281,22 +318,9 @@ def scf_batch(mols, tol=1e-8, maxiter=100, *, tau=1e-12, precision=0, tau_mixed=
             break
         if counter <= 2:
             continue
-        if len(Fs) > 6:
-            Fs.pop(0)
-            errs.pop(0)
-        errs.append(X @ (F @ P @ S - S @ P @ F) @ X)
-        Fs.append(F)
-        k = len(Fs)
-        # B_ij = <e_i, e_j> per molecule: one batched GEMM over the stacked residuals
-        R = torch.stack(errs, dim=1).reshape(B, k, n * n)
-        Bm = torch.full((B, k + 1, k + 1), -1.0, dtype=torch.float64, device=dev)
-        Bm[:, :k, :k] = torch.bmm(R, R.transpose(1, 2).contiguous())
-        Bm[:, k, k] = 0.0
-        rhs = torch.zeros((B, k + 1), dtype=torch.float64, device=dev)
-        rhs[:, -1] = -1.0
-        coef, info = torch.linalg.solve_ex(Bm, rhs)
-        ok = (info == 0) & torch.isfinite(coef).all(dim=1) & (~done)
-        F_diis = torch.einsum("bk,bkij->bij", coef[:, :k], torch.stack(Fs, dim=1))
+        pulay.push(F, X @ (F @ P @ S - S @ P @ F) @ X)
+        F_diis, ok = pulay.extrapolate()
+        ok = ok & (~done)
         F = torch.where(ok[:, None, None], F_diis, F)
     torch.cuda.synchronize()
     t2 = time.perf_counter()

@@ -97,8 +97,8 @@ def test_singular_pulay_system_falls_back(oracle, water_sp, monkeypatch):
     B = np.array([[9.0, 3.0, -1.0], [3.0, 3.0, -1.0], [-1.0, -1.0, 0.0]])
     c = np.linalg.solve(B, [0.0, 0.0, -1.0])
     assert torch.allclose(F, (2 * c[0] + 3 * c[1]) * Fs[0], rtol=1e-14, atol=0)
-    refused = lambda Fs, res: (Fs[-1], torch.tensor(False))  # noqa: E731
-    monkeypatch.setattr(hf, "_diis_extrapolate", refused)
+    refused = lambda self: (self.Fs[-1], torch.tensor(False))  # noqa: E731
+    monkeypatch.setattr(hf._Pulay, "extrapolate", refused)
     rd = hf.scf_diis(water_sp, tol=1e-10, maxiter=60, basis=OracleBasis(oracle, water_sp))
     rp = hf.scf(water_sp, tol=1e-10, maxiter=60, basis=OracleBasis(oracle, water_sp))
     assert rd.history == rp.history
@@ -129,3 +129,27 @@ def test_library_rejects_bad_primitives_and_nuclei():
                                np.zeros(1, dtype=np.int32), ctypes.byref(h))
         assert rc == 1 and not h.value, (ex, cf, cen)
     assert L.sf_basis_set_nuclei(None, 1, one, np.zeros(3)) == 1
+
+
+def test_batched_pulay_equals_per_system():
+    """The batched DIIS store (one (b, n, n) system per molecule) gives each
+    molecule the extrapolation of its own store, and the Gram matrix kept
+    row by row equals the full one, also after the oldest vectors drop."""
+    from segfault_py_amd import hf
+    g = torch.Generator().manual_seed(3)
+    b, n = 3, 5
+    batched = hf._Pulay(4)
+    single = [hf._Pulay(4) for _ in range(b)]
+    for _ in range(6):
+        F = torch.randn(b, n, n, generator=g, dtype=torch.float64)
+        r = torch.randn(b, n, n, generator=g, dtype=torch.float64)
+        batched.push(F, r)
+        for m in range(b):
+            single[m].push(F[m], r[m])
+    R = torch.stack(batched.res, dim=1).reshape(b, 4, -1)
+    assert torch.allclose(batched.B, R @ R.transpose(1, 2), rtol=1e-13, atol=1e-12)
+    Fb, okb = batched.extrapolate()
+    for m in range(b):
+        Fm, okm = single[m].extrapolate()
+        assert bool(okb[m]) and bool(okm)
+        assert torch.allclose(Fb[m], Fm, rtol=1e-12, atol=1e-12)
