@@ -3,9 +3,14 @@ container only; the GPU hosts receive the committed outputs).
 
 Writes:
   cc-pvdz-HCNO.dat      -- the H, C, N, O blocks of src/hf/data/basis/cc-pvdz.dat
-  molecules/*.xyz       -- the reference geometries of the H/C/N/O molecules
+  cc-pvdz-HS.dat        -- its H and S blocks (sulfur: the element whose header
+                           line the reference's parser misreads, molecule.py:128-133)
+  molecules/*.xyz       -- the reference geometries of the H/C/N/O molecules and
+                           of hydrogen_sulfide
   molecule_test.json    -- src/hf/benchmark/molecule_test.json (energies, wall
                            times) as published by the reference
+  psi4_energies.json    -- "@RHF Final Energy" of every Psi4 output the reference
+                           ships (src/hf/benchmark/psi4_output/*.out)
 """
 import json
 import os
@@ -18,23 +23,32 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 
 def main():
     text = open(os.path.join(REF, "data/basis/cc-pvdz.dat")).read()
-    keep = []
-    for block in text.split("****"):
-        b = block.strip()
-        if re.match(r"^(H|C|N|O)\s+0", b):
-            keep.append(b)
-    with open(os.path.join(HERE, "cc-pvdz-HCNO.dat"), "w") as fh:
-        fh.write("\n****\n".join(keep) + "\n****\n")
+    for out, elems in (("cc-pvdz-HCNO.dat", "H|C|N|O"), ("cc-pvdz-HS.dat", "H|S")):
+        keep = []
+        for block in text.split("****"):
+            b = block.strip()
+            if re.match(rf"^({elems})\s+0", b):
+                keep.append(b)
+        with open(os.path.join(HERE, out), "w") as fh:
+            fh.write("\n****\n".join(keep) + "\n****\n")
     os.makedirs(os.path.join(HERE, "molecules"), exist_ok=True)
     for name in sorted(os.listdir(os.path.join(REF, "data/molecules"))):
         src = os.path.join(REF, "data/molecules", name)
         lines = open(src).read().strip().split("\n")
         syms = {ln.split()[0] for ln in lines[2:] if ln.strip()}
-        if syms <= {"H", "C", "N", "O"}:
+        if syms <= {"H", "C", "N", "O"} or name == "hydrogen_sulfide.xyz":
             shutil.copy(src, os.path.join(HERE, "molecules", name))
     data = json.load(open(os.path.join(REF, "benchmark/molecule_test.json")))
     with open(os.path.join(HERE, "molecule_test.json"), "w") as fh:
         json.dump(data, fh, indent=1)
+    psi4 = {}
+    pdir = os.path.join(REF, "benchmark/psi4_output")
+    for name in sorted(os.listdir(pdir)):
+        m = re.search(r"@RHF Final Energy:\s+(-?[0-9.]+)", open(os.path.join(pdir, name)).read())
+        if m:
+            psi4[name[:-4]] = float(m.group(1))
+    with open(os.path.join(HERE, "psi4_energies.json"), "w") as fh:
+        json.dump(psi4, fh, indent=1)
 
 
 if __name__ == "__main__":

@@ -334,6 +334,18 @@ def test_gpu_scf_reproduces_every_published_energy(gpu):
     assert max(worst)[0] < 1e-9, sorted(worst)[-3:]
 
 
+def test_gpu_scf_h2s_matches_psi4(gpu):
+    """Sulfur (the element whose basis header the reference misparses):
+    GPU DIIS SCF of H2S in cc-pVDZ against the reference's Psi4 energy."""
+    from segfault_py_amd import hf
+    from segfault_py_amd.molecule import Molecule
+    mol = Molecule(os.path.join(GOLDEN, "molecules", "hydrogen_sulfide.xyz"),
+                   os.path.join(GOLDEN, "cc-pvdz-HS.dat"))
+    r = hf.scf_diis(mol, tol=1e-10)
+    psi4 = _golden("psi4_energies.json")["hydrogen_sulfide"]
+    assert r.converged and abs(r.energy - psi4) < 1e-8, (r.energy, psi4)
+
+
 def test_gpu_scf_variants_agree(gpu, water_sp):
     from segfault_py_amd import hf
     e = [f(water_sp, tol=1e-10).energy for f in (hf.scf, hf.scf_damping, hf.scf_diis)]
