@@ -56,11 +56,15 @@ def parse():
     ap.add_argument("--gradient-only", action="store_true",
                     help="only the (H2O)10 SCF + three FP64 nuclear gradients (for kernel tracing)")
     ap.add_argument("--builds-only", action="store_true",
-                    help="autotune, then only W + K FP64 J/K builds (for kernel tracing)")
+                    help="autotune, then only W + K J/K builds (for kernel tracing)")
+    ap.add_argument("--precision", type=int, default=0,
+                    help="precision of the --builds-only builds (0 FP64, 1 mixed)")
     ap.add_argument("--only-batch", action="store_true",
                     help="run only the batched SCF workload and print its JSON")
     ap.add_argument("--only-published", action="store_true",
                     help="run only the reference's published cc-pVDZ molecules")
+    ap.add_argument("--mixed-sweep", action="store_true",
+                    help="only sweep the FP32 quartets' primitive threshold (mixed builds)")
     ap.add_argument("--workload", type=int, default=1, help="workload index (1 = 20 waters)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (for HBM counter scaling)")
@@ -188,6 +192,9 @@ def main():
     if args.only_published:
         print(json.dumps({"published": published_workload()}))
         return
+    if args.mixed_sweep:
+        print(json.dumps({"mixed_sweep": mixed_sweep(args.steps)}))
+        return
     if args.gradient_only:
         print(json.dumps({"gradient_calls": gradient_calls(3)}))
         return
@@ -206,17 +213,18 @@ def main():
     if args.builds_only:
         # W warm-up and K timed FP64 builds, nothing else: each build starts
         # with the density-scaling kernel, which splits a kernel trace per build
-        basis.autotune(P, 0)
+        prec = args.precision
+        basis.autotune(P, prec)
         for _ in range(args.warmup):
-            basis.jk(P, 0, out=(J, K))
+            basis.jk(P, prec, out=(J, K))
         torch.cuda.synchronize()
         t = time.perf_counter()
         for _ in range(args.steps):
-            basis.jk(P, 0, out=(J, K))
+            basis.jk(P, prec, out=(J, K))
         torch.cuda.synchronize()
         ms = (time.perf_counter() - t) / args.steps * 1e3
-        print(json.dumps({"builds_only": {"workload": mol.nbf, "ms_per_build": ms,
-                                          "builds": args.steps}}))
+        print(json.dumps({"builds_only": {"workload": mol.nbf, "precision": prec,
+                                          "ms_per_build": ms, "builds": args.steps}}))
         return
     # flops count the primitive quartets the FP64 kernels evaluate
     work = metrics.class_work(basis, mol, evaluated=basis.prim_quartet_counts(0))
@@ -343,6 +351,41 @@ def main():
         line["cpu_baseline"] = cpu_baseline(engine.Basis(mol).screen(tau, tau_mixed), mol, work,
                                             args.cpu_seconds)
     print(json.dumps(line))
+
+
+def mixed_sweep(steps: int) -> dict:
+    """(H2O)20: mixed J/K builds with the FP32 quartets' primitive threshold
+    tau_prim swept, each against the FP64 build: time per build, and the
+    two-electron energy error at the converged FP64 SCF density."""
+    import torch
+    from segfault_py_amd import engine, hf, workloads as W
+
+    mol = W.config_molecule(1)
+    b = engine.Basis(mol).screen(1e-12, 1e-5)
+    r = hf.scf_diis(mol, basis=b, tol=1e-9)
+    P = r.P.contiguous()
+    J, K = torch.empty_like(P), torch.empty_like(P)
+
+    def build(prec):
+        b.set_tuning(b.autotune(P, prec)[0])
+        b.jk(P, prec, out=(J, K))
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(steps):
+            b.jk(P, prec, out=(J, K))
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / steps * 1e3, float(0.5 * (P * (J - 0.5 * K)).sum())
+
+    ms64, e64 = build(0)
+    out = {"workload": f"(H2O)20, N={mol.nbf}, tau_mixed 1e-5, density of the converged FP64 SCF",
+           "fp64_ms": ms64, "points": []}
+    for tp in (-1.0, 1e-12, 1e-11, 1e-10, 1e-9):
+        b.set_prim_tau(tp)
+        ms, e = build(1)
+        out["points"].append({"tau_prim_fp32": tp if tp >= 0 else 1e-13, "mixed_ms": ms,
+                              "speedup": ms64 / ms, "dE2": abs(e - e64)})
+    b.set_prim_tau(-1.0)
+    return out
 
 
 def published_workload() -> dict:

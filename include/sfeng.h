@@ -197,7 +197,7 @@ int sf_jk_graph(sf_basis *b, const double *P, double *J, double *K, int precisio
  * first 2^20 quartets of the segment) and keeps the fastest; for slot 2 it
  * also times the segment with FP64 arithmetic (same primitive threshold;
  * never less accurate) and keeps that where it is faster; a class with at
- * most 81 integrals per quartet whose FP32 segment runs in FP64 arithmetic
+ * most 54 integrals per quartet whose FP32 segment runs in FP64 arithmetic
  * then builds both segments in one launch (slot 1's shape), each quartet
  * with its own segment's primitive threshold.  sf_jk and
  * sf_jk_graph use the choice.  shapes[9c + 3slot + {0,1,2}] receives

@@ -113,9 +113,13 @@ struct QuartetShape {
     static constexpr int T_BC = T_AD + NA * ND;
     static constexpr int T_BD = T_BC + NB * NC;
     static constexpr int NTILE = T_BD + NB * ND;
-    // classes with more than 81 integrals per quartet run one wavefront per
-    // bra component pair (eri_bra_component) and digest in a second kernel
-    static constexpr bool SPLIT = NINT > 81;
+    // classes with more than 54 integrals per quartet run one wavefront per
+    // component of A or per bra component pair (eri_bra_row /
+    // eri_bra_component) and digest in a second kernel: measured on the
+    // 20-water build, (pp|pp) (81 integrals, 256 VGPRs + 164 AGPRs as one
+    // thread per quartet) is faster split, every class up to 54 integrals
+    // is faster unsplit (the split recomputes nodes per component)
+    static constexpr bool SPLIT = NINT > 54;
     // split classes whose NB*NC*ND integrals of one bra component of A fit
     // in registers run one wavefront per (64 quartets, component of A)
     // (eri_bra_row), the others one per (64 quartets, component pair)
@@ -724,7 +728,10 @@ SF_HD SF_INLINE void grad_bra_component(const PrimPair* __restrict__ bra, int nb
                                         const double* A, const double* B, const double* C,
                                         const double* D, const double* __restrict__ Dfac,
                                         double* __restrict__ g,
-                                        const NodeTable& nt = global_node_table<(QuartetShape<LA, LB, LC, LD>::L + 1) / 2 + 1>())
+                                        const NodeTable& nt = global_node_table<(QuartetShape<LA, LB, LC, LD>::L + 1) / 2 + 1>(),
+                                        const double* __restrict__ bq = nullptr,
+                                        const double* __restrict__ kq = nullptr,
+                                        double tau_prim = 0.0)
 {
     using S = QuartetShape<LA, LB, LC, LD>;
     constexpr int NR = (S::L + 1) / 2 + 1;
@@ -740,12 +747,20 @@ SF_HD SF_INLINE void grad_bra_component(const PrimPair* __restrict__ bra, int nb
 #pragma unroll
         for (int e = 1; e <= LD; ++e) CDp[k][e] = CDp[k][e - 1] * cd;
     }
+    // primitive screening as in eri_shell_quartet when bq/kq are given:
+    // both loops stop at the first primitive quartet with bq * kq < tau_prim
     for (int ik = 0; ik < nket; ++ik) {
+        double qk = 0.0;
+        if (kq) {
+            qk = kq[ik];
+            if (nbra == 0 || qk * bq[0] < tau_prim) break;
+        }
         const PrimPair pk = ket[ik];
         const double q = pk.p;
         const R tc = (R)(2.0 * (q - pk.eb));
         const double QC[3] = {pk.Px - C[0], pk.Py - C[1], pk.Pz - C[2]};
         for (int ib = 0; ib < nbra; ++ib) {
+            if (kq && bq[ib] * qk < tau_prim) break;
             const PrimPair pb = bra[ib];
             const double p = pb.p;
             const R ta = (R)(2.0 * (p - pb.eb)), tb = (R)(2.0 * pb.eb);
@@ -822,7 +837,8 @@ SF_HD SF_INLINE void grad_all_a(const PrimPair* __restrict__ bra, int nbra,
                                 const PrimPair* __restrict__ ket, int nket, const double* A,
                                 const double* B, const double* C, const double* D,
                                 const double* __restrict__ Dfac, double* __restrict__ g,
-                                const NodeTable& nt)
+                                const NodeTable& nt, const double* __restrict__ bq = nullptr,
+                                const double* __restrict__ kq = nullptr, double tau_prim = 0.0)
 {
     static_assert(LB == 0, "grad_all_a: shell B must be an s shell");
     using S = QuartetShape<LA, LB, LC, LD>;
@@ -838,12 +854,20 @@ SF_HD SF_INLINE void grad_all_a(const PrimPair* __restrict__ bra, int nbra,
 #pragma unroll
         for (int e = 1; e <= LD; ++e) CDp[k][e] = CDp[k][e - 1] * cd;
     }
+    // primitive screening as in eri_shell_quartet when bq/kq are given:
+    // both loops stop at the first primitive quartet with bq * kq < tau_prim
     for (int ik = 0; ik < nket; ++ik) {
+        double qk = 0.0;
+        if (kq) {
+            qk = kq[ik];
+            if (nbra == 0 || qk * bq[0] < tau_prim) break;
+        }
         const PrimPair pk = ket[ik];
         const double q = pk.p;
         const R tc = (R)(2.0 * (q - pk.eb));
         const double QC[3] = {pk.Px - C[0], pk.Py - C[1], pk.Pz - C[2]};
         for (int ib = 0; ib < nbra; ++ib) {
+            if (kq && bq[ib] * qk < tau_prim) break;
             const PrimPair pb = bra[ib];
             const double p = pb.p;
             const R ta = (R)(2.0 * (p - pb.eb)), tb = (R)(2.0 * pb.eb);

@@ -177,12 +177,27 @@ __device__ __forceinline__ void class_quartet(const ClassParams& cp, const NodeT
     }
 }
 
+// Minimum wavefronts per SIMD the register allocator must leave room for,
+// per class (1 = no constraint).  As one thread per quartet (ds|pp) and
+// (dp|ps) (54 integrals per quartet) need more than 256 registers per lane,
+// i.e. one wavefront per SIMD; held to two they spill ~64 registers to
+// scratch and still run faster (20-water build: 0.305 -> 0.268 ms and
+// 0.226 -> 0.189 ms), as (ds|ps) held to three (175 -> 168 registers,
+// 0.405 -> 0.380 ms).  The same for (ps|ps), (pp|ss) (held to four) and
+// (pp|ps) (three) measured slower; they keep the allocator's choice.
+template <int LA, int LB, int LC, int LD>
+struct ClassWaves {
+    static constexpr int code = LA * 1000 + LB * 100 + LC * 10 + LD;
+    static constexpr int min = (code == 2110 || code == 2011) ? 2 : (code == 2010) ? 3 : 1;
+};
+
 // One thread per shell quartet; the blocks stride over the segment, so a
 // capped grid (Tune::grid) makes them persistent.  Consecutive quartets walk
 // the kets of one bra, so most wavefronts have a single bra: those take the
 // wave-uniform path.
 template <int LA, int LB, int LC, int LD, typename R, int MODE>
-__global__ __launch_bounds__(256) void k_eri_class(const ClassParams cp)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ClassWaves<LA, LB, LC, LD>::min)))
+void k_eri_class(const ClassParams cp)
 {
     using S = QuartetShape<LA, LB, LC, LD>;
     __shared__ double s_tab[node_table_lds(S::NR) > 0 ? node_table_lds(S::NR) : 1];
@@ -408,7 +423,9 @@ __global__ __launch_bounds__(256) void k_grad_split(const ClassParams cp, double
                                Pa[qm.col[3] + fd] * Pb[qm.col[2] + fc]);
         }
         grad_all_a<LA, LB, LC, LD, R>(cp.prim + cp.bra.poff[i], cp.bra.np[i], cp.prim + cp.ket.poff[j],
-                                      cp.ket.np[j], A, B, C, D, Df, g, nt);
+                                      cp.ket.np[j], A, B, C, D, Df, g, nt,
+                                      cp.primq + cp.bra.poff[i], cp.primq + cp.ket.poff[j],
+                                      cp.tau_prim);
     } else {
     static_switch<0, NS>(s, [&](auto sc) {
         constexpr int FA = decltype(sc)::value / S::NB, FB = decltype(sc)::value % S::NB;
@@ -426,7 +443,9 @@ __global__ __launch_bounds__(256) void k_grad_split(const ClassParams cp, double
                            Pa[qm.col[3] + fd] * Pb[qm.col[2] + fc]);
         grad_bra_component<LA, LB, LC, LD, FA, FB, R>(cp.prim + cp.bra.poff[i], cp.bra.np[i],
                                                       cp.prim + cp.ket.poff[j], cp.ket.np[j], A,
-                                                      B, C, D, Df, g, nt);
+                                                      B, C, D, Df, g, nt,
+                                                      cp.primq + cp.bra.poff[i],
+                                                      cp.primq + cp.ket.poff[j], cp.tau_prim);
     });
     }
 #pragma unroll

@@ -1787,7 +1787,7 @@ extern "C" int sf_autotune(sf_basis* b, const double* P, int precision, void* st
             seg_ms[seg] = best_ms;
             if (ms) ms[3 * c + slot] = best_ms;
         }
-        // classes with at most 81 integrals per quartet: FP64 arithmetic in
+        // classes with at most 54 integrals per quartet: FP64 arithmetic in
         // the FP32 segment means one launch over both segments (run_jk), so
         // the choice is between that launch and two with FP32 arithmetic
         if (!rc && precision == 1 && b->engine[c] == 0 && !class_split(c) && Q.nq64 > 0 &&

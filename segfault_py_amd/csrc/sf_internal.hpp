@@ -68,7 +68,7 @@ struct Tune {
     int grid = 0;
     // FP32 segment of a mixed build only: 1 = run its quartets with FP64
     // arithmetic (same primitive threshold), where that is the faster one;
-    // classes with at most 81 integrals per quartet then run both segments
+    // classes with at most 54 integrals per quartet then run both segments
     // as one launch (ClassParams::n64)
     int arith64 = 0;
 };

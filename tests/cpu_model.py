@@ -85,8 +85,8 @@ def shell_quartet(shells, a, b, c, d, fp32=False, split=False, pairs=False):
 
 def uses_split(la, lb, lc, ld):
     """Classes the GPU computes one bra component pair per wavefront (more
-    than 81 integrals per quartet); the rule of QuartetShape::SPLIT."""
-    return ncart(la) * ncart(lb) * ncart(lc) * ncart(ld) > 81
+    than 54 integrals per quartet); the rule of QuartetShape::SPLIT."""
+    return ncart(la) * ncart(lb) * ncart(lc) * ncart(ld) > 54
 
 
 def canonical_pairs(shells):

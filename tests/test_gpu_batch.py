@@ -66,7 +66,7 @@ def test_batch_jk_bitwise_equal_single(batch, precision):
 
 def test_batch_mixed_one_launch_bitwise_equal_single(batch):
     """Mixed builds with every FP32 segment in FP64 arithmetic (one launch
-    over both segments for the classes with at most 81 integrals per
+    over both segments for the classes with at most 54 integrals per
     quartet, thresholds chosen per quartet from each bra's n64 within its
     molecule): the batch equals each molecule's own handle bit for bit."""
     mols, bb, singles, P, Ps = batch

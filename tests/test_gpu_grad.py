@@ -89,3 +89,26 @@ def test_cluster_gradient_fp64_and_mixed(gpu):
     fdm = _fd(mol, [(0, 0), (17, 0)], energy=lambda m: _energy(m, tol=1e-10, precision=1))
     errm = max(abs(gmx[a, k] - v) for (a, k), v in fdm.items())
     assert errm < 1e-5, errm
+
+
+def test_gradient_primitive_screening(gpu):
+    """The derivative kernels skip primitive quartets below their segment's
+    threshold (FP64 quartets: 1e-3 tau; FP32 quartets of a mixed gradient:
+    1e-8 tau_mixed), as the J/K kernels do: against the same gradients with
+    primitive screening off, ten waters with the s/p/d basis."""
+    from segfault_py_amd import engine, hf, workloads as W
+    mol = W.config_molecule(3)
+    b = engine.Basis(mol).screen(1e-12, 1e-5)
+    r = hf.scf_diis(mol, basis=b, tol=1e-10)
+    Wd = r.energy_weighted_density()
+    g = b.gradient(r.P, Wd, 0)
+    gm = b.gradient(r.P, Wd, 1)
+    b.set_prim_tau_fp64(0.0)
+    b.set_prim_tau(0.0)
+    g0 = b.gradient(r.P, Wd, 0)
+    gm0 = b.gradient(r.P, Wd, 1)
+    b.set_prim_tau_fp64(-1.0)
+    b.set_prim_tau(-1.0)
+    assert np.max(np.abs(g - g0)) < 1e-10
+    assert np.max(np.abs(gm - gm0)) < 1e-8
+    assert np.max(np.abs(gm - g)) < 1e-8
