@@ -332,6 +332,7 @@ def main():
         Aw = torch.eye(mol.nbf, dtype=torch.float64, device="cuda")
         torch.linalg.eigh(Aw)
         torch.linalg.solve(Aw[:8, :8], Aw[:8, 0])
+        torch.linalg.solve_ex(Aw[:8, :8], Aw[:8, 0])
         torch.cuda.synchronize()
         r64 = hf.scf_diis(mol, basis=basis, precision=0)
         rmx = hf.scf_diis(mol, basis=basis, precision=1)

@@ -223,7 +223,11 @@ int sf_set_engine(sf_basis *b, int qcls, int engine);
  * W = 2 sum_occ e_i C_i C_i^T (device buffers, N x N).  Derivative ERIs
  * over the screened quartet list of sf_screen, deterministic reductions.
  * precision 0 = all FP64; 1 = derivative quartets below tau_mixed in FP32
- * (nodes, weights and every sum in FP64), as sf_jk.
+ * (nodes, weights and every sum in FP64), as sf_jk, or -- per class, where
+ * the first mixed call after sf_screen measures it faster -- in FP64
+ * arithmetic within one launch over the class.  Primitive quartets below
+ * the segment's threshold are skipped as in sf_jk (sf_set_prim_tau_fp64,
+ * sf_set_prim_tau).
  * No counterpart in the reference (single-point energies only); nuclear
  * repulsion is not included.  Requires sf_basis_set_nuclei for every atom. */
 int sf_grad(sf_basis *b, const double *P, const double *W, double *grad, int precision,

@@ -402,6 +402,10 @@ __global__ __launch_bounds__(256) void k_grad_split(const ClassParams cp, double
     const QuartetMap qm = quartet_map(cp.map, a, b, c, d);
     const double* P = cp.P;
     const long long n = qm.ld;
+    // the segment's primitive threshold; in one launch over both mixed
+    // segments, a bra's kets past its n64 FP64 ones take the FP32 one
+    double tau = cp.tau_prim;
+    if (cp.n64 && j - cp.ket0[i] >= cp.n64[i]) tau = cp.tau_prim_lo;
     double g[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) g[k] = 0.0;
@@ -424,8 +428,7 @@ __global__ __launch_bounds__(256) void k_grad_split(const ClassParams cp, double
         }
         grad_all_a<LA, LB, LC, LD, R>(cp.prim + cp.bra.poff[i], cp.bra.np[i], cp.prim + cp.ket.poff[j],
                                       cp.ket.np[j], A, B, C, D, Df, g, nt,
-                                      cp.primq + cp.bra.poff[i], cp.primq + cp.ket.poff[j],
-                                      cp.tau_prim);
+                                      cp.primq + cp.bra.poff[i], cp.primq + cp.ket.poff[j], tau);
     } else {
     static_switch<0, NS>(s, [&](auto sc) {
         constexpr int FA = decltype(sc)::value / S::NB, FB = decltype(sc)::value % S::NB;
@@ -445,7 +448,7 @@ __global__ __launch_bounds__(256) void k_grad_split(const ClassParams cp, double
                                                       cp.prim + cp.ket.poff[j], cp.ket.np[j], A,
                                                       B, C, D, Df, g, nt,
                                                       cp.primq + cp.bra.poff[i],
-                                                      cp.primq + cp.ket.poff[j], cp.tau_prim);
+                                                      cp.primq + cp.ket.poff[j], tau);
     });
     }
 #pragma unroll

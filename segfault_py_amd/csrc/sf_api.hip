@@ -201,6 +201,11 @@ struct sf_basis {
     DevBuf d_Z, d_nuc, d_atom;
     // nuclear gradient scratch (built on first use after each sf_screen)
     bool grad_ready = false;
+    // how a class's FP32 derivative quartets run in mixed gradients: -1 =
+    // not yet measured, 0 = their own launch in FP32 arithmetic, 1 = one
+    // launch over the class in FP64 arithmetic (chosen by timing both on
+    // the first mixed sf_grad after each sf_screen)
+    int grad_arith[kNumQuartetClass];
     std::vector<long long> gpart_bra, gpart_ket;  // per class offsets into d_gpart
     DevBuf d_gpart, d_gout, d_gcsr_off, d_gcsr_idx, d_grows, d_Ws, d_g2, d_gtot;
     DevBuf d_gonee;  // one-electron gradient: per (pair, nucleus) d/dA, d/dB partials
@@ -928,6 +933,7 @@ extern "C" int sf_screen(sf_basis* b, double tau, double tau_mixed)
     b->drop_graph();
     b->screened = false;
     b->grad_ready = false;
+    for (int c = 0; c < kNumQuartetClass; ++c) b->grad_arith[c] = -1;
     b->tau = tau;
     b->tau_mixed = tau_mixed;
     b->tau_prim = b->tau_prim_set >= 0.0 ? b->tau_prim_set : 1e-8 * tau_mixed;
@@ -1979,10 +1985,54 @@ extern "C" int sf_grad(sf_basis* b, const double* P, const double* W, double* gr
         if (Q.nq_all == 0) continue;
         ClassParams cp = class_params(b, c, b->d_Ps.as<double>());
         // every segment writes its quartets' partials at their class-wide
-        // index, so the reductions below see one array either way
-        for (int seg = 0; seg <= precision; ++seg) {
-            set_segment(b, c, precision, seg, cp, b->d_zero.as<int>());
-            SF_TRY(launch_grad_class(c, seg, cp, b->d_gout.as<double>(), s));
+        // index, so the reductions below see one array either way.  Mixed
+        // gradients have two ways to evaluate a class with FP32 quartets:
+        // its two segments as two launches (the FP32 one in FP32
+        // arithmetic), or one launch over the class in FP64 arithmetic
+        // with each quartet's own segment threshold (never less accurate;
+        // on this GPU, without packed FP32, it can be the faster one).  The
+        // first mixed sf_grad after sf_screen times both and keeps the
+        // faster, running it last so its partials are this call's.
+        auto two_launches = [&]() -> int {
+            cp.n64 = nullptr;
+            for (int seg = 0; seg <= precision; ++seg) {
+                set_segment(b, c, precision, seg, cp, b->d_zero.as<int>());
+                SF_TRY(launch_grad_class(c, seg, cp, b->d_gout.as<double>(), s));
+            }
+            return kOk;
+        };
+        auto one_launch = [&]() -> int {
+            set_segment(b, c, 0, 0, cp, b->d_zero.as<int>());
+            cp.tau_prim = b->tau_prim64;
+            cp.n64 = Q.d_k032.as<int>();
+            cp.tau_prim_lo = b->tau_prim;
+            return launch_grad_class(c, 0, cp, b->d_gout.as<double>(), s);
+        };
+        if (precision == 1 && Q.nq32 > 0) {
+            if (b->grad_arith[c] < 0) {
+                float ms[2] = {0.f, 0.f};
+                for (int v = 0; v < 2; ++v) {
+                    hipEvent_t e0, e1;
+                    SF_HIP(hipEventCreate(&e0));
+                    SF_HIP(hipEventCreate(&e1));
+                    SF_HIP(hipEventRecord(e0, s));
+                    const int rc = v == 0 ? two_launches() : one_launch();
+                    SF_HIP(hipEventRecord(e1, s));
+                    SF_HIP(hipEventSynchronize(e1));
+                    SF_HIP(hipEventElapsedTime(&ms[v], e0, e1));
+                    (void)hipEventDestroy(e0);
+                    (void)hipEventDestroy(e1);
+                    SF_TRY(rc);
+                }
+                b->grad_arith[c] = ms[1] < ms[0] ? 1 : 0;
+                if (b->grad_arith[c] == 0) SF_TRY(two_launches());
+            } else if (b->grad_arith[c] == 1) {
+                SF_TRY(one_launch());
+            } else {
+                SF_TRY(two_launches());
+            }
+        } else {
+            SF_TRY(two_launches());
         }
         SF_TRY(launch_grad_reduce_bra(b->d_gout.as<double>(), class_nslice(c), Q.nq_all,
                                       Q.d_base_all.as<long long>(), Q.nbra,
