@@ -175,8 +175,11 @@ int sf_jk(sf_basis *b, const double *P, double *J, double *K, int precision, voi
 /* sf_jk with HIP events between launch groups on `stream` (for measurement;
  * no counterpart in the reference, whose only timer is utils.timeit,
  * src/hf/utils.py:76-83).  ms[44] receives milliseconds: ms[2c] / ms[2c+1]
- * = FP64 / FP32 ERI+digestion kernel of quartet class c, ms[42] = density
- * scaling, ms[43] = deterministic plan reduction and finalisation. */
+ * = FP64 / FP32 ERI+digestion kernel of quartet class c (for a class with
+ * more than 54 integrals per quartet in a mixed build, whose two segments
+ * share one digestion launch: its integrals / the FP32 segment's integrals
+ * plus that digestion), ms[42] = density scaling, ms[43] = deterministic
+ * plan reduction and finalisation. */
 int sf_jk_profile(sf_basis *b, const double *P, double *J, double *K, int precision,
                   void *stream, double *ms);
 /* sf_jk through a captured hipGraph: the first call (and the first after
@@ -196,10 +199,10 @@ int sf_jk_graph(sf_basis *b, const double *P, double *J, double *K, int precisio
  * current screened lists (density P, device; each candidate timed on the
  * first 2^20 quartets of the segment) and keeps the fastest; for slot 2 it
  * also times the segment with FP64 arithmetic (same primitive threshold;
- * never less accurate) and keeps that where it is faster; a class with at
- * most 54 integrals per quartet whose FP32 segment runs in FP64 arithmetic
- * then builds both segments in one launch (slot 1's shape), each quartet
- * with its own segment's primitive threshold.  sf_jk and
+ * never less accurate) and keeps that where it is faster; a class whose
+ * FP32 segment runs in FP64 arithmetic then builds both segments in one
+ * launch (slot 1's shape), each quartet with its own segment's primitive
+ * threshold.  sf_jk and
  * sf_jk_graph use the choice.  shapes[9c + 3slot + {0,1,2}] receives
  * (block, grid; 0 = one per quartet, arith: 1 = FP64 arithmetic in slot 2)
  * for every slot, ms[3c + slot] the chosen time of each tuned slot (others

@@ -287,18 +287,22 @@ __global__ __launch_bounds__(256) void k_eri_split(const ClassParams cp, double*
     R acc[NOUT];
 #pragma unroll
     for (int k = 0; k < NOUT; ++k) acc[k] = (R)0;
+    // the segment's primitive threshold; in one launch over both mixed
+    // segments, a bra's kets past its n64 FP64 ones take the FP32 one
+    double tau = cp.tau_prim;
+    if (cp.n64 && j - cp.ket0[i] >= cp.n64[i]) tau = cp.tau_prim_lo;
     static_switch<0, NS>(s, [&](auto sc) {
         constexpr bool PS = true;
         if constexpr (S::SPLIT_ROWS) {
             constexpr int FA = decltype(sc)::value;
             eri_bra_row<LA, LB, LC, LD, FA, R, PS>(
                 cp.prim + cp.bra.poff[i], cp.bra.np[i], cp.prim + cp.ket.poff[j], cp.ket.np[j], A, B,
-                C, D, acc, nt, cp.primq + cp.bra.poff[i], cp.primq + cp.ket.poff[j], cp.tau_prim);
+                C, D, acc, nt, cp.primq + cp.bra.poff[i], cp.primq + cp.ket.poff[j], tau);
         } else {
             constexpr int FA = decltype(sc)::value / S::NB, FB = decltype(sc)::value % S::NB;
             eri_bra_component<LA, LB, LC, LD, FA, FB, R, PS>(
                 cp.prim + cp.bra.poff[i], cp.bra.np[i], cp.prim + cp.ket.poff[j], cp.ket.np[j], A, B,
-                C, D, acc, nt, cp.primq + cp.bra.poff[i], cp.primq + cp.ket.poff[j], cp.tau_prim);
+                C, D, acc, nt, cp.primq + cp.bra.poff[i], cp.primq + cp.ket.poff[j], tau);
         }
     });
     // raw plane of integral (fa fb | fc fd): ((fa*NB + fb)*NK + fc*ND + fd);
@@ -517,14 +521,16 @@ static int launch_split(int fp32, int mode, const ClassParams& cp, const Tune& t
         set_error(kEState, "split class without a raw buffer");
         return kEState;
     }
-    if (fp32 && mode != kModeRaw)
-        hipLaunchKernelGGL((k_eri_split<LA, LB, LC, LD, float>), dim3(nb1), dim3(tn.block), 0, s,
-                           cp, raw);
-    else
-        hipLaunchKernelGGL((k_eri_split<LA, LB, LC, LD, double>), dim3(nb1), dim3(tn.block), 0, s,
-                           cp, raw);
-    SF_HIP(hipGetLastError());
-    if (mode == kModeJK) {
+    if (mode != kModeDigestOnly) {
+        if (fp32 && mode != kModeRaw)
+            hipLaunchKernelGGL((k_eri_split<LA, LB, LC, LD, float>), dim3(nb1), dim3(tn.block), 0,
+                               s, cp, raw);
+        else
+            hipLaunchKernelGGL((k_eri_split<LA, LB, LC, LD, double>), dim3(nb1), dim3(tn.block), 0,
+                               s, cp, raw);
+        SF_HIP(hipGetLastError());
+    }
+    if (mode == kModeJK || mode == kModeDigestOnly) {
         hipLaunchKernelGGL((k_digest_split<LA, LB, LC, LD>), dim3(nb2), dim3(tn.block), 0, s, cp,
                            (const double*)raw, rows);
         SF_HIP(hipGetLastError());
@@ -540,6 +546,10 @@ static int launch_class(int fp32, int mode, const ClassParams& cp, const Tune& t
     if constexpr (QuartetShape<LA, LB, LC, LD>::SPLIT) {
         return launch_split<LA, LB, LC, LD>(fp32, mode, cp, tn, s);
     } else {
+        if (mode == kModeEriOnly || mode == kModeDigestOnly) {
+            set_error(kEInval, "separate integral and digestion launches are for split classes");
+            return kEInval;
+        }
         unsigned nb = 0;
         SF_TRY_K(grid_blocks(cp.nq, tn.block, tn, &nb));
         dim3 grid(nb), block(tn.block);

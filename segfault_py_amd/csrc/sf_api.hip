@@ -1575,7 +1575,7 @@ static int run_jk(sf_basis* b, const double* P, double* J, double* K, int precis
         QuartetClassData& Q = b->qc[c];
         if (Q.nq_all == 0) continue;
         ClassParams cp = class_params(b, c, b->d_Ps.as<double>());
-        if (precision == 1 && b->engine[c] == 0 && !class_split(c) && b->tune[c][2].arith64) {
+        if (precision == 1 && b->engine[c] == 0 && b->tune[c][2].arith64) {
             // both segments in FP64 arithmetic: one launch over every quartet
             // of the class, each bra's FP32-segment kets with that segment's
             // primitive threshold (the same integrals as two launches)
@@ -1586,6 +1586,24 @@ static int run_jk(sf_basis* b, const double* P, double* J, double* K, int precis
             SF_TRY(launch_class_engine(b, c, 0, kModeJK, cp, b->tune[c][1], s));
             SF_TRY(mark());
             slot.push_back(2 * c);
+            continue;
+        }
+        if (precision == 1 && b->engine[c] == 0 && class_split(c)) {
+            // split class, mixed build: the two segments write their raw
+            // planes at class-wide quartet indices, then one digestion
+            // launch over the class (as in an FP64 build) reads them all
+            for (int seg = 0; seg <= 1; ++seg) {
+                set_segment(b, c, precision, seg, cp, b->d_zero.as<int>());
+                SF_TRY(launch_class_engine(b, c, seg, kModeEriOnly, cp, b->tune[c][1 + seg], s));
+                if (seg == 0) {
+                    SF_TRY(mark());
+                    slot.push_back(2 * c);
+                }
+            }
+            set_segment(b, c, 0, 0, cp, b->d_zero.as<int>());
+            SF_TRY(launch_class_engine(b, c, 0, kModeDigestOnly, cp, b->tune[c][0], s));
+            SF_TRY(mark());
+            slot.push_back(2 * c + 1);
             continue;
         }
         for (int seg = 0; seg <= precision; ++seg) {
@@ -1793,11 +1811,10 @@ extern "C" int sf_autotune(sf_basis* b, const double* P, int precision, void* st
             seg_ms[seg] = best_ms;
             if (ms) ms[3 * c + slot] = best_ms;
         }
-        // classes with at most 54 integrals per quartet: FP64 arithmetic in
-        // the FP32 segment means one launch over both segments (run_jk), so
-        // the choice is between that launch and two with FP32 arithmetic
-        if (!rc && precision == 1 && b->engine[c] == 0 && !class_split(c) && Q.nq64 > 0 &&
-            Q.nq32 > 0) {
+        // FP64 arithmetic in the FP32 segment means one launch over both
+        // segments (run_jk), so the choice is between that launch and two
+        // with FP32 arithmetic
+        if (!rc && precision == 1 && b->engine[c] == 0 && Q.nq64 > 0 && Q.nq32 > 0) {
             ClassParams mcp = class_params(b, c, b->d_Ps.as<double>());
             set_segment(b, c, 0, 0, mcp, b->d_zero.as<int>());
             mcp.tau_prim = b->tau_prim64;

@@ -68,8 +68,7 @@ struct Tune {
     int grid = 0;
     // FP32 segment of a mixed build only: 1 = run its quartets with FP64
     // arithmetic (same primitive threshold), where that is the faster one;
-    // classes with at most 54 integrals per quartet then run both segments
-    // as one launch (ClassParams::n64)
+    // the class then runs both segments as one launch (ClassParams::n64)
     int arith64 = 0;
 };
 
@@ -97,7 +96,12 @@ struct ClassParams {
     double* tiles;             // kModeJK: destination-sorted digestion tiles
 };
 
-enum EriMode : int { kModeJK = 0, kModeRaw = 1 };
+// kModeJK: integrals and digestion tiles; kModeRaw: raw integral planes
+// into ClassParams::out (dense export).  Split classes only:
+// kModeEriOnly writes the raw planes into ClassParams::raw without
+// digesting them, kModeDigestOnly digests ClassParams::raw (so the two
+// segments of a mixed build can share one digestion launch).
+enum EriMode : int { kModeJK = 0, kModeRaw = 1, kModeEriOnly = 2, kModeDigestOnly = 3 };
 
 // Where the plan-position kernel writes the slots of one quartet class.
 struct ClassTab {

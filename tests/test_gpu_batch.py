@@ -66,9 +66,9 @@ def test_batch_jk_bitwise_equal_single(batch, precision):
 
 def test_batch_mixed_one_launch_bitwise_equal_single(batch):
     """Mixed builds with every FP32 segment in FP64 arithmetic (one launch
-    over both segments for the classes with at most 54 integrals per
-    quartet, thresholds chosen per quartet from each bra's n64 within its
-    molecule): the batch equals each molecule's own handle bit for bit."""
+    over both segments of every class, thresholds chosen per quartet from
+    each bra's n64 within its molecule): the batch equals each molecule's
+    own handle bit for bit."""
     mols, bb, singles, P, Ps = batch
     sh = np.tile(np.array([256, 0, 0], dtype=np.int32), 63).reshape(21, 3, 3)
     sh[:, 2, 2] = 1

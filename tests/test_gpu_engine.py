@@ -212,9 +212,9 @@ def test_fp64_primitive_screening(gpu, oracle):
 
 
 def test_mixed_one_launch_equals_fp64_with_equal_thresholds(gpu):
-    """Mixed build with every FP32 segment in FP64 arithmetic (classes with
-    at most 54 integrals per quartet then run both segments as one launch,
-    with a threshold per quartet) and both primitive thresholds equal: the
+    """Mixed build with every FP32 segment in FP64 arithmetic (each class
+    then runs both segments as one launch, with a threshold per quartet)
+    and both primitive thresholds equal: the
     same integrals as the FP64 build, so J and K are equal bit for bit,
     eagerly and from a replayed graph."""
     from segfault_py_amd import engine, workloads as W
