@@ -277,6 +277,14 @@ def main():
                for w in work if w["shellq"] > 0}
     engines["build_ms"] = {"rys": round(float(sum(ms[:44])), 3), "mmd_mfma": round(float(sum(ms_mmd[:44])), 3)}
     achieved = dom["flops"] / (dom_ms * 1e-3) / 1e12
+    # every class against the same ceiling: its kernels' time in this build
+    # (integrals + digestion) and its algorithmic flops
+    class_roofline = {"eri_" + "".join(map(str, w["ls"])): {
+        "ms": round(float(ms[2 * w["class"]]), 4),
+        "shell_quartets": int(w["shellq"]), "prim_quartets": int(w["primq"]),
+        "tflops": round(w["flops"] / (ms[2 * w["class"]] * 1e-3) / 1e12, 2),
+        "frac": round(w["flops"] / (ms[2 * w["class"]] * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 3)}
+        for w in work if w["shellq"] > 0 and ms[2 * w["class"]] > 0}
     total_flops = sum(w["flops"] for w in work)
     shellq = sum(w["shellq"] for w in work)
     funcq = sum(w["funcq"] for w in work)
@@ -313,6 +321,7 @@ def main():
         "quartets_per_build": {"function": funcq, "shell": shellq, "primitive": primq},
         "build_tflops_fp64": total_flops / t64 / 1e12,
         "kernel_ms_fp64": kernel_ms,
+        "class_roofline_fp64": class_roofline,
         "engines_ms": engines,
         "setup_s": setup_s,
         "roofline": {"bound": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
