@@ -219,6 +219,14 @@ int sf_set_tuning(sf_basis *b, const int *shapes);
  * computes in FP64 in both precision segments.  Used by sf_jk, sf_jk_graph,
  * sf_autotune and sf_eri_dense. */
 int sf_set_engine(sf_basis *b, int qcls, int engine);
+/* Side streams of sf_jk / sf_jk_graph (0..3; default 1): the quartet-class
+ * launches, and then the per-tile-shape reductions, are spread over the
+ * caller's stream and nside private streams (forked from and joined back
+ * into the caller's stream with events), so small classes run beside large
+ * ones.  0 = every launch on the caller's stream, in class order.  Results
+ * are the same either way (every launch writes its own slots).  Launch
+ * scheduling only; no counterpart in the reference. */
+int sf_set_streams(sf_basis *b, int nside);
 
 /* Nuclear gradient of the RHF electronic energy (host output grad[3*natom],
  * hartree/bohr, atom order of the basis): sum P dH/dR - sum W dS/dR +

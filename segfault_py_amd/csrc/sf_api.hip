@@ -242,11 +242,39 @@ struct sf_basis {
     int engine[kNumQuartetClass] = {0};
     JKGraph graph;
     hipStream_t cap_stream = nullptr;
+    // concurrent builds: the class launches and the per-shape reductions of
+    // sf_jk / sf_jk_graph are spread over the caller's stream and `nside`
+    // side streams (0 = all on the caller's stream), forked from and joined
+    // back into the caller's stream with events.  One side stream measured
+    // best for eager builds of 20 waters (FP64 7.05 -> 6.35 ms, mixed 6.97
+    // -> 6.16 ms; three: 6.52 / 6.32 eager, 6.24 / 6.06 as a graph)
+    static constexpr int kMaxSide = 3;
+    int nside = 1;
+    // per class and precision, its launches' time in one build (ms; 0 =
+    // unknown), from sf_autotune's estimates or a sf_jk_profile build: the
+    // cost the launches are balanced over the streams by
+    double class_ms[kNumQuartetClass][2] = {};
+    hipStream_t side[kMaxSide] = {};
+    hipEvent_t ev_fork = nullptr, ev_join[kMaxSide] = {};
 
     ~sf_basis()
     {
         drop_graph();
         if (cap_stream) (void)hipStreamDestroy(cap_stream);
+        for (int k = 0; k < kMaxSide; ++k) {
+            if (side[k]) (void)hipStreamDestroy(side[k]);
+            if (ev_join[k]) (void)hipEventDestroy(ev_join[k]);
+        }
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+    }
+    int ensure_side()
+    {
+        for (int k = 0; k < nside; ++k) {
+            if (!side[k]) SF_HIP(hipStreamCreateWithFlags(&side[k], hipStreamNonBlocking));
+            if (!ev_join[k]) SF_HIP(hipEventCreateWithFlags(&ev_join[k], hipEventDisableTiming));
+        }
+        if (!ev_fork) SF_HIP(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+        return kOk;
     }
     void drop_graph()
     {
@@ -1272,6 +1300,7 @@ extern "C" int sf_screen(sf_basis* b, double tau, double tau_mixed)
     SF_HIP(hipDeviceSynchronize());
     tm.mark("positions + buffers");
     b->screened = true;
+    for (int c = 0; c < kNumQuartetClass; ++c) b->class_ms[c][0] = b->class_ms[c][1] = 0.0;
     return kOk;
 }
 
@@ -1369,6 +1398,17 @@ static int launch_class_engine(sf_basis* b, int c, int seg, int mode, const Clas
 {
     if (b->engine[c] == 1) return launch_eri_mmd(c, mode, cp, tn, s);
     return launch_eri_class(c, (seg == 1 && !tn.arith64) ? 1 : 0, mode, cp, tn, s);
+}
+
+extern "C" int sf_set_streams(sf_basis* b, int nside)
+{
+    if (!b || nside < 0 || nside > sf_basis::kMaxSide) {
+        set_error(kEInval, "sf_set_streams: nside must be 0..%d", sf_basis::kMaxSide);
+        return kEInval;
+    }
+    b->nside = nside;
+    b->drop_graph();
+    return kOk;
 }
 
 extern "C" int sf_set_engine(sf_basis* b, int qcls, int engine)
@@ -1571,9 +1611,49 @@ static int run_jk(sf_basis* b, const double* P, double* J, double* K, int precis
     SF_TRY(launch_scale_density(P, b->d_fnorm.as<double>(), ml, b->d_Ps.as<double>(), s));
     SF_TRY(mark());
     slot.push_back(42);
-    for (int c = 0; c < kNumQuartetClass; ++c) {
+    // concurrent launches (not when timing launch groups): the classes write
+    // disjoint tile slots and raw planes, the reductions disjoint blocks of
+    // X and Y, so each group can go to any stream between a fork and a join
+    const int nside = ms ? 0 : b->nside;
+    if (nside > 0) SF_TRY(b->ensure_side());
+    hipStream_t st[1 + sf_basis::kMaxSide] = {s};
+    for (int k = 0; k < nside; ++k) st[1 + k] = b->side[k];
+    auto fork = [&]() -> int {
+        if (nside == 0) return kOk;
+        SF_HIP(hipEventRecord(b->ev_fork, s));
+        for (int k = 0; k < nside; ++k) SF_HIP(hipStreamWaitEvent(b->side[k], b->ev_fork, 0));
+        return kOk;
+    };
+    auto join = [&]() -> int {
+        for (int k = 0; k < nside; ++k) {
+            SF_HIP(hipEventRecord(b->ev_join[k], b->side[k]));
+            SF_HIP(hipStreamWaitEvent(s, b->ev_join[k], 0));
+        }
+        return kOk;
+    };
+    // longest-first onto the least-loaded stream, by a cost estimate
+    double load[1 + sf_basis::kMaxSide] = {0.0};
+    auto pick = [&](double cost) -> hipStream_t {
+        int k = 0;
+        for (int t = 1; t <= nside; ++t)
+            if (load[t] < load[k]) k = t;
+        load[k] += cost;
+        return st[k];
+    };
+    int order[kNumQuartetClass];
+    for (int c = 0; c < kNumQuartetClass; ++c) order[c] = c;
+    bool measured = true;
+    for (int c = 0; c < kNumQuartetClass; ++c)
+        if (b->qc[c].nq_all > 0 && !(b->class_ms[c][precision] > 0.0)) measured = false;
+    auto class_cost = [&](int c) {
+        return measured ? b->class_ms[c][precision]
+                        : (double)b->qc[c].nq_all * (double)class_nint(c) + 2.0e5;
+    };
+    if (nside > 0)
+        std::stable_sort(order, order + kNumQuartetClass,
+                         [&](int x, int y) { return class_cost(x) > class_cost(y); });
+    auto one_class = [&](int c, hipStream_t cs) -> int {
         QuartetClassData& Q = b->qc[c];
-        if (Q.nq_all == 0) continue;
         ClassParams cp = class_params(b, c, b->d_Ps.as<double>());
         if (precision == 1 && b->engine[c] == 0 && b->tune[c][2].arith64) {
             // both segments in FP64 arithmetic: one launch over every quartet
@@ -1583,10 +1663,10 @@ static int run_jk(sf_basis* b, const double* P, double* J, double* K, int precis
             cp.tau_prim = b->tau_prim64;
             cp.n64 = Q.d_k032.as<int>();
             cp.tau_prim_lo = b->tau_prim;
-            SF_TRY(launch_class_engine(b, c, 0, kModeJK, cp, b->tune[c][1], s));
+            SF_TRY(launch_class_engine(b, c, 0, kModeJK, cp, b->tune[c][1], cs));
             SF_TRY(mark());
             slot.push_back(2 * c);
-            continue;
+            return kOk;
         }
         if (precision == 1 && b->engine[c] == 0 && class_split(c)) {
             // split class, mixed build: the two segments write their raw
@@ -1594,29 +1674,41 @@ static int run_jk(sf_basis* b, const double* P, double* J, double* K, int precis
             // launch over the class (as in an FP64 build) reads them all
             for (int seg = 0; seg <= 1; ++seg) {
                 set_segment(b, c, precision, seg, cp, b->d_zero.as<int>());
-                SF_TRY(launch_class_engine(b, c, seg, kModeEriOnly, cp, b->tune[c][1 + seg], s));
+                SF_TRY(launch_class_engine(b, c, seg, kModeEriOnly, cp, b->tune[c][1 + seg], cs));
                 if (seg == 0) {
                     SF_TRY(mark());
                     slot.push_back(2 * c);
                 }
             }
             set_segment(b, c, 0, 0, cp, b->d_zero.as<int>());
-            SF_TRY(launch_class_engine(b, c, 0, kModeDigestOnly, cp, b->tune[c][0], s));
+            SF_TRY(launch_class_engine(b, c, 0, kModeDigestOnly, cp, b->tune[c][0], cs));
             SF_TRY(mark());
             slot.push_back(2 * c + 1);
-            continue;
+            return kOk;
         }
         for (int seg = 0; seg <= precision; ++seg) {
             set_segment(b, c, precision, seg, cp, b->d_zero.as<int>());
-            SF_TRY(launch_class_engine(b, c, seg, kModeJK, cp, b->tune[c][tune_slot(precision, seg)], s));
+            SF_TRY(launch_class_engine(b, c, seg, kModeJK, cp, b->tune[c][tune_slot(precision, seg)], cs));
             SF_TRY(mark());
             slot.push_back(2 * c + seg);
         }
+        return kOk;
+    };
+    SF_TRY(fork());
+    int rc = kOk;
+    for (int oc = 0; oc < kNumQuartetClass && rc == kOk; ++oc) {
+        const int c = order[oc];
+        if (b->qc[c].nq_all == 0) continue;
+        rc = one_class(c, nside ? pick(class_cost(c)) : s);
     }
+    SF_TRY(join());
+    SF_TRY(rc);
     SF_TRY(launch_fill(b->d_X.as<double>(), b->extent, 0.0, s));
     SF_TRY(launch_fill(b->d_Y.as<double>(), b->extent, 0.0, s));
+    SF_TRY(fork());
     const int nf[3] = {1, 3, 6};
-    for (int g = 0; g < 9; ++g) {
+    for (int k = 0; k <= nside; ++k) load[k] = 0.0;
+    for (int g = 0; g < 9 && rc == kOk; ++g) {
         DestList dl;
         dl.s1 = b->d_ds1[g].as<int>();
         dl.s2 = b->d_ds2[g].as<int>();
@@ -1631,9 +1723,12 @@ static int run_jk(sf_basis* b, const double* P, double* J, double* K, int precis
         dl.m_nch = b->d_m_nch[g].as<int>();
         dl.nm = b->nmulti[g];
         dl.part = b->d_part.as<double>() + b->part_off[g];
-        SF_TRY(launch_reduce(nf[g / 3], nf[g % 3], dl, b->d_tiles.as<double>(), b->map(),
-                             b->d_X.as<double>(), b->d_Y.as<double>(), s));
+        rc = launch_reduce(nf[g / 3], nf[g % 3], dl, b->d_tiles.as<double>(), b->map(),
+                           b->d_X.as<double>(), b->d_Y.as<double>(),
+                           nside ? pick((double)b->nchunk[g] * nf[g / 3] * nf[g % 3]) : s);
     }
+    SF_TRY(join());
+    SF_TRY(rc);
     SF_TRY(launch_finalize(b->d_X.as<double>(), b->d_Y.as<double>(), b->d_fnorm.as<double>(), ml, J,
                            K, s));
     SF_TRY(mark());
@@ -1647,6 +1742,8 @@ static int run_jk(sf_basis* b, const double* P, double* J, double* K, int precis
             ms[slot[k - 1]] += t;
         }
         for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+        for (int c = 0; c < kNumQuartetClass; ++c)
+            b->class_ms[c][precision] = b->qc[c].nq_all > 0 ? ms[2 * c] + ms[2 * c + 1] : 0.0;
     }
     return kOk;
 }
@@ -1811,6 +1908,9 @@ extern "C" int sf_autotune(sf_basis* b, const double* P, int precision, void* st
             seg_ms[seg] = best_ms;
             if (ms) ms[3 * c + slot] = best_ms;
         }
+        if (!rc && Q.nq_all > 0)
+            b->class_ms[c][precision] = precision == 0 ? full(seg_ms[0], Q.nq_all)
+                                                       : full(seg_ms[0], Q.nq64) + full(seg_ms[1], Q.nq32);
         // FP64 arithmetic in the FP32 segment means one launch over both
         // segments (run_jk), so the choice is between that launch and two
         // with FP32 arithmetic
@@ -1829,9 +1929,11 @@ extern "C" int sf_autotune(sf_basis* b, const double* P, int precision, void* st
                 b->tune[c][1] = bm;
                 b->tune[c][2].arith64 = 1;
                 if (ms) ms[3 * c + 1] = mm, ms[3 * c + 2] = 0.0;
+                b->class_ms[c][1] = full(mm, Q.nq_all);
             } else {
                 b->tune[c][2] = best32;
                 if (ms) ms[3 * c + 2] = ms32;
+                b->class_ms[c][1] = two;
             }
         }
     }

@@ -94,6 +94,7 @@ def lib():
     L.sf_set_prim_tau_fp64.argtypes = [vp, ctypes.c_double]
     L.sf_prim_quartet_counts.argtypes = [vp, ctypes.c_int, _i64p]
     L.sf_set_engine.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    L.sf_set_streams.argtypes = [vp, ctypes.c_int]
     _LIB = L
     return L
 
@@ -111,7 +112,7 @@ def exported_symbols() -> list[str]:
             "sf_eri_dense", "sf_jk", "sf_jk_profile", "sf_basis_set_nuclei", "sf_one_electron",
             "sf_grad", "sf_basis_create_batch", "sf_basis_set_layout", "sf_basis_layout",
             "sf_jk_graph", "sf_autotune", "sf_set_tuning", "sf_set_prim_tau", "sf_set_prim_tau_fp64",
-            "sf_set_engine", "sf_prim_quartet_counts"]
+            "sf_set_engine", "sf_set_streams", "sf_prim_quartet_counts"]
 
 
 def _torch():
@@ -552,6 +553,13 @@ class Basis:
         """Integral engine of one quartet class (or all, qcls = -1): 0 = Rys,
         1 = McMurchie-Davidson with MFMA E.R.E transforms (sf_set_engine)."""
         check(lib().sf_set_engine(self._h, int(qcls), int(engine)))
+        return self
+
+    def set_streams(self, nside: int) -> "Basis":
+        """Side streams of jk()/jk_graph() (0..3): class launches and
+        reductions spread over the current stream and nside more; 0 runs
+        them all in order on the current stream (sf_set_streams)."""
+        check(lib().sf_set_streams(self._h, int(nside)))
         return self
 
     def set_tuning(self, shapes):

@@ -158,6 +158,33 @@ def test_jk_deterministic_and_mixed_close(gpu):
     assert counts[:, 2].sum() > 0
 
 
+def test_jk_streams_bitwise(gpu):
+    """Class launches and reductions spread over side streams (eager and
+    replayed as a graph, replays between eager builds) give J/K bitwise
+    equal to the all-in-order build: every launch writes its own slots."""
+    from segfault_py_amd import engine, workloads as W
+    mol = W.config_molecule(3)  # 10 waters, N = 220
+    b = engine.Basis(mol).screen(1e-12, 1e-5)
+    _, P = _density(mol.nbf, 7, "cuda")
+    for prec in (0, 1):
+        b.set_streams(0)
+        J0, K0 = b.jk(P, prec)
+        for ns in (1, 2, 3):
+            b.set_streams(ns)
+            J, K = b.jk(P, prec)
+            assert gpu.equal(J, J0) and gpu.equal(K, K0), (prec, ns)
+            Jg, Kg = gpu.empty_like(P), gpu.empty_like(P)
+            for _ in range(3):
+                b.jk_graph(P, (Jg, Kg), prec)
+                gpu.cuda.synchronize()
+                assert gpu.equal(Jg, J0) and gpu.equal(Kg, K0), (prec, ns, "graph")
+                b.jk(P, prec, out=(J, K))
+                gpu.cuda.synchronize()
+    for bad in (-1, 4):
+        with pytest.raises(engine.EngineError):
+            b.set_streams(bad)
+
+
 def test_mixed_primitive_screening(gpu):
     """FP32 quartets skip primitive quartets below tau_prim (default
     1e-8 * tau_mixed): against the same FP32 build without that screening,

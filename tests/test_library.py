@@ -139,6 +139,7 @@ def test_batch_and_graph_argument_errors(lib):
                                      np.zeros(2, dtype=np.int32), ctypes.byref(h)) == 1
     shapes = np.full(189, 256, dtype=np.int32)
     assert lib.sf_set_tuning(None, shapes) == 1
+    assert lib.sf_set_streams(None, 1) == 1
     assert lib.sf_jk_graph(None, None, None, None, 0, None) == 1
     assert lib.sf_autotune(None, None, 0, None, shapes, np.zeros(63)) == 1
     assert lib.sf_basis_set_layout(None, np.zeros(1, dtype=np.int64), np.zeros(1, dtype=np.int32)) == 1
