@@ -33,7 +33,9 @@ int sf_version(void);
  * (core.f90:232-812) in batched form.
  * sf_boys: F[i*(mmax+1)+m] = F_m(T[i]), 0 <= mmax <= 24.
  * sf_rys_roots: nodes as t^2 (the convention of root5's output,
- * core.f90:810-811), ascending, and weights; 1 <= nroots <= 5. */
+ * core.f90:810-811), ascending, and weights; 1 <= nroots <= 7 (root5
+ * has 5; 6 and 7 nodes are the exact rules for the f-shell quartets with
+ * total angular momentum 10..13). */
 int sf_boys(long long n, int mmax, const double *T, double *F, void *stream);
 int sf_rys_roots(long long n, int nroots, const double *X, double *u, double *w,
                  void *stream);
@@ -43,7 +45,10 @@ int sf_rys_roots(long long n, int nroots, const double *X, double *u, double *w,
  * (core.f90:967-1002; called at src/hf/hf.py:26-30 and :100-104):
  * for function k (0..3), npgf[k] primitives stored back to back in coefs,
  * expns and norms (norms = the reference's normalize_factors), shells[3k..]
- * = (lx,ly,lz), centers[3k..] in bohr.  Total angular momentum <= 9.
+ * = (lx,ly,lz), centers[3k..] in bohr.  Total angular momentum L <= 13,
+ * with floor(L/2) + 1 nodes: the reference's 5 nodes (core.f90:952) up to
+ * L = 9, where they are exact; above, the exact 6- or 7-node rule (the
+ * reference's 5-node sum is not exact there), status 5 beyond.
  * sf_eri_components: nq quartets over a table of nf such functions
  * (primitives of function f at [poff[f], poff[f]+npgf[f])); the batched
  * form of the Pool.map over _build_eri (hf.py:18-47). */
@@ -70,7 +75,9 @@ int sf_eri_components_noalloc(int nf, const int *npgf, const int *poff, const do
                               double *out);
 
 /* ---- Shell basis handle -------------------------------------------------
- * nshell contracted shells (L <= 2) in AO order.  Shell s: angular momentum
+ * nshell contracted shells (L <= 3: s, p, d, f; status 5 above) in AO
+ * order (f: xxx, xxy, xxz, xyy, xyz, xzz, yyy, yyz, yzz, zzz as
+ * molecule.py:155-165).  Shell s: angular momentum
  * L[s], nprim[s] primitives (stored back to back in expns/coefs), centre
  * center[3s..3s+2] (bohr), atom shell_atom[s] in [0, natom) (status 1
  * otherwise; in a batch, one of its own molecule's nuclei).  shell_atom may
@@ -145,7 +152,8 @@ int sf_set_prim_tau(sf_basis *b, double tau_prim);
  * screening) therefore skips nothing.  0 disables it. */
 int sf_set_prim_tau_fp64(sf_basis *b, double tau_prim);
 /* counts[3*c + {0,1,2}] = surviving / FP64 / FP32 shell quartets of
- * quartet class c (0..20); *funcq = surviving function-level quartets. */
+ * quartet class c (0..54: classes 0..20 hold s, p, d shells, 21..54 an f
+ * shell); *funcq = surviving function-level quartets. */
 int sf_quartet_counts(const sf_basis *b, long long *counts, long long *funcq,
                       long long *primq);
 /* Primitive quartets the Rys class kernels evaluate in one build of
@@ -154,7 +162,8 @@ int sf_quartet_counts(const sf_basis *b, long long *counts, long long *funcq,
  * the whole class (precision 0) or its FP64 segment, seg 1 = its FP32
  * segment (0 for precision 0).  Runs a counting kernel on the null stream. */
 int sf_prim_quartet_counts(sf_basis *b, int precision, long long *counts);
-/* Sorted significant pairs of pair class pc (0..5): shells, Q and the
+/* Sorted significant pairs of pair class pc (0..9: ss ps pp ds dp dd fs
+ * fp fd ff): shells, Q and the
  * number of primitive pairs kept after primitive-pair screening (any output
  * pointer may be NULL). */
 int sf_pair_list(const sf_basis *b, int pc, int *n, int *sa, int *sb, double *Q, int *nprim);
@@ -174,12 +183,12 @@ int sf_eri_dense(sf_basis *b, double *eri);
 int sf_jk(sf_basis *b, const double *P, double *J, double *K, int precision, void *stream);
 /* sf_jk with HIP events between launch groups on `stream` (for measurement;
  * no counterpart in the reference, whose only timer is utils.timeit,
- * src/hf/utils.py:76-83).  ms[44] receives milliseconds: ms[2c] / ms[2c+1]
+ * src/hf/utils.py:76-83).  ms[112] receives milliseconds: ms[2c] / ms[2c+1]
  * = FP64 / FP32 ERI+digestion kernel of quartet class c (for a class with
  * more than 54 integrals per quartet in a mixed build, whose two segments
  * share one digestion launch: its integrals / the FP32 segment's integrals
- * plus that digestion), ms[42] = density scaling, ms[43] = deterministic
- * plan reduction and finalisation. */
+ * plus that digestion; c = 0..54), ms[110] = density scaling, ms[111] =
+ * deterministic plan reduction and finalisation. */
 int sf_jk_profile(sf_basis *b, const double *P, double *J, double *K, int precision,
                   void *stream, double *ms);
 /* sf_jk through a captured hipGraph: the first call (and the first after
@@ -209,15 +218,16 @@ int sf_jk_graph(sf_basis *b, const double *P, double *J, double *K, int precisio
  * untouched); either may be NULL. */
 int sf_autotune(sf_basis *b, const double *P, int precision, void *stream, int *shapes,
                 double *ms);
-/* Set the shapes (all 3 slots of all 21 classes, 189 ints) directly, in
+/* Set the shapes (all 3 slots of all 55 classes, 495 ints) directly, in
  * the layout sf_autotune reports. */
 int sf_set_tuning(sf_basis *b, const int *shapes);
-/* Integral engine of quartet class qcls (0..20, or -1 for all): 0 = Rys
+/* Integral engine of quartet class qcls (0..54, or -1 for all): 0 = Rys
  * quadrature (default; the method of rys.eri_rys, core.f90:917-1002),
  * 1 = McMurchie-Davidson (the method of mmd.eri, core.f90:100-198) with the
  * Hermite-to-Cartesian E.R.E transforms on the FP64 matrix cores.  Engine 1
- * computes in FP64 in both precision segments.  Used by sf_jk, sf_jk_graph,
- * sf_autotune and sf_eri_dense. */
+ * computes in FP64 in both precision segments and covers the s/p/d classes
+ * 0..20 (status 5 for an f-shell class; qcls = -1 leaves those on Rys).
+ * Used by sf_jk, sf_jk_graph, sf_autotune and sf_eri_dense. */
 int sf_set_engine(sf_basis *b, int qcls, int engine);
 /* Side streams of sf_jk / sf_jk_graph (0..3; default 1): the quartet-class
  * launches, and then the per-tile-shape reductions, are spread over the
@@ -240,7 +250,8 @@ int sf_set_streams(sf_basis *b, int nside);
  * the segment's threshold are skipped as in sf_jk (sf_set_prim_tau_fp64,
  * sf_set_prim_tau).
  * No counterpart in the reference (single-point energies only); nuclear
- * repulsion is not included.  Requires sf_basis_set_nuclei for every atom. */
+ * repulsion is not included.  Requires sf_basis_set_nuclei for every atom.
+ * s, p, d shells: a basis with f shells returns status 5. */
 int sf_grad(sf_basis *b, const double *P, const double *W, double *grad, int precision,
             void *stream);
 

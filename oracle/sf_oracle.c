@@ -383,12 +383,22 @@ static double orc_ix(int l1, int l2, int l3, int l4, double x1, double x2, doubl
     return orc_shift(G, l3 + l4 + 1, l1, l2, l3, l4, x1 - x2, x3 - x4);
 }
 
-/* rys.eri_3d_rys, core.f90:917-964: primitive quartet, always 5 nodes;
+/* rys.eri_3d_rys, core.f90:917-964: primitive quartet with 5 nodes, as
+ * the reference (core.f90:952), up to total angular momentum L = 9, where 5
+ * nodes integrate the polynomial part exactly.  Above (quartets with f
+ * shells, L = 10..12) the reference's 5-node sum is no longer exact; the
+ * oracle then takes floor(L/2) + 1 nodes of the exact quadrature, i.e. it
+ * departs from the reference there on purpose and stays the exact integral
+ * (pinned against the McMurchie-Davidson restatement, which has no
+ * quadrature, in tests/test_oracle.py).
  * root_method 0 = the exact quadrature (orc_rys_roots), 1 = the fitted
- * evaluator (orc_rys5_fast, tables built). */
+ * 5-node evaluator (orc_rys5_fast, tables built) where 5 nodes are used. */
 static double eri_3d_rys_m(const int l[12], const double xyz[12], const double ex[4],
                            int root_method)
 {
+    int L = 0;
+    for (int k = 0; k < 12; ++k) L += l[k];
+    const int nr = L / 2 + 1 > 5 ? L / 2 + 1 : 5;
     const double a = ex[0], b = ex[1], c = ex[2], d = ex[3];
     const double a_ = a + b, b_ = c + d, rho = a_ * b_ / (a_ + b_);
     double rpq2 = 0.0;
@@ -397,11 +407,11 @@ static double eri_3d_rys_m(const int l[12], const double xyz[12], const double e
         double Q = (c * xyz[6 + k] + d * xyz[9 + k]) / b_;
         rpq2 += (P - Q) * (P - Q);
     }
-    double u[5], w[5];
-    if (root_method == 1) orc_rys5_fast(rpq2 * rho, u, w);
-    else orc_rys_roots(5, rpq2 * rho, u, w);
+    double u[8], w[8];
+    if (root_method == 1 && nr == 5) orc_rys5_fast(rpq2 * rho, u, w);
+    else orc_rys_roots(nr, rpq2 * rho, u, w);
     double res = 0.0;
-    for (int r = 0; r < 5; ++r) {
+    for (int r = 0; r < nr; ++r) {
         double ix = orc_ix(l[0], l[3], l[6], l[9], xyz[0], xyz[3], xyz[6], xyz[9], a, b, c, d, u[r]);
         double iy = orc_ix(l[1], l[4], l[7], l[10], xyz[1], xyz[4], xyz[7], xyz[10], a, b, c, d, u[r]);
         double iz = orc_ix(l[2], l[5], l[8], l[11], xyz[2], xyz[5], xyz[8], xyz[11], a, b, c, d, u[r]);

@@ -21,11 +21,22 @@ OUT = os.path.join(HERE, "libsfeng.so")
 # quartet class so the 21 classes build in parallel.
 # The d-shell classes (6..20) are further split into their J/K, gradient
 # and McMurchie-Davidson parts (SF_PART), the heaviest compiles.
+# f-shell classes with compiled kernels (the SF_FOR_EACH_FCLASS list of
+# k_eri.hip): class id and (la, lb, lc, ld); J/K only, one unit each.
+FCLASSES = [(21, 3, 0, 0, 0), (22, 3, 0, 1, 0), (23, 3, 0, 1, 1), (24, 3, 0, 2, 0),
+            (25, 3, 0, 2, 1), (26, 3, 0, 2, 2), (27, 3, 0, 3, 0), (28, 3, 1, 0, 0),
+            (29, 3, 1, 1, 0), (30, 3, 1, 1, 1), (31, 3, 1, 2, 0), (32, 3, 1, 2, 1),
+            (34, 3, 1, 3, 0), (36, 3, 2, 0, 0), (37, 3, 2, 1, 0), (38, 3, 2, 1, 1),
+            (39, 3, 2, 2, 0), (45, 3, 3, 0, 0), (46, 3, 3, 1, 0)]
 UNITS = ([("k_eri.hip", [], "k_eri_dispatch.o")]
          + [("k_eri.hip", [f"-DSF_CLASS={c}"], f"k_eri_c{c:02d}.o") for c in range(6)]
          + [("k_eri.hip", [f"-DSF_CLASS={c}", f"-DSF_PART={p}"], f"k_eri_c{c:02d}p{p}.o")
             for c in range(6, 21) for p in range(3)]
-         + [("k_util.hip", [], "k_util.o"), ("k_onee.hip", [], "k_onee.o"),
+         + [("k_eri.hip", ["-DSF_CLASS=99", f"-DSF_FCLASS={c}", f"-DSF_FLA={a}", f"-DSF_FLB={b}",
+                           f"-DSF_FLC={cc}", f"-DSF_FLD={d}"], f"k_eri_f{c:02d}.o")
+            for c, a, b, cc, d in FCLASSES]
+         + [("k_eri_gen.hip", [], "k_eri_gen.o"),
+            ("k_util.hip", [], "k_util.o"), ("k_onee.hip", [], "k_onee.o"),
             ("sf_api.hip", [], "sf_api.o")])
 ARCH = os.environ.get("SF_OFFLOAD_ARCH", "gfx950")
 
@@ -79,6 +90,13 @@ def build(verbose: bool = False, extra_flags: list[str] | None = None,
     # dominate the build time
     def cost(c):
         name = c[0]
+        if name.startswith("k_eri_f"):
+            # integrals per quartet of the f class (compile time grows with it)
+            cid = int(name[7:9])
+            ncart = lambda L: (L + 1) * (L + 2) // 2  # noqa: E731
+            ls = next(f[1:] for f in FCLASSES if f[0] == cid)
+            n = ncart(ls[0]) * ncart(ls[1]) * ncart(ls[2]) * ncart(ls[3])
+            return n / 20.0
         if not name.startswith("k_eri_c"):
             return 0
         part = name[10:11] if name[9:10] == "p" else "0"

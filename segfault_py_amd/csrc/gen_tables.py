@@ -2,7 +2,8 @@
 
 Run once in a container with mpmath; the generated file is committed.
 
-1. Rys quadrature, n = 1..5 nodes: for X in [0, XMAX[n]) split into unit
+1. Rys quadrature, n = 1..7 nodes (7 nodes are exact up to total angular
+   momentum 13, i.e. every f-shell quartet): for X in [0, XMAX[n]) split into unit
    intervals, each root u_r(X) = t_r^2 and weight w_r(X) is a degree-11
    Chebyshev series on its interval (interpolation at the 12 Chebyshev
    points).  The exact values at the nodes come from Gautschi's Chebyshev
@@ -25,8 +26,8 @@ import mpmath as mp
 
 mp.mp.dps = 50
 
-NMAX = 5
-XMAX = {1: 40, 2: 48, 3: 54, 4: 60, 5: 66}
+NMAX = 7
+XMAX = {1: 40, 2: 48, 3: 54, 4: 60, 5: 66, 6: 72, 7: 78}
 DEG = 11
 BOYS_DX = 0.1
 BOYS_NGRID = 401          # grid up to X = 40
@@ -90,6 +91,19 @@ def cheb_coefs(values_at_nodes):
     return out
 
 
+def cheb_eval(c, s):
+    """sum_j c_j T_j(s) (Clenshaw, as the kernels evaluate it)."""
+    b1 = b2 = mp.mpf(0)
+    for j in range(len(c) - 1, 0, -1):
+        b1, b2 = c[j] + 2 * s * b1 - b2, b1
+    return c[0] + s * b1 - b2
+
+
+# the tables of n <= NCHECKED nodes were checked against mpmath by the tests
+# when they were generated; larger n are checked here as well
+NCHECKED = 5
+
+
 def fmt(v) -> str:
     return mp.nstr(v, 20, min_fixed=-1, max_fixed=-1) if v != 0 else "0.0"
 
@@ -100,7 +114,8 @@ def main(path: str):
     lines.append(f"#define SF_BOYS_DX {BOYS_DX!r}")
     lines.append(f"#define SF_BOYS_NGRID {BOYS_NGRID}")
     lines.append(f"#define SF_BOYS_MTAB {BOYS_MTAB}")
-    lines.append("SF_TABLE_QUAL int sf_rys_xmax[6] = {0, " + ", ".join(str(XMAX[n]) for n in range(1, 6)) + "};")
+    lines.append(f"#define SF_RYS_NMAX {NMAX}")
+    lines.append(f"SF_TABLE_QUAL int sf_rys_xmax[{NMAX + 1}] = {{0, " + ", ".join(str(XMAX[n]) for n in range(1, NMAX + 1)) + "};")
     M = DEG
     nodes = [mp.cos(mp.pi * (k + mp.mpf(1) / 2) / (M + 1)) for k in range(M + 1)]
     for n in range(1, NMAX + 1):
@@ -110,15 +125,32 @@ def main(path: str):
         nint = XMAX[n]
         # layout: [interval][value: u_0..u_{n-1}, w_0..w_{n-1}][coef 0..DEG]
         lines.append(f"SF_TABLE_QUAL double sf_rys_cheb{n}[{nint * 2 * n * (DEG + 1)}] = {{")
+        worst = mp.mpf(0)
         for it in range(nint):
             a, b = mp.mpf(it), mp.mpf(it + 1)
             vals = [rys_mp(n, a + (b - a) * (s + 1) / 2) for s in nodes]
+            coefs = []
             for v in range(2 * n):
                 series = [vals[k][0][v] if v < n else vals[k][1][v - n] for k in range(M + 1)]
                 c = cheb_coefs(series)
+                coefs.append(c)
                 lines.append("  " + ", ".join(fmt(x) for x in c) + ",")
+            if n > NCHECKED:
+                # relative interpolation error between the interpolation points
+                for s in (mp.mpf(-3) / 4, mp.mpf(-1) / 3, mp.mpf(0), mp.mpf(1) / 2, mp.mpf(7) / 8):
+                    u, w = rys_mp(n, a + (b - a) * (s + 1) / 2)
+                    for v in range(2 * n):
+                        ref = u[v] if v < n else w[v - n]
+                        worst = max(worst, abs(cheb_eval(coefs[v], s) - ref) / abs(ref))
             print(f"n={n} interval {it}", file=sys.stderr) if it % 20 == 0 else None
         lines.append("};")
+        if n > NCHECKED:
+            u, w = rys_mp(n, XMAX[n])
+            lim = max(max(abs(h[r] ** 2 / XMAX[n] - u[r]) / u[r], abs(W[r] / mp.sqrt(XMAX[n]) - w[r]) / w[r])
+                      for r in range(n))
+            print(f"n={n}: interpolation error {mp.nstr(worst, 3)}, large-X limit error at X={XMAX[n]} "
+                  f"{mp.nstr(lim, 3)}", file=sys.stderr)
+            assert worst < 1e-14 and lim < 2e-15, (n, worst, lim)
     lines.append(f"SF_TABLE_QUAL double sf_boys_grid[{BOYS_NGRID * BOYS_MTAB}] = {{")
     for k in range(BOYS_NGRID):
         X = mp.mpf(k) / 10  # k * BOYS_DX exactly

@@ -26,6 +26,20 @@
     X(15, 2, 2, 0, 0) X(16, 2, 2, 1, 0) X(17, 2, 2, 1, 1) X(18, 2, 2, 2, 0) X(19, 2, 2, 2, 1)    \
     X(20, 2, 2, 2, 2)
 
+// f-shell classes with compiled kernels (J/K and the dense export; the
+// gradient and the McMurchie-Davidson engine cover s, p, d): those with at
+// most 54 integrals per quartet (one thread per quartet) or at most 540 and
+// at most 54 per component of A (one wavefront per (64 quartets, component
+// of A)).  Each is built in a unit of its own (-DSF_FCLASS=id with
+// -DSF_FLA..SF_FLD); compile time grows steeply beyond (measured: (fd|pp)
+// 43 s, (fd|dp) 6 min, (ff|ff) 26 min), so the 15 larger f classes run the
+// run-time-L kernels of k_eri_gen.hip.
+#define SF_FOR_EACH_FCLASS(X)                                                                   \
+    X(21, 3, 0, 0, 0) X(22, 3, 0, 1, 0) X(23, 3, 0, 1, 1) X(24, 3, 0, 2, 0) X(25, 3, 0, 2, 1)   \
+    X(26, 3, 0, 2, 2) X(27, 3, 0, 3, 0) X(28, 3, 1, 0, 0) X(29, 3, 1, 1, 0) X(30, 3, 1, 1, 1)   \
+    X(31, 3, 1, 2, 0) X(32, 3, 1, 2, 1) X(34, 3, 1, 3, 0) X(36, 3, 2, 0, 0) X(37, 3, 2, 1, 0)   \
+    X(38, 3, 2, 1, 1) X(39, 3, 2, 2, 0) X(45, 3, 3, 0, 0) X(46, 3, 3, 1, 0)
+
 namespace sf {
 
 using SchwarzLaunch = int (*)(const int*, const int*, const int*, const int*, int, const PrimPair*,
@@ -39,42 +53,11 @@ using SchwarzLaunch = int (*)(const int*, const int*, const int*, const int*, in
     int launch_mmd_##id(int mode, const ClassParams& cp, const Tune& tn, hipStream_t s);
 SF_FOR_EACH_CLASS(X)
 #undef X
-
-__device__ __forceinline__ int find_bra(const long long* __restrict__ base, int n, long long q)
-{
-    int lo = 0, hi = n;  // base[lo] <= q < base[hi]
-    while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (base[mid] <= q) lo = mid;
-        else hi = mid;
-    }
-    return lo;
-}
-
-// Flat segment index q -> bra i, ket j (index in the ket list) and the
-// class-wide quartet index qa.
-struct QuartetIdx {
-    int i, j;
-    long long qa;
-};
-
-// The bra of quartet q: one binary search per wavefront for its first
-// lane's quartet qw (wave-uniform, so scalar loads), then each lane walks
-// forward over the few bras the wavefront spans.  Requires qw <= q < nq.
-__device__ __forceinline__ QuartetIdx locate(const ClassParams& cp, long long q, long long qw)
-{
-    const unsigned lo32 = __builtin_amdgcn_readfirstlane((unsigned)(qw & 0xffffffffLL));
-    const unsigned hi32 = __builtin_amdgcn_readfirstlane((unsigned)(qw >> 32));
-    const long long qu = ((long long)hi32 << 32) | lo32;
-    int i = __builtin_amdgcn_readfirstlane(find_bra(cp.seg_base, cp.bra.n, qu));
-    while (cp.seg_base[i + 1] <= q) ++i;
-    QuartetIdx r;
-    r.i = i;
-    const int jr = cp.seg_k0[i] + (int)(q - cp.seg_base[i]);
-    r.j = cp.ket0[i] + jr;
-    r.qa = cp.qall_base[i] + jr;
-    return r;
-}
+#define X(id, a, b, c, d)                                                                       \
+    int launch_class_##id(int fp32, int mode, const ClassParams& cp, const Tune& tn,           \
+                          hipStream_t s);
+SF_FOR_EACH_FCLASS(X)
+#undef X
 
 #ifdef SF_CLASS
 
@@ -782,6 +765,18 @@ static int launch_schwarz_t(const int* sa, const int* sb, const int* poff, const
 
 SF_FOR_EACH_CLASS(SF_PICK)
 
+// An f-shell class unit: the J/K / dense-export launcher of one class.
+#ifdef SF_FCLASS
+#define SF_FNAME2(id) launch_class_##id
+#define SF_FNAME(id) SF_FNAME2(id)
+int SF_FNAME(SF_FCLASS)(int fp32, int mode, const ClassParams& cp, const Tune& tn, hipStream_t s)
+{
+    static_assert(quartet_class(pair_class(SF_FLA, SF_FLB), pair_class(SF_FLC, SF_FLD)) ==
+                      SF_FCLASS, "f class id and angular momenta disagree");
+    return launch_class<SF_FLA, SF_FLB, SF_FLC, SF_FLD>(fp32, mode, cp, tn, s);
+}
+#endif
+
 #else  // dispatch tables
 
 static_assert(quartet_class(pair_class(2, 1), pair_class(1, 1)) == 12, "class numbering");
@@ -831,6 +826,25 @@ int launch_count_prims(const ClassParams& cp, int nblock, unsigned long long* pa
     return kOk;
 }
 
+// (la, lb, lc, ld) of quartet class c.
+static GenClass class_shape(int c)
+{
+    const int pcb = pair_of_class(c), pck = c - pcb * (pcb + 1) / 2;
+    return gen_class(pair_la(pcb), pair_lb(pcb), pair_la(pck), pair_lb(pck));
+}
+
+int class_generic(int qcls)
+{
+    switch (qcls) {
+#define X(id, a, b, c, d) \
+    case id: return 0;
+        SF_FOR_EACH_CLASS(X)
+        SF_FOR_EACH_FCLASS(X)
+#undef X
+    }
+    return (qcls >= kNumClassSPD && qcls < kNumQuartetClass) ? 1 : 0;
+}
+
 int launch_eri_class(int qcls, int fp32, int mode, const ClassParams& cp, const Tune& tn,
                      hipStream_t s)
 {
@@ -838,14 +852,22 @@ int launch_eri_class(int qcls, int fp32, int mode, const ClassParams& cp, const 
 #define X(id, a, b, c, d) \
     case id: return launch_class_##id(fp32, mode, cp, tn, s);
         SF_FOR_EACH_CLASS(X)
+        SF_FOR_EACH_FCLASS(X)
 #undef X
     }
+    // the larger f-shell classes: the run-time-L kernels, FP64 arithmetic
+    if (class_generic(qcls)) return launch_eri_generic(qcls, mode, cp, tn, s);
     set_error(kEUnsupported, "quartet class %d", qcls);
     return kEUnsupported;
 }
 
 int launch_eri_mmd(int qcls, int mode, const ClassParams& cp, const Tune& tn, hipStream_t s)
 {
+    if (qcls >= kNumClassSPD && qcls < kNumQuartetClass) {
+        set_error(kEUnsupported, "quartet class %d: the McMurchie-Davidson engine covers s, p, d "
+                  "shells only", qcls);
+        return kEUnsupported;
+    }
     switch (qcls) {
 #define X(id, a, b, c, d) \
     case id: return launch_mmd_##id(mode, cp, tn, s);
@@ -858,6 +880,7 @@ int launch_eri_mmd(int qcls, int mode, const ClassParams& cp, const Tune& tn, hi
 
 int class_ntile(int qcls)
 {
+    if (qcls >= kNumClassSPD && qcls < kNumQuartetClass) return class_shape(qcls).ntile;
     switch (qcls) {
 #define X(id, a, b, c, d) \
     case id: return QuartetShape<a, b, c, d>::NTILE;
@@ -869,6 +892,11 @@ int class_ntile(int qcls)
 
 int launch_grad_class(int qcls, int fp32, const ClassParams& cp, double* gout, hipStream_t s)
 {
+    if (qcls >= kNumClassSPD && qcls < kNumQuartetClass) {
+        set_error(kEUnsupported, "quartet class %d: nuclear gradients cover s, p, d shells only",
+                  qcls);
+        return kEUnsupported;
+    }
     switch (qcls) {
 #define X(id, a, b, c, d) \
     case id: return launch_grad_##id(fp32, cp, gout, s);
@@ -881,6 +909,7 @@ int launch_grad_class(int qcls, int fp32, const ClassParams& cp, double* gout, h
 
 int class_nslice(int qcls)
 {
+    if (qcls >= kNumClassSPD) return 0;
     switch (qcls) {
 #define X(id, a, b, c, d) \
     case id: return QuartetShape<a, b, c, d>::NSLICE_GRAD;
@@ -896,13 +925,17 @@ int class_split(int qcls)
 #define X(id, a, b, c, d) \
     case id: return QuartetShape<a, b, c, d>::SPLIT ? 1 : 0;
         SF_FOR_EACH_CLASS(X)
+        SF_FOR_EACH_FCLASS(X)
 #undef X
     }
+    // the run-time-L classes write raw planes and digest them after
+    if (class_generic(qcls)) return 1;
     return 0;
 }
 
 int class_nint(int qcls)
 {
+    if (qcls >= kNumClassSPD && qcls < kNumQuartetClass) return class_shape(qcls).nint;
     switch (qcls) {
 #define X(id, a, b, c, d) \
     case id: return QuartetShape<a, b, c, d>::NINT;
@@ -914,6 +947,11 @@ int class_nint(int qcls)
 
 void class_tile_offsets(int qcls, int off[6])
 {
+    if (qcls >= kNumClassSPD && qcls < kNumQuartetClass) {
+        const GenClass g = class_shape(qcls);
+        for (int t = 0; t < 6; ++t) off[t] = g.toff[t];
+        return;
+    }
     switch (qcls) {
 #define X(id, a, b, c, d)                                        \
     case id: {                                                   \
@@ -932,13 +970,15 @@ int launch_schwarz(int pc, const int* sa, const int* sb, const int* poff, const 
                    hipStream_t s)
 {
     // the diagonal quartet class (pc|pc) holds the pair class's Schwarz kernel
-    static const SchwarzLaunch diag[kNumPairClass] = {launch_schwarz_0, launch_schwarz_2,
-                                                      launch_schwarz_5, launch_schwarz_9,
-                                                      launch_schwarz_14, launch_schwarz_20};
+    static const SchwarzLaunch diag[kNumPairSPD] = {launch_schwarz_0, launch_schwarz_2,
+                                                         launch_schwarz_5, launch_schwarz_9,
+                                                         launch_schwarz_14, launch_schwarz_20};
     if (pc < 0 || pc >= kNumPairClass) {
         set_error(kEUnsupported, "pair class %d", pc);
         return kEUnsupported;
     }
+    if (pc >= kNumPairSPD)
+        return launch_schwarz_generic(pc, sa, sb, poff, np, n, prim, shell_xyz, Q, Qp, s);
     return diag[pc](sa, sb, poff, np, n, prim, shell_xyz, Q, Qp, s);
 }
 

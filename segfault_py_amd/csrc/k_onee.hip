@@ -70,6 +70,7 @@ int launch_one_electron(int pc, const int* sa, const int* sb, const int* poff, c
         break;
     switch (pc) {
         O_(0, 0, 0) O_(1, 1, 0) O_(2, 1, 1) O_(3, 2, 0) O_(4, 2, 1) O_(5, 2, 2)
+        O_(6, 3, 0) O_(7, 3, 1) O_(8, 3, 2) O_(9, 3, 3)
     default: set_error(kEUnsupported, "pair class %d", pc); return kEUnsupported;
     }
 #undef O_
@@ -77,7 +78,7 @@ int launch_one_electron(int pc, const int* sa, const int* sb, const int* poff, c
     return kOk;
 }
 
-// Gradient of the one-electron energy, in two passes over a pair class:
+// Gradient of the one-electron energy (s, p, d shells), in two passes over a pair class:
 //  k_grad_onee_v:  one thread per (shell pair, nucleus c) computes the
 //                  attraction derivatives for c: writes d/dC into the pair's
 //                  row at atom c and d/dA, d/dB into part[(i*natom + c)*6];

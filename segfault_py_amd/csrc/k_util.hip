@@ -65,7 +65,9 @@ int launch_rys(long long n, int nroots, const double* X, double* u, double* w, h
     case 3: hipLaunchKernelGGL(k_rys<3>, g, b, 0, s, n, X, u, w); break;
     case 4: hipLaunchKernelGGL(k_rys<4>, g, b, 0, s, n, X, u, w); break;
     case 5: hipLaunchKernelGGL(k_rys<5>, g, b, 0, s, n, X, u, w); break;
-    default: set_error(kEInval, "nroots %d outside [1, 5]", nroots); return kEInval;
+    case 6: hipLaunchKernelGGL(k_rys<6>, g, b, 0, s, n, X, u, w); break;
+    case 7: hipLaunchKernelGGL(k_rys<7>, g, b, 0, s, n, X, u, w); break;
+    default: set_error(kEInval, "nroots %d outside [1, 7]", nroots); return kEInval;
     }
     SF_HIP(hipGetLastError());
     return kOk;
@@ -336,6 +338,7 @@ int launch_reduce(int n1, int n2, const DestList& dl, const double* tiles, const
         return kOk;                                                                           \
     }
     R_(1, 1) R_(1, 3) R_(1, 6) R_(3, 1) R_(3, 3) R_(3, 6) R_(6, 1) R_(6, 3) R_(6, 6)
+    R_(1, 10) R_(3, 10) R_(6, 10) R_(10, 1) R_(10, 3) R_(10, 6) R_(10, 10)
 #undef R_
     set_error(kEUnsupported, "tile %dx%d", n1, n2);
     return kEUnsupported;
@@ -591,7 +594,9 @@ int launch_fill(double* p, long long n, double v, hipStream_t s)
 // One thread per contracted Cartesian-component quartet, with the argument
 // meaning of rys.eri_rys (src/hf/core/core.f90:967-1002): per-primitive
 // normalisation factors and contraction coefficients, per-function
-// (lx, ly, lz).  Total angular momentum <= 9 (5 nodes), like root5.
+// (lx, ly, lz).  Total angular momentum L <= 13 with floor(L/2) + 1 <= 7
+// nodes: the reference's 5 nodes (root5) up to L = 9, where they are exact;
+// above, the exact rule instead of the reference's 5-node approximation.
 template <int N>
 __device__ double component_prim(const int* l, const double* xyz, const double* ex)
 {
@@ -617,7 +622,7 @@ __device__ double component_prim(const int* l, const double* xyz, const double* 
         for (int k = 0; k < 3; ++k) {
             const int li = l[k], lj = l[3 + k], lk = l[6 + k], ll = l[9 + k];
             const int nn = li + lj, mm = lk + ll;
-            double G[10][10];
+            double G[14][14];  // total L <= 13 (checked by the caller)
             const double c00 = (P[k] - xyz[k]) - q * ipq * (P[k] - Q[k]) * t2;
             const double c00p = (Q[k] - xyz[6 + k]) + p * ipq * (P[k] - Q[k]) * t2;
             G[0][0] = 1.0;
@@ -678,7 +683,9 @@ __global__ __launch_bounds__(128) void k_eri_components(
                     case 2: v = component_prim<2>(l, xyz, ex); break;
                     case 3: v = component_prim<3>(l, xyz, ex); break;
                     case 4: v = component_prim<4>(l, xyz, ex); break;
-                    default: v = component_prim<5>(l, xyz, ex); break;
+                    case 5: v = component_prim<5>(l, xyz, ex); break;
+                    case 6: v = component_prim<6>(l, xyz, ex); break;
+                    default: v = component_prim<7>(l, xyz, ex); break;
                     }
                     res += norms[p1] * norms[p2] * norms[p3] * norms[p4] * coefs[p1] * coefs[p2] *
                            coefs[p3] * coefs[p4] * v;

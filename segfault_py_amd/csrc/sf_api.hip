@@ -227,10 +227,12 @@ struct sf_basis {
     // digestion plan: destination-sorted tile slots
     DevBuf d_segb, d_sege, d_key_base, d_tiles;
     // reduction lists per tile shape g = 3 L1 + L2 (see DestList)
-    DevBuf d_ds1[9], d_ds2[9], d_dxy[9], d_ch_src[9], d_ch_n[9], d_ch_dest[9], d_ch_part[9];
-    DevBuf d_m_dest[9], d_m_pbase[9], d_m_nch[9], d_part;
-    int ndest[9] = {0}, nchunk[9] = {0}, nmulti[9] = {0};
-    long long part_off[9] = {0};
+    DevBuf d_ds1[kNumTileGroup], d_ds2[kNumTileGroup], d_dxy[kNumTileGroup],
+        d_ch_src[kNumTileGroup], d_ch_n[kNumTileGroup], d_ch_dest[kNumTileGroup],
+        d_ch_part[kNumTileGroup];
+    DevBuf d_m_dest[kNumTileGroup], d_m_pbase[kNumTileGroup], d_m_nch[kNumTileGroup], d_part;
+    int ndest[kNumTileGroup] = {0}, nchunk[kNumTileGroup] = {0}, nmulti[kNumTileGroup] = {0};
+    long long part_off[kNumTileGroup] = {0};
     unsigned long long ntile_total = 0;
     DevBuf d_Ps, d_X, d_Y;
     // launch shapes per quartet class and precision segment (sf_autotune)
@@ -360,8 +362,8 @@ static int check_components_args(int nf, const int* npgf, const int* poff, const
             }
             L += shell[3 * f] + shell[3 * f + 1] + shell[3 * f + 2];
         }
-        if (L > 9) {
-            set_error(kEUnsupported, "quartet %lld: total angular momentum %d > 9", q, L);
+        if (L > 13) {
+            set_error(kEUnsupported, "quartet %lld: total angular momentum %d > 13", q, L);
             return kEUnsupported;
         }
     }
@@ -582,7 +584,7 @@ static int create_basis(int nmol, const int* mol_shell, int nshell, const int* L
         b->mol_fbase.push_back(off);
         for (int s = mol_shell[m]; s < mol_shell[m + 1]; ++s) {
             if (L[s] < 0 || L[s] > kMaxL) {
-                set_error(kEUnsupported, "shell %d: L = %d (s, p, d supported)", s, L[s]);
+                set_error(kEUnsupported, "shell %d: L = %d (s, p, d, f supported)", s, L[s]);
                 delete b;
                 return kEUnsupported;
             }
@@ -1153,12 +1155,13 @@ extern "C" int sf_screen(sf_basis* b, double tau, double tau_mixed)
     // destination group, then (after prefix sums) the lists in place.
     std::vector<unsigned long long> key_base(nkeys);
     std::vector<int> key_ne(nkeys);
-    std::vector<int> ds1[9], ds2[9], dxy[9], ch_n[9], ch_dest[9], ch_part[9];
-    std::vector<unsigned long long> ch_src[9];
-    std::vector<int> m_dest[9], m_pbase[9], m_nch[9];
+    constexpr int NG = kNumTileGroup;
+    std::vector<int> ds1[NG], ds2[NG], dxy[NG], ch_n[NG], ch_dest[NG], ch_part[NG];
+    std::vector<unsigned long long> ch_src[NG];
+    std::vector<int> m_dest[NG], m_pbase[NG], m_nch[NG];
     struct SlotCount {
         unsigned long long acc;
-        long long nd[9], nch[9], nm[9], npart[9];
+        long long nd[kNumTileGroup], nch[kNumTileGroup], nm[kNumTileGroup], npart[kNumTileGroup];
     };
     std::vector<SlotCount> cnt_m((size_t)nmol_ + 1, SlotCount{});
     auto key_shells = [&](int m, unsigned long long k, int& s1, int& s2, int& xy) {
@@ -1177,7 +1180,7 @@ extern "C" int sf_screen(sf_basis* b, double tau, double tau_mixed)
                 int s1, s2, xy;
                 key_shells((int)m, k, s1, s2, xy);
                 const long long n = sege[k] - segb[k];
-                const int g = 3 * b->L[s1] + b->L[s2];
+                const int g = 4 * b->L[s1] + b->L[s2];
                 const long long cw = reduce_chunk_tiles(cart_n(b->L[s1]) * cart_n(b->L[s2]));
                 const long long nch = (n + cw - 1) / cw;
                 c.acc += (unsigned long long)n * (cart_n(b->L[s1]) * cart_n(b->L[s2]));
@@ -1196,7 +1199,7 @@ extern "C" int sf_screen(sf_basis* b, double tau, double tau_mixed)
         SlotCount& n = cnt_m[m + 1];
         const SlotCount& p = cnt_m[m];
         n.acc += p.acc;
-        for (int g = 0; g < 9; ++g) {
+        for (int g = 0; g < NG; ++g) {
             n.nd[g] += p.nd[g];
             n.nch[g] += p.nch[g];
             n.nm[g] += p.nm[g];
@@ -1208,8 +1211,8 @@ extern "C" int sf_screen(sf_basis* b, double tau, double tau_mixed)
         set_error(kEUnsupported, "digestion plan of %llu doubles exceeds 32-bit slots", acc);
         return kEUnsupported;
     }
-    int npart[9];
-    for (int g = 0; g < 9; ++g) {
+    int npart[NG];
+    for (int g = 0; g < NG; ++g) {
         const SlotCount& t = cnt_m[nmol_];
         if (t.nd[g] >= (1LL << 31) || t.nch[g] >= (1LL << 31) || t.npart[g] >= (1LL << 31)) {
             set_error(kEUnsupported, "digestion plan: too many destinations or chunks");
@@ -1238,7 +1241,7 @@ extern "C" int sf_screen(sf_basis* b, double tau, double tau_mixed)
                 if (sege[k] <= segb[k]) continue;
                 const long long n = sege[k] - segb[k];
                 o.acc += (unsigned long long)n * key_ne[k];
-                const int g = 3 * b->L[s1] + b->L[s2];
+                const int g = 4 * b->L[s1] + b->L[s2];
                 const int d = (int)o.nd[g]++;
                 ds1[g][d] = s1;
                 ds2[g][d] = s2;
@@ -1278,13 +1281,13 @@ extern "C" int sf_screen(sf_basis* b, double tau, double tau_mixed)
     d_vals2.release();
     SF_TRY(b->d_tiles.ensure(std::max<size_t>(1, acc) * sizeof(double)));
     long long part_total = 0;
-    const int nf3[3] = {1, 3, 6};
-    for (int g = 0; g < 9; ++g) {
+    const int nf4[4] = {1, 3, 6, 10};
+    for (int g = 0; g < NG; ++g) {
         b->ndest[g] = (int)ds1[g].size();
         b->nchunk[g] = (int)ch_n[g].size();
         b->nmulti[g] = (int)m_dest[g].size();
         b->part_off[g] = part_total;
-        part_total += (long long)npart[g] * nf3[g / 3] * nf3[g % 3];
+        part_total += (long long)npart[g] * nf4[g / 4] * nf4[g % 4];
         SF_TRY(b->d_ds1[g].upload(ds1[g]));
         SF_TRY(b->d_ds2[g].upload(ds2[g]));
         SF_TRY(b->d_dxy[g].upload(dxy[g]));
@@ -1417,8 +1420,14 @@ extern "C" int sf_set_engine(sf_basis* b, int qcls, int engine)
         set_error(kEInval, "sf_set_engine: bad arguments");
         return kEInval;
     }
+    if (engine == 1 && qcls >= kNumClassSPD) {
+        set_error(kEUnsupported, "sf_set_engine: quartet class %d has an f shell; the "
+                  "McMurchie-Davidson engine covers s, p, d shells", qcls);
+        return kEUnsupported;
+    }
     for (int c = 0; c < kNumQuartetClass; ++c) {
         if (qcls >= 0 && c != qcls) continue;
+        if (c >= kNumClassSPD) continue;  // f-shell classes: Rys only
         b->engine[c] = engine;
         QuartetClassData& Q = b->qc[c];
         if (b->screened && engine == 1)
@@ -1590,8 +1599,8 @@ static int check_jk_args(sf_basis* b, const double* P, double* J, double* K, int
 // memset, so a stream capture of it is a valid graph); when ms is non-null,
 // HIP events bracket every launch group on the stream and ms receives
 // milliseconds per group: [2c] FP64 and [2c+1] FP32 ERI+digestion kernels
-// of quartet class c (c = 0..20), [42] density scaling, [43] plan
-// reduction + finalisation.
+// of quartet class c (c = 0..54), [kSlotScale] density scaling,
+// [kSlotReduce] plan reduction + finalisation.
 static int run_jk(sf_basis* b, const double* P, double* J, double* K, int precision,
                   hipStream_t s, double* ms)
 {
@@ -1610,7 +1619,7 @@ static int run_jk(sf_basis* b, const double* P, double* J, double* K, int precis
     SF_TRY(mark());
     SF_TRY(launch_scale_density(P, b->d_fnorm.as<double>(), ml, b->d_Ps.as<double>(), s));
     SF_TRY(mark());
-    slot.push_back(42);
+    slot.push_back(kSlotScale);
     // concurrent launches (not when timing launch groups): the classes write
     // disjoint tile slots and raw planes, the reductions disjoint blocks of
     // X and Y, so each group can go to any stream between a fork and a join
@@ -1655,7 +1664,8 @@ static int run_jk(sf_basis* b, const double* P, double* J, double* K, int precis
     auto one_class = [&](int c, hipStream_t cs) -> int {
         QuartetClassData& Q = b->qc[c];
         ClassParams cp = class_params(b, c, b->d_Ps.as<double>());
-        if (precision == 1 && b->engine[c] == 0 && b->tune[c][2].arith64) {
+        if (precision == 1 && b->engine[c] == 0 &&
+            (b->tune[c][2].arith64 || class_generic(c))) {
             // both segments in FP64 arithmetic: one launch over every quartet
             // of the class, each bra's FP32-segment kets with that segment's
             // primitive threshold (the same integrals as two launches)
@@ -1706,9 +1716,9 @@ static int run_jk(sf_basis* b, const double* P, double* J, double* K, int precis
     SF_TRY(launch_fill(b->d_X.as<double>(), b->extent, 0.0, s));
     SF_TRY(launch_fill(b->d_Y.as<double>(), b->extent, 0.0, s));
     SF_TRY(fork());
-    const int nf[3] = {1, 3, 6};
+    const int nf[4] = {1, 3, 6, 10};
     for (int k = 0; k <= nside; ++k) load[k] = 0.0;
-    for (int g = 0; g < 9 && rc == kOk; ++g) {
+    for (int g = 0; g < kNumTileGroup && rc == kOk; ++g) {
         DestList dl;
         dl.s1 = b->d_ds1[g].as<int>();
         dl.s2 = b->d_ds2[g].as<int>();
@@ -1723,18 +1733,18 @@ static int run_jk(sf_basis* b, const double* P, double* J, double* K, int precis
         dl.m_nch = b->d_m_nch[g].as<int>();
         dl.nm = b->nmulti[g];
         dl.part = b->d_part.as<double>() + b->part_off[g];
-        rc = launch_reduce(nf[g / 3], nf[g % 3], dl, b->d_tiles.as<double>(), b->map(),
+        rc = launch_reduce(nf[g / 4], nf[g % 4], dl, b->d_tiles.as<double>(), b->map(),
                            b->d_X.as<double>(), b->d_Y.as<double>(),
-                           nside ? pick((double)b->nchunk[g] * nf[g / 3] * nf[g % 3]) : s);
+                           nside ? pick((double)b->nchunk[g] * nf[g / 4] * nf[g % 4]) : s);
     }
     SF_TRY(join());
     SF_TRY(rc);
     SF_TRY(launch_finalize(b->d_X.as<double>(), b->d_Y.as<double>(), b->d_fnorm.as<double>(), ml, J,
                            K, s));
     SF_TRY(mark());
-    slot.push_back(43);
+    slot.push_back(kSlotReduce);
     if (ms) {
-        for (int k = 0; k < 44; ++k) ms[k] = 0.0;
+        for (int k = 0; k < kNumSlots; ++k) ms[k] = 0.0;
         SF_HIP(hipEventSynchronize(ev.back()));
         for (size_t k = 1; k < ev.size(); ++k) {
             float t = 0.f;
@@ -1849,6 +1859,9 @@ extern "C" int sf_autotune(sf_basis* b, const double* P, int precision, void* st
         SF_TRY(launch_class_engine(b, c, seg, kModeJK, tcp, warm, s));  // untimed warm-up
         for (int bi = 0; bi < 3; ++bi)
             for (int mi = 0; mi < 3; ++mi) {
+                // the run-time-L classes keep 256-thread blocks (timed once,
+                // for the stream balance of run_jk)
+                if (class_generic(c) && (bi != 2 || mi != 0)) continue;
                 Tune t;
                 t.block = blocks[bi];
                 t.grid = mults[mi] * ncu;
@@ -2080,6 +2093,11 @@ extern "C" int sf_grad(sf_basis* b, const double* P, const double* W, double* gr
     if (!b->screened) {
         set_error(kEState, "sf_grad: call sf_screen first");
         return kEState;
+    }
+    if (std::find(b->L.begin(), b->L.end(), 3) != b->L.end()) {
+        set_error(kEUnsupported, "sf_grad: nuclear gradients cover s, p, d shells; the basis has "
+                  "f shells");
+        return kEUnsupported;
     }
     if (b->nmol != 1) {
         set_error(kEUnsupported, "sf_grad: one molecule only");

@@ -5,6 +5,7 @@
 
 #include <cstdint>
 
+#include "eri_gen.hpp"
 #include "mmd.hpp"
 
 namespace sf {
@@ -31,14 +32,38 @@ void set_error(int code, const char* fmt, ...);
         }                                                                                \
     } while (0)
 
-constexpr int kMaxL = 2;           // s, p, d shells
-constexpr int kNumPairClass = 6;   // (00) (10) (11) (20) (21) (22)
-constexpr int kNumQuartetClass = 21;
+constexpr int kMaxL = 3;            // s, p, d, f shells
+constexpr int kNumPairClass = 10;   // (00) (10) (11) (20) (21) (22) (30) (31) (32) (33)
+constexpr int kNumQuartetClass = 55;
+// Quartet classes 0..20 hold s, p, d shells only, classes 21..54 an f
+// shell; pair classes 0..5 s, p, d, 6..9 an f shell.  Every s/p/d class and
+// the f classes of up to 540 integrals per quartet have compiled kernels
+// (k_eri.hip); the larger f classes run the run-time-L kernels of
+// k_eri_gen.hip (class_generic).
+constexpr int kNumClassSPD = 21;
+constexpr int kNumPairSPD = 6;
+// Digestion tile shapes N1 x N2 with N1, N2 in {1, 3, 6, 10} (s, p, d, f
+// components): tile group g = 4 L1 + L2.
+constexpr int kNumTileGroup = 16;
+// sf_jk_profile: milliseconds per launch group, [2c] / [2c + 1] the FP64 /
+// FP32 segment of quartet class c, then density scaling and the reduction.
+constexpr int kSlotScale = 2 * kNumQuartetClass;
+constexpr int kSlotReduce = kSlotScale + 1;
+constexpr int kNumSlots = kSlotScale + 2;
 
 constexpr int pair_class(int la, int lb) { return la * (la + 1) / 2 + lb; }  // la >= lb
-constexpr int pair_la(int pc) { return pc == 0 ? 0 : pc < 3 ? 1 : 2; }
+constexpr int pair_la(int pc) { return pc == 0 ? 0 : pc < 3 ? 1 : pc < 6 ? 2 : 3; }
 constexpr int pair_lb(int pc) { return pc - pair_la(pc) * (pair_la(pc) + 1) / 2; }
 constexpr int quartet_class(int pcb, int pck) { return pcb * (pcb + 1) / 2 + pck; }  // pcb >= pck
+// bra pair class of quartet class c
+constexpr int pair_of_class(int c)
+{
+    int p = 0;
+    while ((p + 1) * (p + 2) / 2 <= c) ++p;
+    return p;
+}
+static_assert(quartet_class(pair_class(3, 0), 0) == kNumClassSPD, "f classes follow s/p/d");
+static_assert(quartet_class(pair_class(3, 3), pair_class(3, 3)) == kNumQuartetClass - 1, "55 classes");
 
 // One sorted pair class on the device.
 struct PairListDev {
@@ -95,6 +120,43 @@ struct ClassParams {
     const unsigned* pos;       // kModeJK: pos[t * qstride + q] = slot of tile t in `tiles`
     double* tiles;             // kModeJK: destination-sorted digestion tiles
 };
+
+// Flat segment index q -> bra i, ket j (index in the ket list) and the
+// class-wide quartet index qa.
+struct QuartetIdx {
+    int i, j;
+    long long qa;
+};
+
+__device__ __forceinline__ int find_bra(const long long* __restrict__ base, int n, long long q)
+{
+    int lo = 0, hi = n;  // base[lo] <= q < base[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (base[mid] <= q) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// The bra of quartet q: one binary search per wavefront for its first
+// lane's quartet qw (wave-uniform, so scalar loads), then each lane walks
+// forward over the few bras the wavefront spans.  Requires qw <= q < nq.
+__device__ __forceinline__ QuartetIdx locate(const ClassParams& cp, long long q, long long qw)
+{
+    const unsigned lo32 = __builtin_amdgcn_readfirstlane((unsigned)(qw & 0xffffffffLL));
+    const unsigned hi32 = __builtin_amdgcn_readfirstlane((unsigned)(qw >> 32));
+    const long long qu = ((long long)hi32 << 32) | lo32;
+    int i = __builtin_amdgcn_readfirstlane(find_bra(cp.seg_base, cp.bra.n, qu));
+    while (cp.seg_base[i + 1] <= q) ++i;
+    QuartetIdx r;
+    r.i = i;
+    const int jr = cp.seg_k0[i] + (int)(q - cp.seg_base[i]);
+    r.j = cp.ket0[i] + jr;
+    r.qa = cp.qall_base[i] + jr;
+    return r;
+}
+
 
 // kModeJK: integrals and digestion tiles; kModeRaw: raw integral planes
 // into ClassParams::out (dense export).  Split classes only:
@@ -160,6 +222,14 @@ struct KeyMap {
     const unsigned* mol_kbase;
 };
 
+// Launchers (k_eri_gen.hip): the f-shell quartet classes without compiled
+// kernels (class_generic) in every mode (integrals always in FP64), and the
+// Schwarz factors of the f pair classes.
+int launch_eri_generic(int qcls, int mode, const ClassParams& cp, const Tune& tn, hipStream_t s);
+int launch_schwarz_generic(int pc, const int* sa, const int* sb, const int* poff, const int* np,
+                           int n, const PrimPair* prim, const double* shell_xyz, double* Q,
+                           double* Qp, hipStream_t s);
+
 // Launchers (k_eri.hip).
 int launch_eri_class(int qcls, int fp32, int mode, const ClassParams& cp, const Tune& tn,
                      hipStream_t s);
@@ -170,6 +240,8 @@ int launch_schwarz(int pc, const int* sa, const int* sb, const int* poff, const 
                    const PrimPair* prim, const double* shell_xyz, double* Q, double* Qp,
                    hipStream_t s);
 int class_ntile(int qcls);
+// 1 for the f-shell classes that run the run-time-L kernels (k_eri_gen.hip)
+int class_generic(int qcls);
 int class_nint(int qcls);
 int class_split(int qcls);
 int class_nslice(int qcls);

@@ -11,7 +11,7 @@
 //    (erfc(sqrt(40)) < 2e-19) and upward recursion.
 //  * F_0, F_1 for one-node quadrature: the same grid Taylor series for both
 //    (no exp) below X = 40, F_0 = sqrt(pi/X)/2, F_1 = F_0/(2X) above.
-//  * Rys nodes/weights for n = 1..5: unit-interval Chebyshev series of
+//  * Rys nodes/weights for n = 1..7: unit-interval Chebyshev series of
 //    degree 11 (tables generated at 50 digits by gen_tables.py) below
 //    XMAX[n], the Gauss-Hermite large-X limit above.
 #pragma once
@@ -150,9 +150,14 @@ SF_HD SF_INLINE void boys01(double X, double& F0, double& F1)
 // rows of the global grid) of 12 Taylor coefficients c_m = F_m(X_k) / m!.
 // 19.3 KB instead of the 32 KB of 400 rows of F_0..F_9, so 8 blocks of 256
 // threads fit a CU's LDS instead of 5.
+// (A 0.1 grid of 9 coefficients -- 6 fewer FMAs per primitive quartet,
+// 28.9 KB -- measured no faster on the 20-water build: 6.41-6.56 ms against
+// 6.37-6.40 ms.)
 constexpr int kBoysTaylorRows = 201;
 constexpr int kBoysTaylorCols = 12;
 constexpr double kBoysTaylorDX = 0.2;
+// rows of the global 0.1 grid per Taylor row
+constexpr int kBoysTaylorStep = 2;
 
 // 1/m! for m <= 11.
 SF_HD SF_INLINE double inv_fact(int m)
@@ -194,7 +199,9 @@ SF_HD SF_INLINE const double* rys_table()
     else if constexpr (N == 2) return sf_rys_cheb2;
     else if constexpr (N == 3) return sf_rys_cheb3;
     else if constexpr (N == 4) return sf_rys_cheb4;
-    else return sf_rys_cheb5;
+    else if constexpr (N == 5) return sf_rys_cheb5;
+    else if constexpr (N == 6) return sf_rys_cheb6;
+    else return sf_rys_cheb7;
 }
 
 template <int N>
@@ -204,7 +211,9 @@ SF_HD SF_INLINE double herm_u(int r)
     else if constexpr (N == 2) return sf_herm_u2[r];
     else if constexpr (N == 3) return sf_herm_u3[r];
     else if constexpr (N == 4) return sf_herm_u4[r];
-    else return sf_herm_u5[r];
+    else if constexpr (N == 5) return sf_herm_u5[r];
+    else if constexpr (N == 6) return sf_herm_u6[r];
+    else return sf_herm_u7[r];
 }
 
 template <int N>
@@ -214,17 +223,22 @@ SF_HD SF_INLINE double herm_w(int r)
     else if constexpr (N == 2) return sf_herm_w2[r];
     else if constexpr (N == 3) return sf_herm_w3[r];
     else if constexpr (N == 4) return sf_herm_w4[r];
-    else return sf_herm_w5[r];
+    else if constexpr (N == 5) return sf_herm_w5[r];
+    else if constexpr (N == 6) return sf_herm_w6[r];
+    else return sf_herm_w7[r];
 }
 
-constexpr int rys_xmax(int n) { return n == 1 ? 40 : n == 2 ? 48 : n == 3 ? 54 : n == 4 ? 60 : 66; }
+constexpr int rys_xmax(int n)
+{
+    return n == 1 ? 40 : n == 2 ? 48 : n == 3 ? 54 : n == 4 ? 60 : n == 5 ? 66 : n == 6 ? 72 : 78;
+}
 
 // Nodes u_r = t_r^2 (ascending) and weights w_r of the N-point Rys rule,
 // from the Chebyshev table c of N nodes (rys_table<N>() or a copy of it).
 template <int N>
 SF_HD SF_INLINE void rys_roots_tab(double X, const double* __restrict__ tab, double* u, double* w)
 {
-    static_assert(N >= 1 && N <= 5, "1..5 Rys nodes");
+    static_assert(N >= 1 && N <= SF_RYS_NMAX, "1..7 Rys nodes");
     if (X < (double)rys_xmax(N)) {
         const int it = (int)X;
         const double s = 2.0 * (X - it) - 1.0;
@@ -308,7 +322,7 @@ template <int N>
 SF_HD SF_INLINE double node_table_elem(int k)
 {
     if constexpr (N == 1)
-        return sf_boys_grid[2 * (k / kBoysTaylorCols) * SF_BOYS_MTAB + k % kBoysTaylorCols] *
+        return sf_boys_grid[kBoysTaylorStep * (k / kBoysTaylorCols) * SF_BOYS_MTAB + k % kBoysTaylorCols] *
                inv_fact(k % kBoysTaylorCols);
     else return rys_table<N>()[k];
 }

@@ -27,8 +27,14 @@ _i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
 
-NUM_QUARTET_CLASSES = 21
-PAIR_CLASSES = [(0, 0), (1, 0), (1, 1), (2, 0), (2, 1), (2, 2)]
+PAIR_CLASSES = [(0, 0), (1, 0), (1, 1), (2, 0), (2, 1), (2, 2), (3, 0), (3, 1), (3, 2), (3, 3)]
+NUM_QUARTET_CLASSES = 55      # classes 0..20: s, p, d shells; 21..54: with an f shell
+NUM_CLASSES_SPD = 21          # classes of s, p, d shells only (the rest have an f shell)
+# sf_jk_profile slots: [2c], [2c + 1] the FP64 / FP32 segment of class c,
+# then density scaling and reduction + finalisation
+SLOT_SCALE = 2 * NUM_QUARTET_CLASSES
+SLOT_REDUCE = SLOT_SCALE + 1
+NUM_SLOTS = SLOT_SCALE + 2
 
 
 def quartet_class_ls(c: int):
@@ -463,7 +469,7 @@ class Basis:
 
     def prim_quartet_counts(self, precision: int = 0) -> np.ndarray:
         """Primitive quartets the Rys kernels evaluate per class and segment
-        under the current thresholds (sf_prim_quartet_counts): [21, 2]."""
+        under the current thresholds (sf_prim_quartet_counts): [55, 2]."""
         c = np.zeros(2 * NUM_QUARTET_CLASSES, dtype=np.int64)
         check(lib().sf_prim_quartet_counts(self._h, int(precision), c))
         return c.reshape(NUM_QUARTET_CLASSES, 2)
@@ -525,8 +531,8 @@ class Basis:
 
     def autotune(self, P, precision: int = 0):
         """Pick launch shapes per quartet class (sf_autotune).  Returns
-        (shapes[21, 3, 3] as (block, grid, arith) per tuning slot -- 0: FP64
-        builds, 1/2: FP64/FP32 segment of mixed builds -- and ms[21, 3],
+        (shapes[55, 3, 3] as (block, grid, arith) per tuning slot -- 0: FP64
+        builds, 1/2: FP64/FP32 segment of mixed builds -- and ms[55, 3],
         zero for the slots not tuned by this call)."""
         torch = _torch()
         P = self._check_matrix(P)
@@ -551,7 +557,8 @@ class Basis:
 
     def set_engine(self, engine: int, qcls: int = -1) -> "Basis":
         """Integral engine of one quartet class (or all, qcls = -1): 0 = Rys,
-        1 = McMurchie-Davidson with MFMA E.R.E transforms (sf_set_engine)."""
+        1 = McMurchie-Davidson with MFMA E.R.E transforms (sf_set_engine;
+        s, p, d classes only -- qcls = -1 leaves the f-shell classes on Rys)."""
         check(lib().sf_set_engine(self._h, int(qcls), int(engine)))
         return self
 
@@ -563,15 +570,21 @@ class Basis:
         return self
 
     def set_tuning(self, shapes):
-        check(lib().sf_set_tuning(self._h, np.ascontiguousarray(shapes, dtype=np.int32).reshape(-1)))
+        """Launch shapes of every class and tuning slot, as autotune()
+        returns them: (NUM_QUARTET_CLASSES, 3, 3) ints (sf_set_tuning)."""
+        flat = np.ascontiguousarray(shapes, dtype=np.int32).reshape(-1)
+        if flat.size != 9 * NUM_QUARTET_CLASSES:
+            raise EngineError(1, f"set_tuning: {flat.size} entries, expected "
+                                 f"{9 * NUM_QUARTET_CLASSES} (classes x slots x (block, grid, arith))")
+        check(lib().sf_set_tuning(self._h, flat))
 
     def jk_profile(self, P, precision: int = 0, out=None):
         """jk() plus per-launch-group milliseconds (HIP events on the stream):
-        returns J, K, ms[44] (see sf_jk_profile in include/sfeng.h)."""
+        returns J, K, ms[NUM_SLOTS] (see sf_jk_profile in include/sfeng.h)."""
         torch = _torch()
         P = self._check_matrix(P)
         J, K = (self._zeros(), self._zeros()) if out is None else self._check_out(out)
-        ms = np.zeros(44)
+        ms = np.zeros(NUM_SLOTS)
         check(lib().sf_jk_profile(self._h, ctypes.c_void_p(P.data_ptr()),
                                   ctypes.c_void_p(J.data_ptr()), ctypes.c_void_p(K.data_ptr()),
                                   int(precision), _stream_ptr(torch), ms))

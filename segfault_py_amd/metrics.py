@@ -16,7 +16,7 @@ from __future__ import annotations
 
 import numpy as np
 
-from .engine import NUM_QUARTET_CLASSES, quartet_class_ls
+from .engine import NUM_QUARTET_CLASSES, PAIR_CLASSES, quartet_class_ls
 
 
 def ncart(l: int) -> int:
@@ -35,13 +35,13 @@ def class_work(basis, mol=None, evaluated=None) -> list[dict]:
     """Per quartet class: shell / primitive / unique-function quartet counts
     (all surviving, and the FP64 part) and algorithmic flops.  Primitive
     quartets count the primitive pairs kept by primitive-pair screening;
-    ``evaluated`` ([21, 2] from Basis.prim_quartet_counts(0)), when given,
+    ``evaluated`` ([55, 2] from Basis.prim_quartet_counts(0)), when given,
     replaces the all-quartet count by the primitive quartets the kernels
     evaluate after primitive-quartet screening, so flops count only work
     done."""
     out = []
     pairs = {}
-    for pc in range(6):
+    for pc in range(len(PAIR_CLASSES)):
         sa, sb, _, npp = basis.pair_list(pc, with_nprim=True)
         pairs[pc] = (sa, sb, npp.astype(np.int64))
     for c in range(NUM_QUARTET_CLASSES):

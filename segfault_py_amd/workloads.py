@@ -9,7 +9,8 @@ Frozen decisions (the workload text leaves them open):
   a_j, a_{j+1}, a_{j+2} -- is rejected: it makes shells of one type nearly
   linearly dependent.
 * A d shell is a single primitive with a = 0.8 (water O in the s/p/d set,
-  and C in the alkane set).
+  and C in the alkane set); an f shell (the f-shell test set only) a single
+  primitive with a = 1.1.
 * Water: O at the origin, H at 0.96 A along x and at 0.96 A rotated by
   104.5 deg in the xy plane.
 * Clusters: per water a rotation from a normalised Gaussian quaternion and an
@@ -28,17 +29,22 @@ from .molecule import Molecule
 ALPHA0 = 0.15
 RATIO = 3.0
 D_EXPONENT = 0.8
+F_EXPONENT = 1.1
 
-# number of (s, p, d) shells per element for each basis family
+# number of (s, p, d[, f]) shells per element for each basis family
 BASIS_SP = {"O": (4, 2, 0), "H": (3, 0, 0), "C": (4, 3, 0)}
 BASIS_SPD = {"O": (4, 2, 1), "H": (3, 0, 0), "C": (4, 3, 1)}
 BASIS_ALKANE = {"C": (4, 3, 1), "H": (3, 1, 0)}
+# f shells on O and H (every one of the 55 quartet classes, f on two centres)
+BASIS_SPDF = {"O": (4, 2, 1, 1), "H": (3, 1, 0, 1)}
 
 
 def even_tempered_text(shell_counts: dict) -> str:
     """Gaussian94 text for the even-tempered family described above."""
     blocks = []
-    for elem, (ns, npp, nd) in shell_counts.items():
+    for elem, counts in shell_counts.items():
+        ns, npp, nd = counts[:3]
+        nf = counts[3] if len(counts) > 3 else 0
         lines = [f"{elem}     0"]
         for letter, count in (("S", ns), ("P", npp)):
             for j in range(count):
@@ -48,6 +54,9 @@ def even_tempered_text(shell_counts: dict) -> str:
         for _ in range(nd):
             lines.append("D   1   1.00")
             lines.append(f"  {D_EXPONENT:.10f}   1.0")
+        for _ in range(nf):
+            lines.append("F   1   1.00")
+            lines.append(f"  {F_EXPONENT:.10f}   1.0")
         blocks.append("\n".join(lines))
     return "\n****\n".join(blocks) + "\n****\n"
 
@@ -137,6 +146,12 @@ def config_molecule(index: int) -> Molecule:
         s, x = water_cluster(10, seed=4)
         return Molecule.from_geometry(s, x, even_tempered_text(BASIS_SPD))
     raise ValueError(f"no single molecule for workload index {index}")
+
+
+def water_spdf() -> Molecule:
+    """The reference water with f shells on O and both H (BASIS_SPDF, N = 58):
+    every quartet class of the engine, f shells on different centres."""
+    return Molecule.from_geometry(list("OHH"), water_geometry(), even_tempered_text(BASIS_SPDF))
 
 
 def batch(n: int = 256, seed: int = 5) -> list[Molecule]:

@@ -45,6 +45,10 @@ def harness():
         H.h_grad_onee.argtypes = [ctypes.c_int, ctypes.c_int, _f64p, ctypes.c_int, _f64p, _f64p,
                                   ctypes.c_int, ctypes.c_int, ctypes.c_int, _f64p, _f64p, _f64p,
                                   _f64p, ctypes.c_double, _f64p]
+        H.h_eri_gen.argtypes = [ctypes.c_int] * 4 + [_f64p, ctypes.c_int, _f64p, ctypes.c_int,
+                                                 _f64p, _f64p, _f64p, _f64p, _f64p]
+        H.h_gen_tiles.argtypes = H.h_digest_tiles.argtypes
+        H.h_eri_bra_runtime.argtypes = H.h_eri_gen.argtypes
         H.h_one_electron.argtypes = [ctypes.c_int, ctypes.c_int, _f64p, ctypes.c_int, _f64p, _f64p,
                                      ctypes.c_int, _f64p, _f64p, _f64p, _f64p, _f64p]
         _H = H

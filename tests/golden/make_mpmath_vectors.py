@@ -19,7 +19,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 T_POINTS = ["0", "1e-14", "1e-9", "1e-5", "0.003", "0.049", "0.05", "0.1", "0.37", "0.5", "0.99",
             "1", "2.5", "3.14159", "7.3", "10", "15.5", "19.99", "25", "29.9", "30", "33.3",
             "39.94", "39.96", "40", "41", "55", "66", "80", "150", "1000", "12345"]
-X_POINTS = T_POINTS + ["0.8", "4.5", "47.9", "48.1", "53.9", "54.1", "59.9", "60.1", "65.9", "66.1"]
+X_POINTS = T_POINTS + ["0.8", "4.5", "47.9", "48.1", "53.9", "54.1", "59.9", "60.1", "65.9", "66.1",
+                       "71.9", "72.1", "77.9", "78.1"]
 
 
 def boys(m, T):
@@ -63,7 +64,7 @@ def main():
     with open(os.path.join(HERE, "boys_mpmath.json"), "w") as fh:
         json.dump(bo, fh, indent=0)
     ry = {"points": []}
-    for n in range(1, 6):
+    for n in range(1, 8):
         for X in X_POINTS:
             u, w = rys(n, X)
             ry["points"].append({"n": n, "X": X, "u": [mp.nstr(v, 30) for v in u],

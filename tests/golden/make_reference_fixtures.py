@@ -5,6 +5,8 @@ Writes:
   cc-pvdz-HCNO.dat      -- the H, C, N, O blocks of src/hf/data/basis/cc-pvdz.dat
   cc-pvdz-HS.dat        -- its H and S blocks (sulfur: the element whose header
                            line the reference's parser misreads, molecule.py:128-133)
+  cc-pvtz-HCO.dat       -- the H, C, O blocks of src/hf/data/basis/cc-pvtz.dat
+                           (f shells on C and O)
   molecules/*.xyz       -- the reference geometries of the H/C/N/O molecules and
                            of hydrogen_sulfide
   molecule_test.json    -- src/hf/benchmark/molecule_test.json (energies, wall
@@ -22,8 +24,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 
 
 def main():
-    text = open(os.path.join(REF, "data/basis/cc-pvdz.dat")).read()
-    for out, elems in (("cc-pvdz-HCNO.dat", "H|C|N|O"), ("cc-pvdz-HS.dat", "H|S")):
+    for src, out, elems in (("cc-pvdz.dat", "cc-pvdz-HCNO.dat", "H|C|N|O"),
+                            ("cc-pvdz.dat", "cc-pvdz-HS.dat", "H|S"),
+                            ("cc-pvtz.dat", "cc-pvtz-HCO.dat", "H|C|O")):
+        text = open(os.path.join(REF, "data/basis", src)).read()
         keep = []
         for block in text.split("****"):
             b = block.strip()

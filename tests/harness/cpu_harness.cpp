@@ -1,6 +1,7 @@
 // CPU harness: compiles the HIP kernels' host+device math headers with g++
 // so that tests can check them against the oracle without a GPU.
 // Test infrastructure only.
+#include "../../segfault_py_amd/csrc/eri_gen.hpp"
 #include "../../segfault_py_amd/csrc/mmd.hpp"
 
 using namespace sf;
@@ -29,6 +30,8 @@ extern "C" int h_rys_kernel(int n, double X, double* u, double* w)
     case 3: rys_nodes_tab<3>(X, global_node_table<3>(), u, w); return 0;
     case 4: rys_nodes_tab<4>(X, global_node_table<4>(), u, w); return 0;
     case 5: rys_nodes_tab<5>(X, global_node_table<5>(), u, w); return 0;
+    case 6: rys_nodes_tab<6>(X, global_node_table<6>(), u, w); return 0;
+    case 7: rys_nodes_tab<7>(X, global_node_table<7>(), u, w); return 0;
     }
     return 1;
 }
@@ -41,6 +44,8 @@ extern "C" int h_rys(int n, double X, double* u, double* w)
     case 3: rys_roots<3>(X, u, w); return 0;
     case 4: rys_roots<4>(X, u, w); return 0;
     case 5: rys_roots<5>(X, u, w); return 0;
+    case 6: rys_roots<6>(X, u, w); return 0;
+    case 7: rys_roots<7>(X, u, w); return 0;
     }
     return 1;
 }
@@ -265,9 +270,79 @@ extern "C" int h_one_electron(int la, int lb, const double* prim, int np, const 
         one_electron_shell_pair<x, y>(pp, np, A, B, natom, Z, C, s, t, v);           \
         return 0;                                                                    \
     }
-    O_(0, 0) O_(1, 0) O_(1, 1) O_(2, 0) O_(2, 1) O_(2, 2)
+    O_(0, 0) O_(1, 0) O_(1, 1) O_(2, 0) O_(2, 1) O_(2, 2) O_(3, 0) O_(3, 1) O_(3, 2) O_(3, 3)
 #undef O_
     return 1;
+}
+
+// The run-time-L path of the f-shell classes (k_eri_gen.hip): every
+// component of one shell quartet, component by component (gen_component,
+// no primitive screening), in (a, b, c, d) order.
+extern "C" int h_eri_gen(int la, int lb, int lc, int ld, const double* bra, int nb,
+                         const double* ket, int nk, const double* A, const double* B,
+                         const double* C, const double* D, double* out)
+{
+    const GenClass g = gen_class(la, lb, lc, ld);
+    if (g.nr > SF_RYS_NMAX) return 1;
+    for (int k = 0; k < g.nint; ++k) {
+        int lcx[12];
+        gen_exps(g, k, lcx);
+        double v = 0.0;
+        static_switch<1, SF_RYS_NMAX + 1>(g.nr, [&](auto nc) {
+            constexpr int NR = decltype(nc)::value;
+            v = gen_component<NR>(lcx, A, B, C, D, (const PrimPair*)bra, nb, (const PrimPair*)ket,
+                                  nk, nullptr, nullptr, 0.0);
+        });
+        out[k] = v;
+    }
+    return 0;
+}
+
+// The large f classes' path (k_eri_gen<LA, LB, LC, LD>): every bra
+// component pair at run time through eri_bra_runtime, no primitive
+// screening, integrals in (a, b, c, d) order.
+template <int LA, int LB, int LC, int LD>
+static void run_bra_runtime(const double* bra, int nb, const double* ket, int nk, const double* A,
+                            const double* B, const double* C, const double* D, double* out)
+{
+    constexpr int NB = cart_n(LB), NK = cart_n(LC) * cart_n(LD);
+    for (int s = 0; s < cart_n(LA) * NB; ++s) {
+        double acc[NK] = {};
+        eri_bra_runtime<LA, LB, LC, LD>(s / NB, s % NB, (const PrimPair*)bra, nb,
+                                        (const PrimPair*)ket, nk, A, B, C, D, acc, nullptr,
+                                        nullptr, 0.0);
+        for (int k = 0; k < NK; ++k) out[s * NK + k] = acc[k];
+    }
+}
+
+extern "C" int h_eri_bra_runtime(int la, int lb, int lc, int ld, const double* bra, int nb,
+                                 const double* ket, int nk, const double* A, const double* B,
+                                 const double* C, const double* D, double* out)
+{
+#define X(a, b, c, d)                                                          \
+    if (la == a && lb == b && lc == c && ld == d) {                            \
+        run_bra_runtime<a, b, c, d>(bra, nb, ket, nk, A, B, C, D, out);        \
+        return 0;                                                              \
+    }
+    X(3, 1, 2, 2) X(3, 1, 3, 1) X(3, 2, 2, 1) X(3, 2, 2, 2) X(3, 2, 3, 0) X(3, 2, 3, 1)
+    X(3, 2, 3, 2) X(3, 3, 1, 1) X(3, 3, 2, 0) X(3, 3, 2, 1) X(3, 3, 2, 2) X(3, 3, 3, 0)
+    X(3, 3, 3, 1) X(3, 3, 3, 2) X(3, 3, 3, 3) X(2, 1, 1, 0) X(1, 0, 0, 0)
+#undef X
+    return 1;
+}
+
+// The six digestion tiles of one quartet element by element
+// (gen_tile_element, as k_digest_gen), laid out as digest_quartet's t[].
+extern "C" int h_gen_tiles(int la, int lb, int lc, int ld, const double* v, double fac,
+                           const double* P, int nbf, const int* f0, double* t)
+{
+    const GenClass g = gen_class(la, lb, lc, ld);
+    const QuartetMap qm = dense_map(nbf, f0);
+    for (int T = 0; T < 6; ++T) {
+        const int n = (T == 5 ? g.ntile : g.toff[T + 1]) - g.toff[T];
+        for (int e = 0; e < n; ++e) t[g.toff[T] + e] = gen_tile_element(g, T, e, v, 1, fac, P, qm);
+    }
+    return 0;
 }
 
 // The kernels' decomposition: every component of A at once (grad_all_a)

@@ -70,9 +70,10 @@ def test_batch_mixed_one_launch_bitwise_equal_single(batch):
     each bra's n64 within its molecule): the batch equals each molecule's
     own handle bit for bit."""
     mols, bb, singles, P, Ps = batch
-    sh = np.tile(np.array([256, 0, 0], dtype=np.int32), 63).reshape(21, 3, 3)
+    from segfault_py_amd.engine import NUM_QUARTET_CLASSES as NQ
+    sh = np.tile(np.array([256, 0, 0], dtype=np.int32), 3 * NQ).reshape(NQ, 3, 3)
     sh[:, 2, 2] = 1
-    default = np.tile(np.array([256, 0, 0], dtype=np.int32), 63)
+    default = np.tile(np.array([256, 0, 0], dtype=np.int32), 3 * NQ)
     bb.set_tuning(sh)
     J, K = bb.jk(P, 1)
     bb.set_tuning(default)
@@ -127,7 +128,7 @@ def test_single_molecule_graph_and_tuning(gpu, water_ccpvdz):
     P = gpu.tensor(_density(b.nbf, 3), dtype=gpu.float64, device="cuda")
     Jd, Kd = b.jk(P, 1)
     shapes, ms = b.autotune(P, 1)
-    assert shapes.shape == (21, 3, 3) and np.all(shapes[:, :, 0] % 64 == 0)
+    assert shapes.shape == (engine.NUM_QUARTET_CLASSES, 3, 3) and np.all(shapes[:, :, 0] % 64 == 0)
     assert set(np.unique(shapes[:, :, 2])) <= {0, 1} and not shapes[:, :2, 2].any()
     J0, K0 = b.jk(P, 1)
     Jg, Kg = b._zeros(), b._zeros()
@@ -151,11 +152,12 @@ def test_persistent_launch_shapes_bitwise(batch, block, grid):
     """Capped (persistent) grids stride over the quartets; every shape gives
     the default build's J and K bit for bit, FP64 and mixed, split classes
     included."""
+    from segfault_py_amd.engine import NUM_QUARTET_CLASSES as NQ
     mols, bb, singles, P, Ps = batch
-    default = np.tile(np.array([256, 0, 0], dtype=np.int32), 63)
+    default = np.tile(np.array([256, 0, 0], dtype=np.int32), 3 * NQ)
     bb.set_tuning(default)
     ref = [bb.jk(P, p) for p in (0, 1)]
-    bb.set_tuning(np.tile(np.array([block, grid, 0], dtype=np.int32), 63))
+    bb.set_tuning(np.tile(np.array([block, grid, 0], dtype=np.int32), 3 * NQ))
     for p in (0, 1):
         J, K = bb.jk(P, p)
         assert bool((J == ref[p][0]).all()) and bool((K == ref[p][1]).all()), p

@@ -80,7 +80,8 @@ def test_alkane_basis_jk_matches_oracle(gpu, oracle, ch4_alkane):
     Jr, Kr = oracle.jk_dense(eri, Pn)
     b = engine.Basis(mol).screen(1e-12, 1e-5)
     counts = b.counts()[0]
-    assert (counts[:, 0] > 0).all(), "CH4 in this basis covers all 21 classes"
+    nc = engine.NUM_CLASSES_SPD
+    assert (counts[:nc, 0] > 0).all(), "CH4 in this basis covers all 21 s/p/d classes"
     assert counts[:, 2].sum() > 0, "some quartets fall below tau_mixed"
     J, K = b.jk(P, 0)
     assert np.max(np.abs(J.cpu().numpy() - Jr)) < 1e-9

@@ -250,7 +250,8 @@ def test_mixed_one_launch_equals_fp64_with_equal_thresholds(gpu):
     _, P = _density(mol.nbf, 8, "cuda")
     b.set_prim_tau_fp64(1e-15)
     b.set_prim_tau(1e-15)
-    sh = np.tile(np.array([256, 0, 0], dtype=np.int32), 63).reshape(21, 3, 3)
+    nq = engine.NUM_QUARTET_CLASSES
+    sh = np.tile(np.array([256, 0, 0], dtype=np.int32), 3 * nq).reshape(nq, 3, 3)
     b.set_tuning(sh)
     J64, K64 = b.jk(P, 0)
     Jf, Kf = b.jk(P, 1)                    # FP32 arithmetic: close, not equal

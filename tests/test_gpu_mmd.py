@@ -43,7 +43,7 @@ def test_mmd_jk_every_class_matches_rys(gpu):
     s, x = W.water_cluster(3, seed=7, box=5.0)
     mol = Molecule.from_geometry(s, x, W.even_tempered_text(W.BASIS_SPD))
     b = engine.Basis(mol).screen(1e-12, 1e-5)
-    assert (b.counts()[0][:, 0] > 0).all()
+    assert (b.counts()[0][:engine.NUM_CLASSES_SPD, 0] > 0).all()
     P = gpu.tensor(_density(mol.nbf, 9), dtype=gpu.float64, device="cuda")
     J0, K0 = b.jk(P, 0)
     b.set_engine(1)

@@ -47,16 +47,16 @@ def test_argument_errors_are_status_codes(lib):
     assert lib.sf_eri_components(0, i, i, d, d, d, np.zeros(12, dtype=np.int32), np.zeros(12), 0,
                                  i, d) == 1
     assert b"bad arguments" in lib.sf_last_error()
-    # total angular momentum beyond 5 Rys nodes
-    shells = np.array([3, 0, 0] * 4, dtype=np.int32)
+    # total angular momentum beyond 7 Rys nodes (L = 16 > 13)
+    shells = np.array([4, 0, 0] * 4, dtype=np.int32)
     q = np.arange(4, dtype=np.int32)
     rc = lib.sf_eri_components(4, np.ones(4, dtype=np.int32), np.arange(4, dtype=np.int32),
                                np.ones(4), np.ones(4), np.ones(4), shells, np.zeros(12), 1, q,
                                np.zeros(1))
     assert rc == 5
-    # f shells are outside the J/K engine
+    # g shells are outside the J/K engine (s, p, d, f)
     h = ctypes.c_void_p()
-    rc = lib.sf_basis_create(1, np.array([3], dtype=np.int32), np.array([1], dtype=np.int32),
+    rc = lib.sf_basis_create(1, np.array([4], dtype=np.int32), np.array([1], dtype=np.int32),
                              np.zeros(3), np.ones(1), np.ones(1), 1, np.zeros(1, dtype=np.int32),
                              ctypes.byref(h))
     assert rc == 5 and not h.value
@@ -119,6 +119,7 @@ def test_wrappers_check_tensors_before_the_library(monkeypatch):
 def test_batch_and_graph_argument_errors(lib):
     """Bad molecule ranges, tuning shapes and null handles are rejected
     before any device work."""
+    from segfault_py_amd import engine
     h = ctypes.c_void_p()
     L = np.zeros(2, dtype=np.int32)
     npr = np.ones(2, dtype=np.int32)
@@ -137,11 +138,11 @@ def test_batch_and_graph_argument_errors(lib):
     assert lib.sf_basis_create_batch(0, np.zeros(1, dtype=np.int32), 2, L, npr, np.zeros(6),
                                      np.ones(2), np.ones(2), 0, np.zeros(1, dtype=np.int32),
                                      np.zeros(2, dtype=np.int32), ctypes.byref(h)) == 1
-    shapes = np.full(189, 256, dtype=np.int32)
+    shapes = np.full(9 * engine.NUM_QUARTET_CLASSES, 256, dtype=np.int32)
     assert lib.sf_set_tuning(None, shapes) == 1
     assert lib.sf_set_streams(None, 1) == 1
     assert lib.sf_jk_graph(None, None, None, None, 0, None) == 1
-    assert lib.sf_autotune(None, None, 0, None, shapes, np.zeros(63)) == 1
+    assert lib.sf_autotune(None, None, 0, None, shapes, np.zeros(3 * engine.NUM_QUARTET_CLASSES)) == 1
     assert lib.sf_basis_set_layout(None, np.zeros(1, dtype=np.int64), np.zeros(1, dtype=np.int32)) == 1
 
 

@@ -56,18 +56,25 @@ def test_boys_recursions(oracle):
 
 # ------------------------------------------------------------------ Rys ----
 def test_rys_matches_mpmath(oracle):
+    # 1..5 nodes (every quartet up to d shells) within 2e-13 relative; the
+    # 6- and 7-node rules of f-shell quartets lose a little in the smallest
+    # weights (Jacobi eigenvectors of the discretised Stieltjes matrix): 5e-13
     g = _golden("rys_mpmath.json")
-    worst_u = worst_w = 0.0
+    worst = {}
     for pt in g["points"]:
         u, w = oracle.rys_roots(pt["n"], float(pt["X"]))
         ur = np.array([float(v) for v in pt["u"]])
         wr = np.array([float(v) for v in pt["w"]])
-        worst_u = max(worst_u, float(np.max(np.abs(u - ur) / ur)))
-        worst_w = max(worst_w, float(np.max(np.abs(w - wr) / wr)))
-    assert worst_u < 2e-13 and worst_w < 2e-13, (worst_u, worst_w)
+        key = pt["n"] <= 5
+        wu, ww = worst.get(key, (0.0, 0.0))
+        worst[key] = (max(wu, float(np.max(np.abs(u - ur) / ur))),
+                      max(ww, float(np.max(np.abs(w - wr) / wr))))
+    assert set(worst) == {True, False}
+    assert max(worst[True]) < 2e-13, worst
+    assert max(worst[False]) < 5e-13, worst
 
 
-@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 6])
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 6, 7])
 def test_rys_moment_identities(oracle, n):
     for X in (0.0, 1e-9, 0.7, 6.0, 22.0, 58.0, 75.0, 400.0):
         u, w = oracle.rys_roots(n, X)
