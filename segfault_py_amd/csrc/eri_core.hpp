@@ -1057,6 +1057,116 @@ SF_HD SF_INLINE void digest_tile(const double* __restrict__ v, long long vs, dou
                 }
 }
 
+// Doubles one lane holds to digest tile T: its N1 x N2 outputs and the
+// M1 x M2 density block they contract with.
+template <int LA, int LB, int LC, int LD>
+constexpr int tile_lane_doubles(int T)
+{
+    using S = QuartetShape<LA, LB, LC, LD>;
+    const int out = S::tile_n1(T) * S::tile_n2(T);
+    const int dens = T == 0 ? S::NC * S::ND : T == 1 ? S::NA * S::NB : T == 2 ? S::NB * S::ND
+                   : T == 3 ? S::NB * S::NC : T == 4 ? S::NA * S::ND : S::NA * S::NC;
+    return out + dens;
+}
+
+// The six tiles of a split class cut into contiguous groups of at most
+// kDigestBudget doubles per lane (a tile above the budget is a group of its
+// own): one wavefront digests one group in one pass over the integrals, so
+// the raw planes are read once per group instead of once per tile.
+// Measured on the 20-water build (FP64, ms per build): one tile per pass
+// 6.40-6.52, budget 48: 6.32-6.35, 72: 6.35-6.37, 120: 6.07-6.10, 160:
+// 6.08-6.16, 200: 6.19-6.25, 256: 6.14-6.33 (registers spill above 120).
+constexpr int kDigestBudget = 120;
+template <int LA, int LB, int LC, int LD>
+struct TileGroups {
+    struct Cuts {
+        int n;
+        int b[7];
+    };
+    static constexpr Cuts cuts()
+    {
+        Cuts c{0, {0, 0, 0, 0, 0, 0, 0}};
+        int used = 0;
+        for (int T = 0; T < 6; ++T) {
+            const int d = tile_lane_doubles<LA, LB, LC, LD>(T);
+            if (T > 0 && used + d > kDigestBudget) {
+                c.b[++c.n] = T;
+                used = 0;
+            }
+            used += d;
+        }
+        c.b[++c.n] = 6;
+        return c;
+    }
+    static constexpr int count = cuts().n;
+    static constexpr int begin(int g) { return cuts().b[g]; }
+    static constexpr int end(int g) { return cuts().b[g + 1]; }
+};
+
+// Tiles TB..TE-1 of one quartet in one pass over its integrals: each tile
+// adds its terms in the same order and with the same expressions as
+// digest_tile, so the results equal digest_tile's bit for bit.
+template <int LA, int LB, int LC, int LD, int TB, int TE>
+SF_HD SF_INLINE void digest_tile_group(const double* __restrict__ v, long long vs, double fac,
+                                       const double* __restrict__ P, const QuartetMap& qm,
+                                       double* const (&dst)[6])
+{
+    using S = QuartetShape<LA, LB, LC, LD>;
+    constexpr auto aoff = [](int T) {
+        int o = 0;
+        for (int u = TB; u < T; ++u) o += S::tile_n1(u) * S::tile_n2(u);
+        return o;
+    };
+    constexpr auto poff = [](int T) {
+        int o = 0;
+        for (int u = TB; u < T; ++u)
+            o += tile_lane_doubles<LA, LB, LC, LD>(u) - S::tile_n1(u) * S::tile_n2(u);
+        return o;
+    };
+    constexpr int NACC = aoff(TE), NPM = poff(TE);
+    double acc[NACC];
+    double pm[NPM];
+#pragma unroll
+    for (int e = 0; e < NACC; ++e) acc[e] = 0.0;
+    static_for<TB, TE>([&](auto tc) {
+        constexpr int T = decltype(tc)::value;
+        using G = TileGeom<LA, LB, LC, LD, T>;
+        double Pm[G::M1][G::M2];
+        load_tile_density<LA, LB, LC, LD, T>(P, qm, Pm);
+#pragma unroll
+        for (int x = 0; x < G::M1; ++x)
+#pragma unroll
+            for (int y = 0; y < G::M2; ++y) pm[poff(T) + x * G::M2 + y] = Pm[x][y];
+    });
+#pragma unroll
+    for (int a = 0; a < S::NA; ++a)
+#pragma unroll
+        for (int b = 0; b < S::NB; ++b)
+#pragma unroll
+            for (int c = 0; c < S::NC; ++c)
+#pragma unroll
+                for (int d = 0; d < S::ND; ++d) {
+                    const double x = fac * v[(long long)(((a * S::NB + b) * S::NC + c) * S::ND + d) * vs];
+                    static_for<TB, TE>([&](auto tc) {
+                        constexpr int T = decltype(tc)::value;
+                        using G = TileGeom<LA, LB, LC, LD, T>;
+                        constexpr int N2 = G::N2, M2 = G::M2, A0 = aoff(T), P0 = poff(T);
+                        if constexpr (T == 0) acc[A0 + a * N2 + b] += 2.0 * x * pm[P0 + c * M2 + d];
+                        else if constexpr (T == 1) acc[A0 + c * N2 + d] += 2.0 * x * pm[P0 + a * M2 + b];
+                        else if constexpr (T == 2) acc[A0 + a * N2 + c] += x * pm[P0 + b * M2 + d];
+                        else if constexpr (T == 3) acc[A0 + a * N2 + d] += x * pm[P0 + b * M2 + c];
+                        else if constexpr (T == 4) acc[A0 + b * N2 + c] += x * pm[P0 + a * M2 + d];
+                        else acc[A0 + b * N2 + d] += x * pm[P0 + a * M2 + c];
+                    });
+                }
+    static_for<TB, TE>([&](auto tc) {
+        constexpr int T = decltype(tc)::value;
+        constexpr int NE = S::tile_n1(T) * S::tile_n2(T), A0 = aoff(T);
+#pragma unroll
+        for (int e = 0; e < NE; ++e) dst[T][e] = acc[A0 + e];
+    });
+}
+
 // J/K digestion of one shell quartet.  v[NINT] are the integrals in the
 // shell-normalised basis times the degeneracy factor; P is the density in
 // the same basis (row-major, ld = nbf).  Writes tiles[NTILE]:

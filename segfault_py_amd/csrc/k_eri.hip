@@ -314,7 +314,8 @@ __global__ __launch_bounds__(256) void k_digest_split(const ClassParams cp,
                                                       const double* __restrict__ raw, int rows)
 {
     using TR = TileRows<LA, LB, LC, LD>;
-    const int per = rows ? TR::total : 6;
+    using TG = TileGroups<LA, LB, LC, LD>;
+    const int per = rows ? TR::total : TG::count;
     const long long nwork = ((cp.nq + 63) >> 6) * per;
     const long long wstride = ((long long)gridDim.x * blockDim.x) >> 6;
     for (long long gw = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6; gw < nwork;
@@ -336,24 +337,29 @@ __global__ __launch_bounds__(256) void k_digest_split(const ClassParams cp,
     const double fac = ((a == b) ? 0.5 : 1.0) * ((c == d) ? 0.5 : 1.0) *
                        ((cp.same && i == j) ? 0.5 : 1.0);
     const QuartetMap qm = quartet_map(cp.map, a, b, c, d);
-    static_switch<0, 6>(t, [&](auto tc) {
-        constexpr int T = decltype(tc)::value;
-        using G = TileGeom<LA, LB, LC, LD, T>;
-        double* dst = cp.tiles + cp.pos[T * cp.qstride + qa];
-        if (rows) {
+    if (rows) {
+        static_switch<0, 6>(t, [&](auto tc) {
+            constexpr int T = decltype(tc)::value;
+            using G = TileGeom<LA, LB, LC, LD, T>;
+            double* dst = cp.tiles + cp.pos[T * cp.qstride + qa];
             double Pm[G::M1][G::M2];
             load_tile_density<LA, LB, LC, LD, T>(cp.P, qm, Pm);
             double row[G::N2];
             digest_tile_row<LA, LB, LC, LD, T>(raw + qa, cp.qstride, fac, Pm, r, row);
 #pragma unroll
             for (int e = 0; e < G::N2; ++e) dst[r * G::N2 + e] = row[e];
-        } else {
-            double tile[G::N1 * G::N2];
-            digest_tile<LA, LB, LC, LD, T>(raw + qa, cp.qstride, fac, cp.P, qm, tile);
+        });
+    } else {
+        // w = the tile group
+        static_switch<0, TG::count>(w, [&](auto gc) {
+            constexpr int g = decltype(gc)::value;
+            double* dst[6];
 #pragma unroll
-            for (int e = 0; e < G::N1 * G::N2; ++e) dst[e] = tile[e];
-        }
-    });
+            for (int T = 0; T < 6; ++T) dst[T] = cp.tiles + cp.pos[T * cp.qstride + qa];
+            digest_tile_group<LA, LB, LC, LD, TG::begin(g), TG::end(g)>(raw + qa, cp.qstride, fac,
+                                                                        cp.P, qm, dst);
+        });
+    }
     }
 }
 
@@ -498,7 +504,9 @@ static int launch_split(int fp32, int mode, const ClassParams& cp, const Tune& t
     // digestion by tile rows when by tiles would give fewer than 1536
     // wavefronts (6 per CU): measured faster below, slower above
     const int rows = (waves * 6 < 1536) ? 1 : 0;
-    SF_TRY_K(grid_blocks(waves * (rows ? TileRows<LA, LB, LC, LD>::total : 6), wpb, tn, &nb2));
+    SF_TRY_K(grid_blocks(waves * (rows ? TileRows<LA, LB, LC, LD>::total
+                                       : TileGroups<LA, LB, LC, LD>::count),
+                         wpb, tn, &nb2));
     double* raw = (mode == kModeRaw) ? cp.out : cp.raw;
     if (!raw) {
         set_error(kEState, "split class without a raw buffer");
@@ -592,7 +600,9 @@ static int launch_mmd_t(int mode, const ClassParams& cp, const Tune& tn, hipStre
         unsigned nb2 = 0;
         const long long waves = (cp.nq + 63) / 64;
         const int rows = (waves * 6 < 1536) ? 1 : 0;
-        SF_TRY_K(grid_blocks(waves * (rows ? TileRows<LA, LB, LC, LD>::total : 6), 4, Tune(), &nb2));
+        SF_TRY_K(grid_blocks(waves * (rows ? TileRows<LA, LB, LC, LD>::total
+                                           : TileGroups<LA, LB, LC, LD>::count),
+                             4, Tune(), &nb2));
         hipLaunchKernelGGL((k_digest_split<LA, LB, LC, LD>), dim3(nb2), dim3(256), 0, s, cp,
                            (const double*)raw, rows);
         SF_HIP(hipGetLastError());

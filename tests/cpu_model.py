@@ -35,6 +35,7 @@ def harness():
         H.h_digest_tiles.argtypes = [ctypes.c_int] * 4 + [_f64p, ctypes.c_double, _f64p,
                                                           ctypes.c_int, _i32p, _f64p]
         H.h_digest_tile_rows.argtypes = H.h_digest_tiles.argtypes
+        H.h_digest_tile_groups.argtypes = H.h_digest_tiles.argtypes
         H.h_rys.argtypes = [ctypes.c_int, ctypes.c_double, _f64p, _f64p]
         H.h_boys.argtypes = [ctypes.c_int, ctypes.c_double, _f64p]
         H.h_boys01.argtypes = [ctypes.c_double, _f64p]

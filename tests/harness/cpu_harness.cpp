@@ -142,6 +142,23 @@ static void run_tiles(const double* v, double fac, const double* P, int nbf, con
     digest_tile<LA, LB, LC, LD, 5>(v, 1, fac, P, qm, t + S::T_BD);
 }
 
+// The same six tiles digested group by group (digest_tile_group over the
+// TileGroups cuts, the split-class digestion's tile mode).
+template <int LA, int LB, int LC, int LD>
+static void run_tile_groups(const double* v, double fac, const double* P, int nbf, const int* f0,
+                            double* t)
+{
+    using S = QuartetShape<LA, LB, LC, LD>;
+    using TG = TileGroups<LA, LB, LC, LD>;
+    const QuartetMap qm = dense_map(nbf, f0);
+    double* const dst[6] = {t + S::T_AB, t + S::T_CD, t + S::T_AC,
+                            t + S::T_AD, t + S::T_BC, t + S::T_BD};
+    static_for<0, TG::count>([&](auto gc) {
+        constexpr int g = decltype(gc)::value;
+        digest_tile_group<LA, LB, LC, LD, TG::begin(g), TG::end(g)>(v, 1, fac, P, qm, dst);
+    });
+}
+
 // The same six tiles built row by row (digest_tile_row, the split-class
 // digestion's row mode).
 template <int LA, int LB, int LC, int LD, int T>
@@ -241,6 +258,19 @@ extern "C" int h_digest_tiles(int la, int lb, int lc, int ld, const double* v, d
     if (la == a && lb == b && lc == c && ld == d) {        \
         run_tiles<a, b, c, d>(v, fac, P, nbf, f0, t);      \
         return 0;                                          \
+    }
+    SF_CLASSES(X)
+#undef X
+    return 1;
+}
+
+extern "C" int h_digest_tile_groups(int la, int lb, int lc, int ld, const double* v, double fac,
+                                    const double* P, int nbf, const int* f0, double* t)
+{
+#define X(a, b, c, d)                                        \
+    if (la == a && lb == b && lc == c && ld == d) {          \
+        run_tile_groups<a, b, c, d>(v, fac, P, nbf, f0, t);  \
+        return 0;                                            \
     }
     SF_CLASSES(X)
 #undef X

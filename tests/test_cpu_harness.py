@@ -217,7 +217,8 @@ def test_digest_tiles_equal_digest_quartet():
     P = rng.standard_normal((nbf, nbf))
     P = np.ascontiguousarray(P + P.T)
     H = CM.harness()
-    for ls in [(0, 0, 0, 0), (1, 0, 1, 0), (2, 1, 1, 0), (2, 2, 2, 1), (2, 2, 2, 2), (1, 1, 1, 1)]:
+    for ls in [(0, 0, 0, 0), (1, 0, 1, 0), (2, 1, 1, 0), (2, 2, 2, 1), (2, 2, 2, 2), (1, 1, 1, 1),
+               (2, 1, 2, 0), (2, 1, 1, 1), (2, 2, 1, 0), (2, 2, 2, 0), (2, 1, 2, 1), (2, 2, 1, 1)]:
         nA, nB, nC, nD = [CM.ncart(l) for l in ls]
         v = rng.standard_normal(nA * nB * nC * nD)
         f0 = np.array([0, 7, 15, 30], dtype=np.int32)
@@ -230,6 +231,10 @@ def test_digest_tiles_equal_digest_quartet():
         t3 = np.zeros(nt)
         assert H.h_digest_tile_rows(*ls, v, 0.25, P, nbf, f0, t3) == 0
         assert np.array_equal(t2, t3)
+        # and its tile mode: groups of tiles in one pass over the integrals
+        t4 = np.zeros(nt)
+        assert H.h_digest_tile_groups(*ls, v, 0.25, P, nbf, f0, t4) == 0
+        assert np.array_equal(t2, t4), ls
 
 
 @pytest.mark.parametrize("fixture", ["water_sp", "water_ccpvdz"])
