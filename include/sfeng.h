@@ -229,7 +229,8 @@ int sf_set_tuning(sf_basis *b, const int *shapes);
  * 0..20 (status 5 for an f-shell class; qcls = -1 leaves those on Rys).
  * Used by sf_jk, sf_jk_graph, sf_autotune and sf_eri_dense. */
 int sf_set_engine(sf_basis *b, int qcls, int engine);
-/* Side streams of sf_jk / sf_jk_graph (0..3; default 1): the quartet-class
+/* Side streams of sf_jk / sf_jk_graph (0..3; default 2 for sf_jk and 3 for
+ * the build sf_jk_graph captures, the fastest of each measured): the quartet-class
  * launches, and then the per-tile-shape reductions, are spread over the
  * caller's stream and nside private streams (forked from and joined back
  * into the caller's stream with events), so small classes run beside large

@@ -246,12 +246,15 @@ struct sf_basis {
     hipStream_t cap_stream = nullptr;
     // concurrent builds: the class launches and the per-shape reductions of
     // sf_jk / sf_jk_graph are spread over the caller's stream and `nside`
-    // side streams (0 = all on the caller's stream), forked from and joined
-    // back into the caller's stream with events.  One side stream measured
-    // best for eager builds of 20 waters (FP64 7.05 -> 6.35 ms, mixed 6.97
-    // -> 6.16 ms; three: 6.52 / 6.32 eager, 6.24 / 6.06 as a graph)
+    // (eager builds) / `nside_graph` (captured builds) side streams (0 = all
+    // on the caller's stream), forked from and joined back into the
+    // caller's stream with events.  Measured on 20 waters (FP64 / mixed ms
+    // per build, eager | graph): one side stream 6.27 / 6.16 | 6.25 / 6.13,
+    // two 6.04 / 5.92 | 6.41 / 6.39, three 6.26 / 6.03 | 6.11 / 5.92 (no
+    // side stream: 6.74 FP64 eager).
     static constexpr int kMaxSide = 3;
-    int nside = 1;
+    int nside = 2;
+    int nside_graph = 3;
     // per class and precision, its launches' time in one build (ms; 0 =
     // unknown), from sf_autotune's estimates or a sf_jk_profile build: the
     // cost the launches are balanced over the streams by
@@ -271,7 +274,8 @@ struct sf_basis {
     }
     int ensure_side()
     {
-        for (int k = 0; k < nside; ++k) {
+        // every side stream either build path may use (nside, nside_graph)
+        for (int k = 0; k < std::max(nside, nside_graph); ++k) {
             if (!side[k]) SF_HIP(hipStreamCreateWithFlags(&side[k], hipStreamNonBlocking));
             if (!ev_join[k]) SF_HIP(hipEventCreateWithFlags(&ev_join[k], hipEventDisableTiming));
         }
@@ -1410,6 +1414,7 @@ extern "C" int sf_set_streams(sf_basis* b, int nside)
         return kEInval;
     }
     b->nside = nside;
+    b->nside_graph = nside;
     b->drop_graph();
     return kOk;
 }
@@ -1602,7 +1607,7 @@ static int check_jk_args(sf_basis* b, const double* P, double* J, double* K, int
 // of quartet class c (c = 0..54), [kSlotScale] density scaling,
 // [kSlotReduce] plan reduction + finalisation.
 static int run_jk(sf_basis* b, const double* P, double* J, double* K, int precision,
-                  hipStream_t s, double* ms)
+                  hipStream_t s, double* ms, bool capture = false)
 {
     SF_TRY(check_jk_args(b, P, J, K, precision));
     const MolLayout ml = b->layout();
@@ -1623,7 +1628,7 @@ static int run_jk(sf_basis* b, const double* P, double* J, double* K, int precis
     // concurrent launches (not when timing launch groups): the classes write
     // disjoint tile slots and raw planes, the reductions disjoint blocks of
     // X and Y, so each group can go to any stream between a fork and a join
-    const int nside = ms ? 0 : b->nside;
+    const int nside = ms ? 0 : capture ? b->nside_graph : b->nside;
     if (nside > 0) SF_TRY(b->ensure_side());
     hipStream_t st[1 + sf_basis::kMaxSide] = {s};
     for (int k = 0; k < nside; ++k) st[1 + k] = b->side[k];
@@ -1790,7 +1795,7 @@ extern "C" int sf_jk_graph(sf_basis* b, const double* P, double* J, double* K, i
         b->drop_graph();
         if (!b->cap_stream) SF_HIP(hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking));
         SF_HIP(hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal));
-        const int rc = run_jk(b, P, J, K, precision, b->cap_stream, nullptr);
+        const int rc = run_jk(b, P, J, K, precision, b->cap_stream, nullptr, true);
         hipGraph_t graph = nullptr;
         const hipError_t e = hipStreamEndCapture(b->cap_stream, &graph);
         if (rc != kOk) {

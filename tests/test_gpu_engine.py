@@ -166,9 +166,20 @@ def test_jk_streams_bitwise(gpu):
     mol = W.config_molecule(3)  # 10 waters, N = 220
     b = engine.Basis(mol).screen(1e-12, 1e-5)
     _, P = _density(mol.nbf, 7, "cuda")
+    # the defaults (two side streams for eager builds, three for the
+    # captured graph) before any set_streams call
+    dflt = {prec: b.jk(P, prec) for prec in (0, 1)}
+    dflt_g = {}
+    for prec in (0, 1):
+        Jg, Kg = gpu.empty_like(P), gpu.empty_like(P)
+        b.jk_graph(P, (Jg, Kg), prec)
+        gpu.cuda.synchronize()
+        dflt_g[prec] = (Jg, Kg)
     for prec in (0, 1):
         b.set_streams(0)
         J0, K0 = b.jk(P, prec)
+        assert gpu.equal(dflt[prec][0], J0) and gpu.equal(dflt[prec][1], K0), prec
+        assert gpu.equal(dflt_g[prec][0], J0) and gpu.equal(dflt_g[prec][1], K0), prec
         for ns in (1, 2, 3):
             b.set_streams(ns)
             J, K = b.jk(P, prec)
