@@ -146,10 +146,14 @@ int sf_screen(sf_basis *b, double tau, double tau_mixed);
  * 0 disables it. */
 int sf_set_prim_tau(sf_basis *b, double tau_prim);
 /* Primitive screening of the FP64 quartets (no reference counterpart): a
- * primitive quartet with Qp_ab Qp_cd < tau_prim is skipped.  Default
- * (tau_prim < 0): 1e-3 * tau, so the primitive terms dropped from one
- * quartet stay below the shell-level Schwarz threshold; tau = 0 (no
- * screening) therefore skips nothing.  0 disables it. */
+ * primitive quartet with Qp_ab Qp_cd < tau_prim is skipped; each skipped
+ * term is bounded by that product (Cauchy-Schwarz per primitive quartet),
+ * so a quartet of np_ab x np_cd primitive pairs drops at most np_ab np_cd
+ * tau_prim in magnitude.  Default (tau_prim < 0): 1e-3 * tau -- one skipped
+ * term a thousandth of the shell-level Schwarz threshold (the sum stays
+ * below tau for up to 1000 primitive quartets per quartet, e.g. every
+ * quartet of 3-primitive s/p shells; contracted cc-pVDZ s shells can have
+ * more); tau = 0 (no screening) therefore skips nothing.  0 disables it. */
 int sf_set_prim_tau_fp64(sf_basis *b, double tau_prim);
 /* counts[3*c + {0,1,2}] = surviving / FP64 / FP32 shell quartets of
  * quartet class c (0..54: classes 0..20 hold s, p, d shells, 21..54 an f
