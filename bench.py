@@ -59,6 +59,9 @@ def parse():
                     help="autotune, then only W + K J/K builds (for kernel tracing)")
     ap.add_argument("--precision", type=int, default=0,
                     help="precision of the --builds-only builds (0 FP64, 1 mixed)")
+    ap.add_argument("--streams", type=int, default=-1,
+                    help="side streams of the --builds-only builds (-1: the library's default; "
+                         "0: every launch in order, so a kernel trace shows each kernel alone)")
     ap.add_argument("--only-batch", action="store_true",
                     help="run only the batched SCF workload and print its JSON")
     ap.add_argument("--only-published", action="store_true",
@@ -214,6 +217,8 @@ def main():
         # W warm-up and K timed FP64 builds, nothing else: each build starts
         # with the density-scaling kernel, which splits a kernel trace per build
         prec = args.precision
+        if args.streams >= 0:
+            basis.set_streams(args.streams)
         basis.autotune(P, prec)
         for _ in range(args.warmup):
             basis.jk(P, prec, out=(J, K))
@@ -224,7 +229,8 @@ def main():
         torch.cuda.synchronize()
         ms = (time.perf_counter() - t) / args.steps * 1e3
         print(json.dumps({"builds_only": {"workload": mol.nbf, "precision": prec,
-                                          "ms_per_build": ms, "builds": args.steps}}))
+                                          "streams": args.streams, "ms_per_build": ms,
+                                          "builds": args.steps}}))
         return
     # flops count the primitive quartets the FP64 kernels evaluate
     work = metrics.class_work(basis, mol, evaluated=basis.prim_quartet_counts(0))
