@@ -273,6 +273,20 @@ def main():
     kernel_ms = {"eri_" + "".join(map(str, w["ls"])): round(t, 4) for t, w in top}
     kernel_ms["reduce+finalize"] = round(float(ms[engine.SLOT_REDUCE]), 4)
     kernel_ms["scale_density"] = round(float(ms[engine.SLOT_SCALE]), 4)
+    # ceiling of the mixed build: its in-order launch groups with the FP32
+    # segments' groups (the FP32-arithmetic launches, and for split classes
+    # their shared digestion launch) taken as free, against the in-order
+    # FP64 build
+    _, _, ms_mix = basis.jk_profile(P, 1, out=(J, K))
+    fp32_groups = float(sum(ms_mix[2 * c + 1] for c in range(engine.NUM_QUARTET_CLASSES)))
+    mixed_bound = {"speedup_bound": float(sum(ms)) / max(1e-9, float(sum(ms_mix)) - fp32_groups),
+                   "fp64_inorder_ms": round(float(sum(ms)), 4),
+                   "mixed_inorder_ms": round(float(sum(ms_mix)), 4),
+                   "fp32_segment_groups_ms": round(fp32_groups, 4),
+                   "note": "Amdahl bound: in-order FP64 build / (in-order mixed build minus the "
+                           "FP32 segments' launch groups), i.e. the mixed speed-up if the "
+                           "FP32-arithmetic launches cost nothing; a loose ceiling, since those "
+                           "launches also do the FP64-pinned node, prefactor and digestion work"}
     # the McMurchie-Davidson engine (MFMA E.R.E) on the same build, per class
     basis.set_engine(1)
     basis.jk_profile(P, 0, out=(J, K))
@@ -315,6 +329,7 @@ def main():
         "fp64_builds_per_s": 1.0 / t64,
         "mixed_builds_per_s": 1.0 / tmix,
         "mixed_speedup": t64 / tmix,
+        "mixed_speedup_bound": mixed_bound,
         "dE_2e_mixed_vs_fp64": abs(Emix - E64),
         "mixed_fp64_arith_classes": [int(c) for c in np.nonzero(tuned[:, 2, 2])[0]],
         "mixed_fp32_arith_only": {"builds_per_s": 1.0 / tmix32, "speedup": t64 / tmix32,
