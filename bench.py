@@ -233,7 +233,8 @@ def main():
                                           "builds": args.steps}}))
         return
     # flops count the primitive quartets the FP64 kernels evaluate
-    work = metrics.class_work(basis, mol, evaluated=basis.prim_quartet_counts(0))
+    work = metrics.class_work(basis, mol, evaluated=basis.prim_quartet_counts(0),
+                              far=basis.far_prim_quartet_counts(0))
 
     basis.autotune(P, 0)
     tuned = basis.autotune(P, 1)[0]      # shapes of every slot after both tunings

@@ -166,6 +166,13 @@ int sf_quartet_counts(const sf_basis *b, long long *counts, long long *funcq,
  * the whole class (precision 0) or its FP64 segment, seg 1 = its FP32
  * segment (0 for precision 0).  Runs a counting kernel on the null stream. */
 int sf_prim_quartet_counts(sf_basis *b, int precision, long long *counts);
+/* The part of those primitive quartets that lies in far quartets (no
+ * reference counterpart; a measurement aid): the (ss|ss) and (ps|ss)
+ * quartets whose every primitive quartet has X = p q/(p+q) |P-Q|^2 >= 40,
+ * which the library evaluates in the point-charge form S_b S_k / |P-Q|
+ * (and its dipole term) instead of the Boys function; same layout as
+ * sf_prim_quartet_counts, 0 for the other classes. */
+int sf_far_prim_quartet_counts(sf_basis *b, int precision, long long *counts);
 /* Sorted significant pairs of pair class pc (0..9: ss ps pp ds dp dd fs
  * fp fd ff): shells, Q and the
  * number of primitive pairs kept after primitive-pair screening (any output

@@ -88,6 +88,40 @@ SF_HD SF_INLINE PrimPair load_prim(Ptr ptr, int i)
     return r;
 }
 
+// The primitive pairs of the ket pair lists transposed (one array per pair
+// class, the kernels' per-lane ket reads): field f of primitive k of sorted
+// pair j at t[(f * kmax + k) * n + j], fields p Px Py Pz ip Kp, then the
+// Schwarz factor Qp and the charge S of each primitive pair.  Lanes holding
+// consecutive kets read consecutive doubles (one cache line per 8 lanes,
+// instead of one per lane from 64-byte records).  K and eb are not stored
+// (the J/K kernels do not read them); the view returns them as NaN so that
+// a use would show.
+constexpr int kKetFields = 8;
+constexpr int kKetQ = 6, kKetS = 7;
+struct KetView {
+    const double* t;  // array + j
+    long long fs, n;  // field stride kmax * n, primitive stride n
+    SF_HD SF_INLINE PrimPair operator[](int k) const
+    {
+        const double* e = t + (long long)k * n;
+        PrimPair r;
+        r.p = e[0];
+        r.Px = e[fs];
+        r.Py = e[2 * fs];
+        r.Pz = e[3 * fs];
+        r.ip = e[4 * fs];
+        r.Kp = e[5 * fs];
+        r.K = __builtin_nan("");
+        r.eb = __builtin_nan("");
+        return r;
+    }
+};
+struct StridedView {
+    const double* t;  // array + field offset + j
+    long long n;
+    SF_HD SF_INLINE double operator[](int k) const { return t[(long long)k * n]; }
+};
+
 // Where the four shells (a, b, c, d) of a quartet sit in an N x N matrix of
 // their molecule: component x of shell k and component y of shell k' are at
 // row[k] + x * ld + col[k'] + y.  For one molecule of nbf functions,
@@ -169,10 +203,11 @@ SF_HD SF_INLINE void one_node_term(const PrimPair& pb, const PrimPair& pk, const
 // The one-node classes' primitive loops (ket outer, bra inner, as in
 // eri_shell_quartet).  Two bra pairs per iteration, or prefetching the next
 // pair, measured slower: the screened loops often stop early.
-template <int LA, typename R, bool PS, typename BraPtr, typename BqPtr>
-SF_HD SF_INLINE void eri_one_node(BraPtr bra, int nbra, const PrimPair* __restrict__ ket, int nket,
+template <int LA, typename R, bool PS, typename BraPtr, typename BqPtr, typename KetPtr,
+          typename KqPtr>
+SF_HD SF_INLINE void eri_one_node(BraPtr bra, int nbra, KetPtr ket, int nket,
                                   const double* A, R* __restrict__ acc, const NodeTable& nt,
-                                  BqPtr bq, const double* __restrict__ kq, double tau_prim)
+                                  BqPtr bq, KqPtr kq, double tau_prim)
 {
     constexpr int NV = (LA == 0) ? 1 : 3;
     for (int ik = 0; ik < nket; ++ik) {
@@ -194,6 +229,46 @@ SF_HD SF_INLINE void eri_one_node(BraPtr bra, int nbra, const PrimPair* __restri
     }
 }
 
+// The one-node classes on far quartets (every primitive quartet has
+// X >= kFarX, where F_0 = sqrt(pi / (4X)) and F_1 = F_0 / (2X)): with
+// S = pi^{3/2} K p^{-3/2} per primitive pair (bs / ks) and R = |P - Q|,
+//   pref F_0 = S_b S_k / R,
+//   pref (PA_k F_0 - q/(p+q) PQ_k F_1) = (S_b S_k / R) (PA_k - PQ_k / (2 p R^2)),
+// the interaction of two point charges and of a charge with a dipole term:
+// no Boys table, no 1/sqrt(p+q).  Same loops and primitive screening as
+// eri_one_node.
+template <int LA, typename R, bool PS, typename BraPtr, typename BsPtr, typename BqPtr,
+          typename KetPtr, typename KsPtr, typename KqPtr>
+SF_HD SF_INLINE void eri_one_node_far(BraPtr bra, BsPtr bs, int nbra, KetPtr ket, KsPtr ks,
+                                      int nket, const double* A, R* __restrict__ acc, BqPtr bq,
+                                      KqPtr kq, double tau_prim)
+{
+    for (int ik = 0; ik < nket; ++ik) {
+        double qk = 0.0;
+        if constexpr (PS) {
+            qk = kq[ik];
+            if (nbra == 0 || qk * bq[0] < tau_prim) break;
+        }
+        const double Qx = ket[ik].Px, Qy = ket[ik].Py, Qz = ket[ik].Pz, sk = ks[ik];
+        for (int ib = 0; ib < nbra; ++ib) {
+            if constexpr (PS) {
+                if (bq[ib] * qk < tau_prim) break;
+            }
+            const double PQ[3] = {bra[ib].Px - Qx, bra[ib].Py - Qy, bra[ib].Pz - Qz};
+            const double ir = fast_rsqrt(PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]);
+            const double v0 = bs[ib] * sk * ir;
+            if constexpr (LA == 0) {
+                acc[0] += (R)v0;
+            } else {
+                const double c = 0.5 * bra[ib].ip * ir * ir;
+                const double PA[3] = {bra[ib].Px - A[0], bra[ib].Py - A[1], bra[ib].Pz - A[2]};
+#pragma unroll
+                for (int k = 0; k < 3; ++k) acc[k] += (R)(v0 * (PA[k] - c * PQ[k]));
+            }
+        }
+    }
+}
+
 // Accumulate (ab|cd) for all Cartesian components into acc[NINT] (not
 // zeroed here).  Bra primitive pairs bra[0..nbra), ket pairs ket[0..nket).
 // R is the type of the recurrences and of acc; X, nodes and weights are
@@ -205,21 +280,21 @@ SF_HD SF_INLINE void eri_one_node(BraPtr bra, int nbra, const PrimPair* __restri
 // factors, sorted descending within each shell pair, and primitive quartets
 // with bq * kq < tau_prim are skipped (both loops stop at the first one).
 template <int LA, int LB, int LC, int LD, typename R, bool PS = false,
-          typename BraPtr = const PrimPair*, typename BqPtr = const double*>
-SF_HD SF_INLINE void eri_shell_quartet(BraPtr bra, int nbra,
-                                       const PrimPair* __restrict__ ket, int nket,
+          typename BraPtr = const PrimPair*, typename BqPtr = const double*,
+          typename KetPtr = const PrimPair*, typename KqPtr = const double*>
+SF_HD SF_INLINE void eri_shell_quartet(BraPtr bra, int nbra, KetPtr ket, int nket,
                                        const double* A, const double* B, const double* C,
                                        const double* D, R* __restrict__ acc,
                                        const NodeTable& nt = global_node_table<QuartetShape<LA, LB, LC, LD>::NR>(),
-                                       BqPtr bq = nullptr,
-                                       const double* __restrict__ kq = nullptr,
+                                       BqPtr bq = nullptr, KqPtr kq = nullptr,
                                        double tau_prim = 0.0)
 {
     using S = QuartetShape<LA, LB, LC, LD>;
     constexpr int NR = S::NR, NAB = S::NAB, NCD = S::NCD;
     if constexpr (S::L <= 1) {
         static_assert(LB == 0 && LC == 0 && LD == 0, "the one-node classes are (ss|ss) and (ps|ss)");
-        eri_one_node<LA, R, PS>(bra, nbra, ket, nket, A, acc, nt, bq, kq, tau_prim);
+        eri_one_node<LA, R, PS, BraPtr, BqPtr, KetPtr, KqPtr>(bra, nbra, ket, nket, A, acc, nt, bq,
+                                                              kq, tau_prim);
         return;
     }
     R ABp[3][LB + 1], CDp[3][LD + 1];
@@ -402,14 +477,15 @@ SF_HD SF_INLINE void dir_component(R g00, R c00, R c00p, R b10, R b01, R b00, R 
 // (fa fb | fc fd) for ONE bra component pair (FA of shell A, FB of shell B)
 // and all ket components, accumulated into acc[NC*ND].  Used where the full
 // class does not fit in registers: one wavefront per bra component pair.
-template <int LA, int LB, int LC, int LD, int FA, int FB, typename R, bool PS = false>
+template <int LA, int LB, int LC, int LD, int FA, int FB, typename R, bool PS = false,
+          typename KetPtr = const PrimPair*, typename KqPtr = const double*>
 SF_HD SF_INLINE void eri_bra_component(const PrimPair* __restrict__ bra, int nbra,
-                                       const PrimPair* __restrict__ ket, int nket,
+                                       KetPtr ket, int nket,
                                        const double* A, const double* B, const double* C,
                                        const double* D, R* __restrict__ acc,
                                        const NodeTable& nt = global_node_table<QuartetShape<LA, LB, LC, LD>::NR>(),
                                        const double* __restrict__ bq = nullptr,
-                                       const double* __restrict__ kq = nullptr,
+                                       KqPtr kq = nullptr,
                                        double tau_prim = 0.0)
 {
     using S = QuartetShape<LA, LB, LC, LD>;
@@ -548,14 +624,15 @@ SF_HD SF_INLINE void dir_row(R g00, R c00, R c00p, R b10, R b01, R b00, R ab, co
 // (index (fb*NC + fc)*ND + fd).  The same sums as eri_bra_component<FA, fb>
 // for each fb, with the nodes, the 2-D recurrence and the ket transfer
 // evaluated once per FA instead of once per (FA, fb).
-template <int LA, int LB, int LC, int LD, int FA, typename R, bool PS = false>
+template <int LA, int LB, int LC, int LD, int FA, typename R, bool PS = false,
+          typename KetPtr = const PrimPair*, typename KqPtr = const double*>
 SF_HD SF_INLINE void eri_bra_row(const PrimPair* __restrict__ bra, int nbra,
-                                 const PrimPair* __restrict__ ket, int nket,
+                                 KetPtr ket, int nket,
                                  const double* A, const double* B, const double* C,
                                  const double* D, R* __restrict__ acc,
                                  const NodeTable& nt = global_node_table<QuartetShape<LA, LB, LC, LD>::NR>(),
                                  const double* __restrict__ bq = nullptr,
-                                 const double* __restrict__ kq = nullptr,
+                                 KqPtr kq = nullptr,
                                  double tau_prim = 0.0)
 {
     using S = QuartetShape<LA, LB, LC, LD>;

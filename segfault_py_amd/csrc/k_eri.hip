@@ -112,8 +112,13 @@ __device__ __forceinline__ void class_quartet(const ClassParams& cp, const NodeT
     const int c = cp.ket.sa[j], d = cp.ket.sb[j];
     const double A[3] = {cp.shell_xyz[3 * a], cp.shell_xyz[3 * a + 1], cp.shell_xyz[3 * a + 2]};
     const double B[3] = {cp.shell_xyz[3 * b], cp.shell_xyz[3 * b + 1], cp.shell_xyz[3 * b + 2]};
-    const double C[3] = {cp.shell_xyz[3 * c], cp.shell_xyz[3 * c + 1], cp.shell_xyz[3 * c + 2]};
-    const double D[3] = {cp.shell_xyz[3 * d], cp.shell_xyz[3 * d + 1], cp.shell_xyz[3 * d + 2]};
+    // the ket's centres and primitive pairs from the transposed arrays:
+    // consecutive lanes (consecutive kets j) read consecutive doubles
+    const long long kn = cp.kt_n;
+    const double C[3] = {cp.kc[j], cp.kc[kn + j], cp.kc[2 * kn + j]};
+    const double D[3] = {cp.kc[3 * kn + j], cp.kc[4 * kn + j], cp.kc[5 * kn + j]};
+    const KetView kv{cp.kt + j, cp.kt_fs, kn};
+    const StridedView kqv{cp.kt + kKetQ * cp.kt_fs + j, kn};
     R acc[S::NINT];
 #pragma unroll
     for (int k = 0; k < S::NINT; ++k) acc[k] = (R)0;
@@ -124,6 +129,10 @@ __device__ __forceinline__ void class_quartet(const ClassParams& cp, const NodeT
     double tau = cp.tau_prim;
     if (cp.n64 && j - cp.ket0[i] >= cp.n64[i]) tau = cp.tau_prim_lo;
     int bpo = cp.bra.poff[i], bnp = cp.bra.np[i];
+    // one-node classes, far quartets: the point-charge form
+    constexpr bool far = MODE == kModeJKFar;
+    static_assert(!far || S::L <= 1, "the point-charge form is for one-node classes");
+    const int knp = cp.ket.np[j];
     if constexpr (UNI) {
         // the bra's primitive pairs are read-only for the whole launch: read
         // them through the constant address space, i.e. with scalar loads
@@ -131,13 +140,28 @@ __device__ __forceinline__ void class_quartet(const ClassParams& cp, const NodeT
         bnp = __builtin_amdgcn_readfirstlane(bnp);
         using CPrim = const __attribute__((address_space(4))) PrimPair*;
         using CDbl = const __attribute__((address_space(4))) double*;
-        eri_shell_quartet<LA, LB, LC, LD, R, PS, CPrim, CDbl>(
-            (CPrim)(cp.prim + bpo), bnp, cp.prim + cp.ket.poff[j], cp.ket.np[j], A, B, C, D, acc,
-            nt, (CDbl)(cp.primq + bpo), cp.primq + cp.ket.poff[j], tau);
+        if constexpr (far) {
+            if constexpr (S::L <= 1)
+                eri_one_node_far<LA, R, PS, CPrim, CDbl, CDbl>(
+                    (CPrim)(cp.prim + bpo), (CDbl)(cp.prims + bpo), bnp, kv,
+                    StridedView{cp.kt + kKetS * cp.kt_fs + j, kn}, knp, A, acc,
+                    (CDbl)(cp.primq + bpo), kqv, tau);
+        } else {
+            eri_shell_quartet<LA, LB, LC, LD, R, PS, CPrim, CDbl>(
+                (CPrim)(cp.prim + bpo), bnp, kv, knp, A, B, C, D, acc, nt,
+                (CDbl)(cp.primq + bpo), kqv, tau);
+        }
     } else {
-        eri_shell_quartet<LA, LB, LC, LD, R, PS>(cp.prim + bpo, bnp, cp.prim + cp.ket.poff[j],
-                                                 cp.ket.np[j], A, B, C, D, acc, nt, cp.primq + bpo,
-                                                 cp.primq + cp.ket.poff[j], tau);
+        if constexpr (far) {
+            if constexpr (S::L <= 1)
+                eri_one_node_far<LA, R, PS, const PrimPair*, const double*, const double*>(
+                    cp.prim + bpo, cp.prims + bpo, bnp, kv,
+                    StridedView{cp.kt + kKetS * cp.kt_fs + j, kn}, knp, A, acc, cp.primq + bpo,
+                    kqv, tau);
+        } else {
+            eri_shell_quartet<LA, LB, LC, LD, R, PS>(cp.prim + bpo, bnp, kv, knp, A, B, C, D, acc,
+                                                     nt, cp.primq + bpo, kqv, tau);
+        }
     }
     if constexpr (MODE == kModeRaw) {
 #pragma unroll
@@ -183,13 +207,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ClassWaves<
 void k_eri_class(const ClassParams cp)
 {
     using S = QuartetShape<LA, LB, LC, LD>;
-    __shared__ double s_tab[node_table_lds(S::NR) > 0 ? node_table_lds(S::NR) : 1];
-    const NodeTable nt = stage_node_table<S::NR>(s_tab);
+    // the far one-node launches need no node table
+    constexpr int NTAB = MODE == kModeJKFar ? 0 : node_table_lds(S::NR);
+    __shared__ double s_tab[NTAB > 0 ? NTAB : 1];
+    NodeTable nt{};
+    if constexpr (MODE != kModeJKFar) nt = stage_node_table<S::NR>(s_tab);
     const long long stride = (long long)gridDim.x * blockDim.x;
     const long long lane0 = (long long)blockIdx.x * blockDim.x + (threadIdx.x & ~63u);
     for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x, qw = lane0; q < cp.nq;
          q += stride, qw += stride) {
-        const QuartetIdx ix = locate(cp, q, qw);
+        QuartetIdx ix;
+        if constexpr (S::L <= 1) ix = cp.qlist ? locate_list(cp, q, qw) : locate(cp, q, qw);
+        else ix = locate(cp, q, qw);
         const int iw = __builtin_amdgcn_readfirstlane(ix.i);
         if (__all(ix.i == iw))
             class_quartet<LA, LB, LC, LD, R, MODE, true>(cp, nt, iw, ix.j, ix.qa);
@@ -262,11 +291,14 @@ __global__ __launch_bounds__(256) void k_eri_split(const ClassParams cp, double*
     const int i = ix.i, j = ix.j;
     const long long qa = ix.qa;
     const int a = cp.bra.sa[i], b = cp.bra.sb[i];
-    const int c = cp.ket.sa[j], d = cp.ket.sb[j];
     const double A[3] = {cp.shell_xyz[3 * a], cp.shell_xyz[3 * a + 1], cp.shell_xyz[3 * a + 2]};
     const double B[3] = {cp.shell_xyz[3 * b], cp.shell_xyz[3 * b + 1], cp.shell_xyz[3 * b + 2]};
-    const double C[3] = {cp.shell_xyz[3 * c], cp.shell_xyz[3 * c + 1], cp.shell_xyz[3 * c + 2]};
-    const double D[3] = {cp.shell_xyz[3 * d], cp.shell_xyz[3 * d + 1], cp.shell_xyz[3 * d + 2]};
+    // the ket's centres and primitive pairs from the transposed arrays
+    const long long kn = cp.kt_n;
+    const double C[3] = {cp.kc[j], cp.kc[kn + j], cp.kc[2 * kn + j]};
+    const double D[3] = {cp.kc[3 * kn + j], cp.kc[4 * kn + j], cp.kc[5 * kn + j]};
+    const KetView kv{cp.kt + j, cp.kt_fs, kn};
+    const StridedView kqv{cp.kt + kKetQ * cp.kt_fs + j, kn};
     R acc[NOUT];
 #pragma unroll
     for (int k = 0; k < NOUT; ++k) acc[k] = (R)0;
@@ -278,14 +310,14 @@ __global__ __launch_bounds__(256) void k_eri_split(const ClassParams cp, double*
         constexpr bool PS = true;
         if constexpr (S::SPLIT_ROWS) {
             constexpr int FA = decltype(sc)::value;
-            eri_bra_row<LA, LB, LC, LD, FA, R, PS>(
-                cp.prim + cp.bra.poff[i], cp.bra.np[i], cp.prim + cp.ket.poff[j], cp.ket.np[j], A, B,
-                C, D, acc, nt, cp.primq + cp.bra.poff[i], cp.primq + cp.ket.poff[j], tau);
+            eri_bra_row<LA, LB, LC, LD, FA, R, PS>(cp.prim + cp.bra.poff[i], cp.bra.np[i], kv,
+                                                   cp.ket.np[j], A, B, C, D, acc, nt,
+                                                   cp.primq + cp.bra.poff[i], kqv, tau);
         } else {
             constexpr int FA = decltype(sc)::value / S::NB, FB = decltype(sc)::value % S::NB;
             eri_bra_component<LA, LB, LC, LD, FA, FB, R, PS>(
-                cp.prim + cp.bra.poff[i], cp.bra.np[i], cp.prim + cp.ket.poff[j], cp.ket.np[j], A, B,
-                C, D, acc, nt, cp.primq + cp.bra.poff[i], cp.primq + cp.ket.poff[j], tau);
+                cp.prim + cp.bra.poff[i], cp.bra.np[i], kv, cp.ket.np[j], A, B, C, D, acc, nt,
+                cp.primq + cp.bra.poff[i], kqv, tau);
         }
     });
     // raw plane of integral (fa fb | fc fd): ((fa*NB + fb)*NK + fc*ND + fd);
@@ -544,6 +576,18 @@ static int launch_class(int fp32, int mode, const ClassParams& cp, const Tune& t
         unsigned nb = 0;
         SF_TRY_K(grid_blocks(cp.nq, tn.block, tn, &nb));
         dim3 grid(nb), block(tn.block);
+        if constexpr (QuartetShape<LA, LB, LC, LD>::L <= 1) {
+            if (mode == kModeJK && cp.far) {
+                if (fp32)
+                    hipLaunchKernelGGL((k_eri_class<LA, LB, LC, LD, float, kModeJKFar>), grid, block,
+                                       0, s, cp);
+                else
+                    hipLaunchKernelGGL((k_eri_class<LA, LB, LC, LD, double, kModeJKFar>), grid,
+                                       block, 0, s, cp);
+                SF_HIP(hipGetLastError());
+                return kOk;
+            }
+        }
         if (mode == kModeRaw)
             hipLaunchKernelGGL((k_eri_class<LA, LB, LC, LD, double, kModeRaw>), grid, block, 0, s, cp);
         else if (fp32)
@@ -804,7 +848,7 @@ __global__ __launch_bounds__(256) void k_count_prims(const ClassParams cp,
     unsigned long long cnt = 0;
     for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x, qw = lane0; q < cp.nq;
          q += stride, qw += stride) {
-        const QuartetIdx ix = locate(cp, q, qw);
+        const QuartetIdx ix = cp.qlist ? locate_list(cp, q, qw) : locate(cp, q, qw);
         const double* bq = cp.primq + cp.bra.poff[ix.i];
         const double* kq = cp.primq + cp.ket.poff[ix.j];
         const int nb = cp.bra.np[ix.i], nk = cp.ket.np[ix.j];

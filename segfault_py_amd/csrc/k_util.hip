@@ -231,6 +231,94 @@ int launch_plan_positions(const unsigned int* keys, const unsigned long long* va
     return kOk;
 }
 
+// The transposed ket arrays of one sorted pair list (KetView layout): one
+// thread per pair j writes field f of its primitive k at
+// kt[(f * kmax + k) * n + j] (zero past its np[j] primitives) and its two
+// centres at kc[e * n + j].
+__global__ __launch_bounds__(256) void k_ket_transpose(const PairListDev pl, int kmax,
+                                                       const PrimPair* __restrict__ prim,
+                                                       const double* __restrict__ primq,
+                                                       const double* __restrict__ prims,
+                                                       const double* __restrict__ xyz,
+                                                       double* __restrict__ kt,
+                                                       double* __restrict__ kc)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= pl.n) return;
+    const long long n = pl.n, fs = (long long)kmax * n;
+    const int o = pl.poff[j], np = pl.np[j];
+    for (int k = 0; k < kmax; ++k) {
+        double f[kKetFields] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        if (k < np) {
+            const PrimPair& pp = prim[o + k];
+            f[0] = pp.p;
+            f[1] = pp.Px;
+            f[2] = pp.Py;
+            f[3] = pp.Pz;
+            f[4] = pp.ip;
+            f[5] = pp.Kp;
+            f[kKetQ] = primq[o + k];
+            f[kKetS] = prims[o + k];
+        }
+#pragma unroll
+        for (int e = 0; e < kKetFields; ++e) kt[e * fs + k * n + j] = f[e];
+    }
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+        kc[e * n + j] = xyz[3 * pl.sa[j] + e];
+        kc[(3 + e) * n + j] = xyz[3 * pl.sb[j] + e];
+    }
+}
+
+int launch_ket_transpose(const PairListDev& pl, int kmax, const PrimPair* prim, const double* primq,
+                         const double* prims, const double* xyz, double* kt, double* kc,
+                         hipStream_t s)
+{
+    if (pl.n <= 0) return kOk;
+    hipLaunchKernelGGL(k_ket_transpose, dim3((unsigned)((pl.n + 255) / 256)), dim3(256), 0, s, pl,
+                       kmax, prim, primq, prims, xyz, kt, kc);
+    SF_HIP(hipGetLastError());
+    return kOk;
+}
+
+// One-node classes: a code per quartet of the class (class-wide index qa),
+// bit 0 = far (every primitive quartet, screened or not, has
+// X = p q / (p + q) |P - Q|^2 >= kFarX), bit 1 = in the FP64 segment of a
+// mixed build (ket index below n64 of its bra).  Depends on the geometry
+// only, so a quartet takes the same path in every build of the handle.
+__global__ __launch_bounds__(256) void k_far_codes(const ClassParams cp, long long nq,
+                                                   const int* __restrict__ n64,
+                                                   unsigned char* __restrict__ code)
+{
+    const long long qa = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (qa >= nq) return;
+    const int i = find_bra(cp.qall_base, cp.bra.n, qa);
+    const int jr = (int)(qa - cp.qall_base[i]);
+    const int j = cp.ket0[i] + jr;
+    const PrimPair* pb = cp.prim + cp.bra.poff[i];
+    const PrimPair* pk = cp.prim + cp.ket.poff[j];
+    const int nb = cp.bra.np[i], nk = cp.ket.np[j];
+    double xmin = 1e300;
+    for (int kk = 0; kk < nk; ++kk)
+        for (int kb = 0; kb < nb; ++kb) {
+            const double dx = pb[kb].Px - pk[kk].Px, dy = pb[kb].Py - pk[kk].Py,
+                         dz = pb[kb].Pz - pk[kk].Pz;
+            const double p = pb[kb].p, q = pk[kk].p;
+            xmin = fmin(xmin, p * q / (p + q) * (dx * dx + dy * dy + dz * dz));
+        }
+    code[qa] = (unsigned char)((xmin >= kFarX ? 1 : 0) | (jr < n64[i] ? 2 : 0));
+}
+
+int launch_far_codes(const ClassParams& cp, long long nq, const int* n64, unsigned char* code,
+                     hipStream_t s)
+{
+    if (nq <= 0) return kOk;
+    hipLaunchKernelGGL(k_far_codes, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, s, cp, nq, n64,
+                       code);
+    SF_HIP(hipGetLastError());
+    return kOk;
+}
+
 // Destination reduction, pass 1: one wavefront per chunk (four per
 // 256-thread block).  A chunk's tiles are contiguous (NE doubles each), so
 // lanes read them coalesced; a fixed strided assignment of tiles to lanes

@@ -7,6 +7,8 @@
 // (src/hf/hf.py:26-30, :100-104), build_eri_parallel (hf.py:33-61) and the
 // J/K einsums (hf.py:300).
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_select.hpp>
+#include <rocprim/iterator/counting_iterator.hpp>
 
 #include <algorithm>
 #include <chrono>
@@ -149,6 +151,11 @@ struct PairClassData {
     std::vector<int> s_sa, s_sb, s_poff, s_np, s_mol, s_mbeg;
     std::vector<double> s_Q;
     DevBuf ds_sa, ds_sb, ds_poff, ds_np, ds_Q;
+    // the significant pairs' primitive pairs transposed (KetView layout,
+    // kmax primitives per pair, zero-padded) and their two centres at
+    // kc[k * n + j] (Cx Cy Cz Dx Dy Dz), for the kernels' per-lane ket reads
+    int kmax = 0;
+    DevBuf d_kt, d_kc;
     PairListDev sorted_dev() const
     {
         PairListDev p;
@@ -169,6 +176,13 @@ struct QuartetClassData {
     std::vector<long long> base_all, base64, base32;
     DevBuf d_base_all, d_base64, d_base32, d_k032, d_ket0, d_nket, d_pos, d_raw;
     int ntile = 0;
+    // one-node classes: class-wide quartet indices of the far quartets
+    // (k_far_codes), then of the near ones, of each index space: [0] the
+    // whole class, [1] the FP64 segment, [2] the FP32 segment of mixed
+    // builds; nlist[sp][0] near and [1] far quartets
+    bool lists = false;
+    DevBuf d_list[3];
+    long long nlist[3][2] = {};
 };
 
 // A captured J/K build (sf_jk_graph) and the arguments it was captured for.
@@ -193,7 +207,8 @@ struct sf_basis {
     std::vector<double> xyz, expn, coef, fnorm;
     std::vector<PrimPair> prim;
     std::vector<double> prim_q;            // Schwarz factor of each primitive pair
-    DevBuf d_prim, d_xyz, d_foff, d_fnorm, d_zero, d_prim_q;
+    std::vector<double> prim_s;            // pi^{3/2} K p^{-3/2} of each primitive pair
+    DevBuf d_prim, d_xyz, d_foff, d_fnorm, d_zero, d_prim_q, d_prim_s;
     DevBuf d_row, d_col, d_ld, d_sh_mol, d_sh_loc, d_mol_ns, d_mol_kbase;
     DevBuf d_mol_off, d_mol_ld, d_mol_n, d_mol_fbase, d_mol_nuc;
     // nuclei (point charges) for the one-electron potential
@@ -926,6 +941,12 @@ static int compute_schwarz(sf_basis* b)
         });
     SF_TRY(b->d_prim.upload(b->prim));
     SF_TRY(b->d_prim_q.upload(b->prim_q.empty() ? std::vector<double>(1, 0.0) : b->prim_q));
+    // charge of each primitive pair for the far one-node quartets
+    // (eri_one_node_far): S = pi^{3/2} K p^{-3/2} = pi^{3/2} (K/p) sqrt(1/p)
+    b->prim_s.resize(b->prim.size());
+    for (size_t k = 0; k < b->prim.size(); ++k)
+        b->prim_s[k] = 5.568327996831708 * b->prim[k].Kp * std::sqrt(b->prim[k].ip);
+    SF_TRY(b->d_prim_s.upload(b->prim_s.empty() ? std::vector<double>(1, 0.0) : b->prim_s));
     b->have_schwarz = true;
     return kOk;
 }
@@ -954,6 +975,8 @@ static int key_bits(unsigned long long nkeys)
     while ((1ULL << bits) < nkeys) ++bits;
     return bits;
 }
+
+static int build_far_lists(sf_basis* b);
 
 extern "C" int sf_screen(sf_basis* b, double tau, double tau_mixed)
 {
@@ -1008,6 +1031,15 @@ extern "C" int sf_screen(sf_basis* b, double tau, double tau_mixed)
         SF_TRY(P.ds_poff.upload(P.s_poff));
         SF_TRY(P.ds_np.upload(P.s_np));
         SF_TRY(P.ds_Q.upload(P.s_Q));
+        const long long n = (long long)P.s_sa.size();
+        P.kmax = 0;
+        for (int v : P.s_np) P.kmax = std::max(P.kmax, v);
+        SF_TRY(P.d_kt.ensure((size_t)std::max(1LL, kKetFields * P.kmax * n) * sizeof(double)));
+        SF_TRY(P.d_kc.ensure((size_t)std::max(1LL, 6 * n) * sizeof(double)));
+        SF_TRY(launch_ket_transpose(P.sorted_dev(), P.kmax, b->d_prim.as<PrimPair>(),
+                                    b->d_prim_q.as<double>(), b->d_prim_s.as<double>(),
+                                    b->d_xyz.as<double>(), P.d_kt.as<double>(),
+                                    P.d_kc.as<double>(), 0));
     }
     tm.mark("pair lists");
     // quartet lists per class (GPU builder), prefix sums on the host
@@ -1306,6 +1338,9 @@ extern "C" int sf_screen(sf_basis* b, double tau, double tau_mixed)
     SF_TRY(b->d_part.ensure((size_t)std::max(1LL, part_total) * sizeof(double)));
     SF_HIP(hipDeviceSynchronize());
     tm.mark("positions + buffers");
+    SF_TRY(build_far_lists(b));
+    SF_HIP(hipDeviceSynchronize());
+    tm.mark("near/far lists");
     b->screened = true;
     for (int c = 0; c < kNumQuartetClass; ++c) b->class_ms[c][0] = b->class_ms[c][1] = 0.0;
     return kOk;
@@ -1404,7 +1439,26 @@ static int launch_class_engine(sf_basis* b, int c, int seg, int mode, const Clas
                                const Tune& tn, hipStream_t s)
 {
     if (b->engine[c] == 1) return launch_eri_mmd(c, mode, cp, tn, s);
-    return launch_eri_class(c, (seg == 1 && !tn.arith64) ? 1 : 0, mode, cp, tn, s);
+    const int fp32 = (seg == 1 && !tn.arith64) ? 1 : 0;
+    QuartetClassData& Q = b->qc[c];
+    if (mode != kModeJK || !Q.lists || cp.qlist)
+        return launch_eri_class(c, fp32, mode, cp, tn, s);
+    // one-node class in J/K mode: its near quartets, then its far ones (the
+    // point-charge form), as two launches over the lists of cp's index space
+    // (a prefix of cp.nq quartets, as the autotuner times, takes the same
+    // share of each list)
+    const int sp = cp.seg_base == Q.d_base_all.as<long long>() ? 0
+                   : cp.seg_base == Q.d_base64.as<long long>() ? 1 : 2;
+    const long long full = sp == 0 ? Q.nq_all : sp == 1 ? Q.nq64 : Q.nq32;
+    for (int far = 0; far < 2; ++far) {
+        ClassParams lc = cp;
+        lc.qlist = Q.d_list[sp].as<unsigned>() + (far ? 0 : Q.nlist[sp][1]);
+        lc.far = far;
+        lc.nq = Q.nlist[sp][far];
+        if (cp.nq < full) lc.nq = (long long)((double)lc.nq * (double)cp.nq / (double)full + 0.5);
+        if (lc.nq > 0) SF_TRY(launch_eri_class(c, fp32, mode, lc, tn, s));
+    }
+    return kOk;
 }
 
 extern "C" int sf_set_streams(sf_basis* b, int nside)
@@ -1467,7 +1521,68 @@ static ClassParams class_params(sf_basis* b, int c, const double* Ps)
     cp.raw = Q.d_raw.as<double>();
     cp.pos = Q.d_pos.as<unsigned>();
     cp.tiles = b->d_tiles.as<double>();
+    cp.qlist = nullptr;
+    cp.far = 0;
+    cp.prims = b->d_prim_s.as<double>();
+    const PairClassData& K = b->pc[Q.pck];
+    cp.kt = K.d_kt.as<double>();
+    cp.kt_n = (long long)K.s_sa.size();
+    cp.kt_fs = (long long)K.kmax * cp.kt_n;
+    cp.kc = K.d_kc.as<double>();
     return cp;
+}
+
+// The near and far lists of the one-node classes (QuartetClassData::d_list),
+// built once per sf_screen from k_far_codes.
+struct CodeIs {
+    const unsigned char* code;
+    unsigned char mask, want;
+    __host__ __device__ bool operator()(unsigned qa) const { return (code[qa] & mask) == want; }
+};
+
+static int build_far_lists(sf_basis* b)
+{
+    for (int c = 0; c < kNumQuartetClass; ++c) {
+        QuartetClassData& Q = b->qc[c];
+        Q.lists = false;
+        for (DevBuf& d : Q.d_list) d.release();
+        if (class_ltot(c) > 1 || Q.nq_all == 0) continue;
+        if (Q.nq_all >= (1LL << 32)) continue;  // 32-bit list entries
+        const ClassParams cp = class_params(b, c, b->d_Ps.as<double>());
+        DevBuf codes, d_cnt, tmp;
+        SF_TRY(codes.ensure((size_t)Q.nq_all));
+        SF_TRY(d_cnt.ensure(sizeof(unsigned long long)));
+        SF_TRY(launch_far_codes(cp, Q.nq_all, Q.d_k032.as<int>(), codes.as<unsigned char>(), 0));
+        // index space sp: mask and wanted bit 1 (FP64 segment); far: bit 0
+        const unsigned char mask[3] = {1, 3, 3}, seg[3] = {0, 2, 0};
+        const long long nsp[3] = {Q.nq_all, Q.nq64, Q.nq32};
+        for (int sp = 0; sp < 3; ++sp) {
+            SF_TRY(Q.d_list[sp].ensure((size_t)std::max(1LL, nsp[sp]) * sizeof(unsigned)));
+            long long at = 0;
+            for (int far = 1; far >= 0; --far) {
+                const CodeIs pred{codes.as<unsigned char>(), mask[sp], (unsigned char)(seg[sp] | far)};
+                rocprim::counting_iterator<unsigned> in(0u);
+                unsigned* out = Q.d_list[sp].as<unsigned>() + at;
+                size_t bytes = 0;
+                SF_HIP(rocprim::select(nullptr, bytes, in, out, d_cnt.as<unsigned long long>(),
+                                       (size_t)Q.nq_all, pred, (hipStream_t)0));
+                SF_TRY(tmp.ensure(bytes));
+                SF_HIP(rocprim::select(tmp.p, bytes, in, out, d_cnt.as<unsigned long long>(),
+                                       (size_t)Q.nq_all, pred, (hipStream_t)0));
+                unsigned long long n = 0;
+                SF_HIP(hipMemcpy(&n, d_cnt.p, sizeof(n), hipMemcpyDeviceToHost));
+                Q.nlist[sp][far] = (long long)n;
+                at += (long long)n;
+            }
+            if (at != nsp[sp]) {
+                set_error(kEState, "near/far lists of class %d: %lld of %lld quartets", c, at,
+                          nsp[sp]);
+                return kEState;
+            }
+        }
+        Q.lists = true;
+    }
+    return kOk;
 }
 
 // Segment `seg` of quartet class c: 0 = all quartets (precision 0) or the
@@ -1515,6 +1630,43 @@ extern "C" int sf_prim_quartet_counts(sf_basis* b, int precision, long long* cou
             counts[2 * c + seg] = 0;
             if (seg > precision || Q.nq_all == 0) continue;
             set_segment(b, c, precision, seg, cp, b->d_zero.as<int>());
+            if (cp.nq == 0) continue;
+            SF_TRY(launch_count_prims(cp, nblock, d_part.as<unsigned long long>(), 0));
+            SF_HIP(hipMemcpy(part.data(), d_part.p, nblock * sizeof(unsigned long long),
+                             hipMemcpyDeviceToHost));
+            unsigned long long t = 0;
+            for (unsigned long long v : part) t += v;
+            counts[2 * c + seg] = (long long)t;
+        }
+    }
+    return kOk;
+}
+
+extern "C" int sf_far_prim_quartet_counts(sf_basis* b, int precision, long long* counts)
+{
+    if (!b || !counts || precision < 0 || precision > 1) {
+        set_error(kEInval, "sf_far_prim_quartet_counts: bad arguments");
+        return kEInval;
+    }
+    if (!b->screened) {
+        set_error(kEState, "sf_far_prim_quartet_counts: call sf_screen first");
+        return kEState;
+    }
+    const int nblock = 2048;
+    DevBuf d_part;
+    SF_TRY(d_part.ensure(nblock * sizeof(unsigned long long)));
+    std::vector<unsigned long long> part(nblock);
+    for (int c = 0; c < kNumQuartetClass; ++c) {
+        QuartetClassData& Q = b->qc[c];
+        ClassParams cp = class_params(b, c, nullptr);
+        for (int seg = 0; seg < 2; ++seg) {
+            counts[2 * c + seg] = 0;
+            if (seg > precision || !Q.lists) continue;
+            set_segment(b, c, precision, seg, cp, b->d_zero.as<int>());
+            // the far list of the segment's index space
+            const int sp = precision == 0 ? 0 : 1 + seg;
+            cp.qlist = Q.d_list[sp].as<unsigned>();
+            cp.nq = Q.nlist[sp][1];
             if (cp.nq == 0) continue;
             SF_TRY(launch_count_prims(cp, nblock, d_part.as<unsigned long long>(), 0));
             SF_HIP(hipMemcpy(part.data(), d_part.p, nblock * sizeof(unsigned long long),

@@ -62,6 +62,13 @@ constexpr int pair_of_class(int c)
     while ((p + 1) * (p + 2) / 2 <= c) ++p;
     return p;
 }
+// total angular momentum la + lb + lc + ld of quartet class c
+constexpr int class_ltot(int c)
+{
+    const int pb = pair_of_class(c), pk = c - pb * (pb + 1) / 2;
+    return pair_la(pb) + pair_lb(pb) + pair_la(pk) + pair_lb(pk);
+}
+static_assert(class_ltot(0) == 0 && class_ltot(1) == 1 && class_ltot(2) == 2, "class order");
 static_assert(quartet_class(pair_class(3, 0), 0) == kNumClassSPD, "f classes follow s/p/d");
 static_assert(quartet_class(pair_class(3, 3), pair_class(3, 3)) == kNumQuartetClass - 1, "55 classes");
 
@@ -119,6 +126,20 @@ struct ClassParams {
     double* raw;               // split classes: raw integral planes (NINT * qstride)
     const unsigned* pos;       // kModeJK: pos[t * qstride + q] = slot of tile t in `tiles`
     double* tiles;             // kModeJK: destination-sorted digestion tiles
+    // one-node classes ((ss|ss), (ps|ss)), kModeJK: with qlist, the launch
+    // covers the class-wide quartet indices qlist[0..nq) instead of the
+    // segment; with far, every primitive quartet of those quartets has
+    // X >= kFarX and the point-charge form runs (prims: S = pi^{3/2} K p^{-3/2}
+    // of each primitive pair, same index as prim)
+    const unsigned* qlist = nullptr;
+    int far = 0;
+    const double* prims = nullptr;
+    // the ket pair list's primitive pairs transposed (KetView: field stride
+    // kt_fs = kmax * kt_n, primitive stride kt_n) and its pairs' centres
+    // (kc[k * kt_n + j], Cx Cy Cz Dx Dy Dz)
+    const double* kt = nullptr;
+    long long kt_fs = 0, kt_n = 0;
+    const double* kc = nullptr;
 };
 
 // Flat segment index q -> bra i, ket j (index in the ket list) and the
@@ -142,6 +163,23 @@ __device__ __forceinline__ int find_bra(const long long* __restrict__ base, int 
 // The bra of quartet q: one binary search per wavefront for its first
 // lane's quartet qw (wave-uniform, so scalar loads), then each lane walks
 // forward over the few bras the wavefront spans.  Requires qw <= q < nq.
+// The same for a launch over a list of class-wide quartet indices (sorted
+// ascending): the wavefront searches for the bra of its first lane's entry.
+__device__ __forceinline__ QuartetIdx locate_list(const ClassParams& cp, long long t, long long tw)
+{
+    const unsigned lo32 = __builtin_amdgcn_readfirstlane((unsigned)(tw & 0xffffffffLL));
+    const unsigned hi32 = __builtin_amdgcn_readfirstlane((unsigned)(tw >> 32));
+    const long long tu = ((long long)hi32 << 32) | lo32;
+    int i = __builtin_amdgcn_readfirstlane(find_bra(cp.qall_base, cp.bra.n, (long long)cp.qlist[tu]));
+    const long long qa = cp.qlist[t];
+    while (cp.qall_base[i + 1] <= qa) ++i;
+    QuartetIdx r;
+    r.i = i;
+    r.j = cp.ket0[i] + (int)(qa - cp.qall_base[i]);
+    r.qa = qa;
+    return r;
+}
+
 __device__ __forceinline__ QuartetIdx locate(const ClassParams& cp, long long q, long long qw)
 {
     const unsigned lo32 = __builtin_amdgcn_readfirstlane((unsigned)(qw & 0xffffffffLL));
@@ -163,7 +201,10 @@ __device__ __forceinline__ QuartetIdx locate(const ClassParams& cp, long long q,
 // kModeEriOnly writes the raw planes into ClassParams::raw without
 // digesting them, kModeDigestOnly digests ClassParams::raw (so the two
 // segments of a mixed build can share one digestion launch).
-enum EriMode : int { kModeJK = 0, kModeRaw = 1, kModeEriOnly = 2, kModeDigestOnly = 3 };
+// kModeJKFar (kernel instantiation only): kModeJK for the far quartets of
+// a one-node class (ClassParams::far), with the point-charge integrals.
+enum EriMode : int { kModeJK = 0, kModeRaw = 1, kModeEriOnly = 2, kModeDigestOnly = 3,
+                     kModeJKFar = 4 };
 
 // Where the plan-position kernel writes the slots of one quartet class.
 struct ClassTab {
@@ -278,6 +319,11 @@ int launch_segments(const unsigned int* keys, long long n, long long* seg_begin,
 int launch_plan_positions(const unsigned int* keys, const unsigned long long* vals, long long n,
                           const long long* seg_begin, const unsigned long long* key_base,
                           const int* key_ne, const ClassTab* tab, hipStream_t s);
+int launch_far_codes(const ClassParams& cp, long long nq, const int* n64, unsigned char* code,
+                     hipStream_t s);
+int launch_ket_transpose(const PairListDev& pl, int kmax, const PrimPair* prim, const double* primq,
+                         const double* prims, const double* xyz, double* kt, double* kc,
+                         hipStream_t s);
 int launch_reduce(int n1, int n2, const DestList& dl, const double* tiles, const ShellMap& map,
                   double* X, double* Y, hipStream_t s);
 int launch_scale_density(const double* P, const double* fnorm, const MolLayout& ml, double* Ps,

@@ -168,6 +168,12 @@ SF_HD SF_INLINE double inv_fact(int m)
     return t[m];
 }
 
+// Quartets whose every primitive quartet has X >= kFarX are "far": there
+// boys01_taylor takes its asymptotic branch (X >= 39.95; the neglected
+// erfc tail is below 1e-17 relative), so F_0 = sqrt(pi / (4X)) and
+// F_1 = F_0 / (2X) exactly as in that branch.
+constexpr double kFarX = 40.0;
+
 // F_0 and F_1 from the Taylor rows: with d = X_k - X (|d| <= 0.1),
 // F_0(X) = sum_m c_m d^m and F_1 = -dF_0/dX = dF_0/dd = sum_m m c_m d^(m-1),
 // both from one Horner pass (truncation below 1e-19 for both).

@@ -99,6 +99,7 @@ def lib():
     L.sf_set_prim_tau.argtypes = [vp, ctypes.c_double]
     L.sf_set_prim_tau_fp64.argtypes = [vp, ctypes.c_double]
     L.sf_prim_quartet_counts.argtypes = [vp, ctypes.c_int, _i64p]
+    L.sf_far_prim_quartet_counts.argtypes = [vp, ctypes.c_int, _i64p]
     L.sf_set_engine.argtypes = [vp, ctypes.c_int, ctypes.c_int]
     L.sf_set_streams.argtypes = [vp, ctypes.c_int]
     _LIB = L
@@ -118,7 +119,8 @@ def exported_symbols() -> list[str]:
             "sf_eri_dense", "sf_jk", "sf_jk_profile", "sf_basis_set_nuclei", "sf_one_electron",
             "sf_grad", "sf_basis_create_batch", "sf_basis_set_layout", "sf_basis_layout",
             "sf_jk_graph", "sf_autotune", "sf_set_tuning", "sf_set_prim_tau", "sf_set_prim_tau_fp64",
-            "sf_set_engine", "sf_set_streams", "sf_prim_quartet_counts"]
+            "sf_set_engine", "sf_set_streams", "sf_prim_quartet_counts",
+            "sf_far_prim_quartet_counts"]
 
 
 def _torch():
@@ -472,6 +474,13 @@ class Basis:
         under the current thresholds (sf_prim_quartet_counts): [55, 2]."""
         c = np.zeros(2 * NUM_QUARTET_CLASSES, dtype=np.int64)
         check(lib().sf_prim_quartet_counts(self._h, int(precision), c))
+        return c.reshape(NUM_QUARTET_CLASSES, 2)
+
+    def far_prim_quartet_counts(self, precision: int = 0) -> np.ndarray:
+        """The part of prim_quartet_counts in far (ss|ss) / (ps|ss) quartets,
+        evaluated in the point-charge form (sf_far_prim_quartet_counts)."""
+        c = np.zeros(2 * NUM_QUARTET_CLASSES, dtype=np.int64)
+        check(lib().sf_far_prim_quartet_counts(self._h, int(precision), c))
         return c.reshape(NUM_QUARTET_CLASSES, 2)
 
     def pair_list(self, pc: int, with_nprim: bool = False):

@@ -50,6 +50,8 @@ def harness():
                                                  _f64p, _f64p, _f64p, _f64p, _f64p]
         H.h_gen_tiles.argtypes = H.h_digest_tiles.argtypes
         H.h_eri_bra_runtime.argtypes = H.h_eri_gen.argtypes
+        H.h_eri_far.argtypes = [ctypes.c_int, _f64p, ctypes.c_int, _f64p, ctypes.c_int, _f64p,
+                                _f64p]
         H.h_one_electron.argtypes = [ctypes.c_int, ctypes.c_int, _f64p, ctypes.c_int, _f64p, _f64p,
                                      ctypes.c_int, _f64p, _f64p, _f64p, _f64p, _f64p]
         _H = H

@@ -4,6 +4,8 @@
 #include "../../segfault_py_amd/csrc/eri_gen.hpp"
 #include "../../segfault_py_amd/csrc/mmd.hpp"
 
+#include <vector>
+
 using namespace sf;
 
 extern "C" void h_boys(int mmax, double X, double* F) { boys_grid(mmax, X, F); }
@@ -445,4 +447,28 @@ extern "C" int h_grad_onee(int la, int lb, const double* prim, int np, const dou
     O_(0, 0) O_(1, 0) O_(1, 1) O_(2, 0) O_(2, 1) O_(2, 2)
 #undef O_
     return 1;
+}
+
+// The one-node classes' point-charge form for far quartets
+// (eri_one_node_far, no primitive screening); bs / ks: S of each primitive
+// pair as the library computes it, pi^{3/2} (K/p) sqrt(1/p).
+extern "C" int h_eri_far(int la, const double* bra, int nb, const double* ket, int nk,
+                         const double* A, double* out)
+{
+    std::vector<double> bs(nb), ks(nk);
+    const PrimPair* pb = (const PrimPair*)bra;
+    const PrimPair* pk = (const PrimPair*)ket;
+    for (int k = 0; k < nb; ++k) bs[k] = 5.568327996831708 * pb[k].Kp * std::sqrt(pb[k].ip);
+    for (int k = 0; k < nk; ++k) ks[k] = 5.568327996831708 * pk[k].Kp * std::sqrt(pk[k].ip);
+    double acc[3] = {0.0, 0.0, 0.0};
+    if (la == 0)
+        eri_one_node_far<0, double, false, const PrimPair*, const double*, const double*>(
+            pb, bs.data(), nb, pk, ks.data(), nk, A, acc, nullptr, nullptr, 0.0);
+    else if (la == 1)
+        eri_one_node_far<1, double, false, const PrimPair*, const double*, const double*>(
+            pb, bs.data(), nb, pk, ks.data(), nk, A, acc, nullptr, nullptr, 0.0);
+    else
+        return 1;
+    for (int k = 0; k < (la == 0 ? 1 : 3); ++k) out[k] = acc[k];
+    return 0;
 }

@@ -350,3 +350,40 @@ def test_one_and_two_electron_gradient_models_match_fd(oracle, water_sp):
             fd2[at, ax] = (e2[0] - e2[1]) / (2 * h)
     assert np.max(np.abs(g1 - fd1)) < 1e-6, (g1, fd1)
     assert np.max(np.abs(g2 - fd2)) < 1e-6, (g2, fd2)
+
+
+def test_far_point_charge_form_equals_full_path():
+    """(ss|ss) and (ps|ss) quartets whose every primitive quartet has
+    X = p q / (p + q) |P - Q|^2 >= 40 (the GPU's far lists): the point-charge
+    form eri_one_node_far (S_b S_k / R, and the dipole term for p) equals
+    the Taylor/asymptotic Boys path of eri_shell_quartet to rounding, on a
+    20-water cluster; and such quartets exist in number."""
+    from segfault_py_amd import workloads as W
+    mol = W.config_molecule(1)
+    shells = mol.shells()
+    H = CM.harness()
+    rng = np.random.default_rng(3)
+    s_sh = [k for k, sh in enumerate(shells) if sh.L == 0]
+    p_sh = [k for k, sh in enumerate(shells) if sh.L == 1]
+    nfar = nnear = 0
+    worst = 0.0
+    for t in range(400):
+        la = t % 2
+        a = int(rng.choice(p_sh if la else s_sh))
+        b, c, d = (int(rng.choice(s_sh)) for _ in range(3))
+        bra, ket = CM.prim_pairs(shells[a], shells[b]), CM.prim_pairs(shells[c], shells[d])
+        pb, pk = bra.reshape(-1, 8), ket.reshape(-1, 8)
+        p, q = pb[:, 0][:, None], pk[:, 0][None, :]
+        d2 = ((pb[:, None, 1:4] - pk[None, :, 1:4]) ** 2).sum(-1)
+        if (p * q / (p + q) * d2).min() < 40.0:
+            nnear += 1
+            continue
+        nfar += 1
+        ref = CM.shell_quartet(shells, a, b, c, d)
+        out = np.zeros(3 if la else 1)
+        assert H.h_eri_far(la, bra, len(pb), ket, len(pk), shells[a].center, out) == 0
+        # relative to the quartet's largest integral (values below 1e-250,
+        # far beneath any screening threshold, against that floor)
+        worst = max(worst, float(np.max(np.abs(out - ref)) / max(np.max(np.abs(ref)), 1e-250)))
+    assert nfar > 100 and nnear > 0, (nfar, nnear)
+    assert worst < 1e-14, worst

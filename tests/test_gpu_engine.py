@@ -137,6 +137,10 @@ def test_jk_cluster_matches_einsum(gpu, oracle):
     J, K = b.jk(P)
     assert np.max(np.abs(J.cpu().numpy() - Jr)) < 1e-9
     assert np.max(np.abs(K.cpu().numpy() - Kr)) < 1e-9
+    # the comparison covers the point-charge form of far (ss|ss) and (ps|ss)
+    # quartets: a large share of their primitive quartets is far here
+    far, ev = b.far_prim_quartet_counts(0), b.prim_quartet_counts(0)
+    assert far[0, 0] > 0.3 * ev[0, 0] and far[1, 0] > 0.3 * ev[1, 0], (far[:2], ev[:2])
 
 
 def test_jk_deterministic_and_mixed_close(gpu):
@@ -306,6 +310,14 @@ def test_prim_quartet_counts(gpu):
     evm2 = b.prim_quartet_counts(1)
     assert (ev2 <= ev).all() and ev2.sum() < ev.sum()
     assert (evm2 <= evm).all() and evm2[:, 1].sum() < evm[:, 1].sum()
+    # far (point-charge) part: only the one-node classes, within the counts,
+    # and the mixed segments' far parts add up to the FP64 build's
+    far, farm = b.far_prim_quartet_counts(0), b.far_prim_quartet_counts(1)
+    assert (far[:2, 0] > 0).all() and not far[2:].any() and not far[:, 1].any()
+    assert (far <= ev2).all() and (farm <= evm2).all()
+    b.set_prim_tau_fp64(1e-15)
+    b.set_prim_tau(1e-15)
+    assert (b.far_prim_quartet_counts(1).sum(axis=1) == b.far_prim_quartet_counts(0)[:, 0]).all()
 
 
 @pytest.mark.parametrize("name,group", [("water", "others"), ("methane", "alkane")])
