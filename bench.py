@@ -349,10 +349,14 @@ def main():
         "roofline": {"bound": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
                      "traffic": measured_traffic(dom["ls"]),
-                     "kernel": f"k_eri_class<{','.join(map(str, dom['ls']))},double,JK>",
+                     "kernel": (f"k_eri_class<{','.join(map(str, dom['ls']))},double,JK>"
+                                + (f" + k_eri_class<{','.join(map(str, dom['ls']))},double,JKFar>"
+                                   if dom["primq_far"] else "")),
                      "kernel_ms": dom_ms,
                      "note": "ceiling: FP64 vector ALU peak (the Rys kernels issue no MFMA); "
-                             "flops count the primitive quartets evaluated after screening"},
+                             "flops count the primitive quartets evaluated after screening; a "
+                             "one-node class is two launches (near quartets, and far ones in the "
+                             "point-charge form, counted at its own flops), timed together"},
     }
     # full SCF (DIIS, tol 1e-8) in FP64 and in mixed precision on the same basis
     from segfault_py_amd import hf
