@@ -354,9 +354,14 @@ def main():
                                    if dom["primq_far"] else "")),
                      "kernel_ms": dom_ms,
                      "note": "ceiling: FP64 vector ALU peak (the Rys kernels issue no MFMA); "
-                             "flops count the primitive quartets evaluated after screening; a "
-                             "one-node class is two launches (near quartets, and far ones in the "
-                             "point-charge form, counted at its own flops), timed together"},
+                             "algorithmic flops = the fixed Rys flop count per primitive quartet "
+                             "x the primitive quartets evaluated after screening; a one-node "
+                             "class is two launches, timed together: near quartets, and far "
+                             "ones in the point-charge form, which evaluates the same primitive "
+                             "quartets with fewer executed flops",
+                     "far_primitive_quartets": int(dom["primq_far"]),
+                     "executed_flops_far_form": int(dom["primq_far"] * metrics.FAR_FLOPS.get(
+                         tuple(dom["ls"]), 0))},
     }
     # full SCF (DIIS, tol 1e-8) in FP64 and in mixed precision on the same basis
     from segfault_py_amd import hf

@@ -31,10 +31,12 @@ def flops_per_prim_quartet(la, lb, lc, ld) -> int:
             + ncart(la) * ncart(lb) * ncart(lc) * ncart(ld) * (4 * nr + 2))
 
 
-# flops per primitive quartet of the point-charge form of far quartets
-# (eri_one_node_far): P-Q 3, |P-Q|^2 5, 1/|P-Q| (rsqrt seed + two Newton
-# steps) 14, S_b S_k / R 2, accumulation 1; (ps|ss) adds c = ip/(2R^2) 3,
-# P-A 3, three components 3 x (fma + mul) and 2 more accumulations
+# Executed flops per primitive quartet of the point-charge form of far
+# quartets (eri_one_node_far): P-Q 3, |P-Q|^2 5, 1/|P-Q| (rsqrt seed + two
+# Newton steps) 13, S_b S_k / R 2, screening test 1, accumulation 1; (ps|ss)
+# adds c = ip/(2R^2) 3, P-A 3, three components 3 x (fma + mul + add) less
+# the one accumulation.  The algorithmic count (class_work's flops) stays
+# the Rys figure per primitive quartet, the fixed unit of work.
 FAR_FLOPS = {(0, 0, 0, 0): 25, (1, 0, 0, 0): 42}
 
 
@@ -45,8 +47,9 @@ def class_work(basis, mol=None, evaluated=None, far=None) -> list[dict]:
     ``evaluated`` ([55, 2] from Basis.prim_quartet_counts(0)), when given,
     replaces the all-quartet count by the primitive quartets the kernels
     evaluate after primitive-quartet screening, so flops count only work
-    done; ``far`` ([55, 2] from Basis.far_prim_quartet_counts(0)) the part
-    of them evaluated in the point-charge form (FAR_FLOPS each)."""
+    done; ``far`` ([55, 2] from Basis.far_prim_quartet_counts(0)) records
+    the part of them evaluated in the point-charge form (``primq_far``;
+    FAR_FLOPS executed flops each, the algorithmic flops unchanged)."""
     out = []
     pairs = {}
     for pc in range(len(PAIR_CLASSES)):
@@ -82,11 +85,10 @@ def class_work(basis, mol=None, evaluated=None, far=None) -> list[dict]:
             if evaluated is not None:
                 rec["primq"] = int(evaluated[c][0])
             fpq = flops_per_prim_quartet(la, lb, lc, ld)
-            # far (ss|ss) / (ps|ss) primitive quartets take the point-charge form
-            nfar = int(far[c][0]) if far is not None else 0
-            rec["primq_far"] = nfar
-            rec["flops"] = float((rec["primq"] - nfar) * fpq
-                                 + nfar * FAR_FLOPS.get((la, lb, lc, ld), fpq) + 12 * rec["funcq"])
+            # far (ss|ss) / (ps|ss) primitive quartets are the same units of
+            # work in the point-charge form (fewer executed flops, FAR_FLOPS)
+            rec["primq_far"] = int(far[c][0]) if far is not None else 0
+            rec["flops"] = float(rec["primq"] * fpq + 12 * rec["funcq"])
             rec["flops64"] = float(rec["primq64"] * fpq + 12 * rec["funcq64"])
         out.append(rec)
     return out
