@@ -298,15 +298,21 @@ __global__ __launch_bounds__(256) void k_far_codes(const ClassParams cp, long lo
     const PrimPair* pb = cp.prim + cp.bra.poff[i];
     const PrimPair* pk = cp.prim + cp.ket.poff[j];
     const int nb = cp.bra.np[i], nk = cp.ket.np[j];
-    double xmin = 1e300;
-    for (int kk = 0; kk < nk; ++kk)
+    // X = p q / (p + q) R^2 >= kFarX, tested without the division as
+    // p q R^2 >= kFarX (p + q); stops at the first near primitive quartet
+    bool far = true;
+    for (int kk = 0; kk < nk && far; ++kk) {
+        const double Qx = pk[kk].Px, Qy = pk[kk].Py, Qz = pk[kk].Pz, q = pk[kk].p;
         for (int kb = 0; kb < nb; ++kb) {
-            const double dx = pb[kb].Px - pk[kk].Px, dy = pb[kb].Py - pk[kk].Py,
-                         dz = pb[kb].Pz - pk[kk].Pz;
-            const double p = pb[kb].p, q = pk[kk].p;
-            xmin = fmin(xmin, p * q / (p + q) * (dx * dx + dy * dy + dz * dz));
+            const double dx = pb[kb].Px - Qx, dy = pb[kb].Py - Qy, dz = pb[kb].Pz - Qz;
+            const double p = pb[kb].p;
+            if (p * q * (dx * dx + dy * dy + dz * dz) < kFarX * (p + q)) {
+                far = false;
+                break;
+            }
         }
-    code[qa] = (unsigned char)((xmin >= kFarX ? 1 : 0) | (jr < n64[i] ? 2 : 0));
+    }
+    code[qa] = (unsigned char)((far ? 1 : 0) | (jr < n64[i] ? 2 : 0));
 }
 
 int launch_far_codes(const ClassParams& cp, long long nq, const int* n64, unsigned char* code,
