@@ -341,12 +341,21 @@ struct TileRows {
     static constexpr int total = base(5) + n1(5);
 };
 
+// Classes digested by tile rows only: (dd|dd), whose one-tile groups (36
+// outputs and a 36-element density block per lane) spill to scratch (304
+// bytes per lane) where the rows do not.
+template <int LA, int LB, int LC, int LD>
+struct DigestRowsOnly {
+    static constexpr bool value = LA == 2 && LB == 2 && LC == 2 && LD == 2;
+};
+
 template <int LA, int LB, int LC, int LD>
 __global__ __launch_bounds__(256) void k_digest_split(const ClassParams cp,
                                                       const double* __restrict__ raw, int rows)
 {
     using TR = TileRows<LA, LB, LC, LD>;
     using TG = TileGroups<LA, LB, LC, LD>;
+    if constexpr (DigestRowsOnly<LA, LB, LC, LD>::value) rows = 1;
     const int per = rows ? TR::total : TG::count;
     const long long nwork = ((cp.nq + 63) >> 6) * per;
     const long long wstride = ((long long)gridDim.x * blockDim.x) >> 6;
@@ -381,7 +390,7 @@ __global__ __launch_bounds__(256) void k_digest_split(const ClassParams cp,
 #pragma unroll
             for (int e = 0; e < G::N2; ++e) dst[r * G::N2 + e] = row[e];
         });
-    } else {
+    } else if constexpr (!DigestRowsOnly<LA, LB, LC, LD>::value) {
         // w = the tile group
         static_switch<0, TG::count>(w, [&](auto gc) {
             constexpr int g = decltype(gc)::value;
@@ -535,7 +544,7 @@ static int launch_split(int fp32, int mode, const ClassParams& cp, const Tune& t
     SF_TRY_K(grid_blocks(waves * S::NSLICE_ERI, wpb, tn, &nb1));
     // digestion by tile rows when by tiles would give fewer than 1536
     // wavefronts (6 per CU): measured faster below, slower above
-    const int rows = (waves * 6 < 1536) ? 1 : 0;
+    const int rows = (DigestRowsOnly<LA, LB, LC, LD>::value || waves * 6 < 1536) ? 1 : 0;
     SF_TRY_K(grid_blocks(waves * (rows ? TileRows<LA, LB, LC, LD>::total
                                        : TileGroups<LA, LB, LC, LD>::count),
                          wpb, tn, &nb2));
@@ -643,7 +652,7 @@ static int launch_mmd_t(int mode, const ClassParams& cp, const Tune& tn, hipStre
     if (mode == kModeJK) {
         unsigned nb2 = 0;
         const long long waves = (cp.nq + 63) / 64;
-        const int rows = (waves * 6 < 1536) ? 1 : 0;
+        const int rows = (DigestRowsOnly<LA, LB, LC, LD>::value || waves * 6 < 1536) ? 1 : 0;
         SF_TRY_K(grid_blocks(waves * (rows ? TileRows<LA, LB, LC, LD>::total
                                            : TileGroups<LA, LB, LC, LD>::count),
                              4, Tune(), &nb2));
