@@ -213,8 +213,9 @@ void k_eri_class(const ClassParams cp)
     NodeTable nt{};
     if constexpr (MODE != kModeJKFar) nt = stage_node_table<S::NR>(s_tab);
     const long long stride = (long long)gridDim.x * blockDim.x;
-    const long long lane0 = (long long)blockIdx.x * blockDim.x + (threadIdx.x & ~63u);
-    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x, qw = lane0; q < cp.nq;
+    const long long blk = xcd_block();
+    const long long lane0 = blk * blockDim.x + (threadIdx.x & ~63u);
+    for (long long q = blk * blockDim.x + threadIdx.x, qw = lane0; q < cp.nq;
          q += stride, qw += stride) {
         QuartetIdx ix;
         if constexpr (S::L <= 1) ix = cp.qlist ? locate_list(cp, q, qw) : locate(cp, q, qw);

@@ -142,6 +142,24 @@ struct ClassParams {
     const double* kc = nullptr;
 };
 
+// Blocks of a launch are dispatched to the 8 XCDs round robin (block p on
+// XCD p % 8), each XCD with its own L2.  xcd_block() renumbers them so that
+// runs of kXcdRun consecutive logical blocks stay on one XCD: neighbouring
+// quartets (mostly one bra's), whose digestion tiles share cache lines, are
+// written through one L2, where their partial-line stores merge, while the
+// runs still interleave over the XCDs (the cost per quartet drifts along a
+// launch).  The blocks past the last whole round of runs are split into one
+// contiguous range per XCD.  A bijection for any grid size.
+constexpr unsigned kXcd = 8, kXcdRun = 16;
+__device__ __forceinline__ unsigned xcd_block()
+{
+    const unsigned p = blockIdx.x, nb = gridDim.x, x = p % kXcd, k = p / kXcd;
+    const unsigned round = kXcd * kXcdRun, whole = nb / round * round;
+    if (p < whole) return k / kXcdRun * round + x * kXcdRun + k % kXcdRun;
+    const unsigned nt = nb - whole, q = nt / kXcd, r = nt % kXcd;
+    return whole + x * q + (x < r ? x : r) + (p - whole) / kXcd;
+}
+
 // Flat segment index q -> bra i, ket j (index in the ket list) and the
 // class-wide quartet index qa.
 struct QuartetIdx {
