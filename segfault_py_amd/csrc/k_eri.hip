@@ -360,7 +360,7 @@ __global__ __launch_bounds__(256) void k_digest_split(const ClassParams cp,
     const int per = rows ? TR::total : TG::count;
     const long long nwork = ((cp.nq + 63) >> 6) * per;
     const long long wstride = ((long long)gridDim.x * blockDim.x) >> 6;
-    for (long long gw = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6; gw < nwork;
+    for (long long gw = ((long long)xcd_block() * blockDim.x + threadIdx.x) >> 6; gw < nwork;
          gw += wstride) {
     const int w = __builtin_amdgcn_readfirstlane((int)(gw % per));
     int t = w, r = 0;

@@ -79,7 +79,7 @@ __global__ __launch_bounds__(256) void k_digest_gen(const ClassParams cp, const 
 {
     const long long nwork = ((cp.nq + 63) >> 6) * g.ntile;
     const long long wstride = ((long long)gridDim.x * blockDim.x) >> 6;
-    for (long long gw = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6; gw < nwork;
+    for (long long gw = ((long long)xcd_block() * blockDim.x + threadIdx.x) >> 6; gw < nwork;
          gw += wstride) {
         const int w = __builtin_amdgcn_readfirstlane((int)(gw % g.ntile));
         int T = 0;
