@@ -150,7 +150,7 @@ struct ClassParams {
 // runs still interleave over the XCDs (the cost per quartet drifts along a
 // launch).  The blocks past the last whole round of runs are split into one
 // contiguous range per XCD.  A bijection for any grid size.
-constexpr unsigned kXcd = 8, kXcdRun = 32;
+constexpr unsigned kXcd = 8, kXcdRun = 64;
 __device__ __forceinline__ unsigned xcd_block()
 {
     const unsigned p = blockIdx.x, nb = gridDim.x, x = p % kXcd, k = p / kXcd;
